@@ -1,0 +1,765 @@
+// chd_decoder lifecycle, chd_decode_frame / chd_decode_frames_async and
+// chd_frame accessors.
+//
+// Reference behaviour followed:
+//   src/decoders/registry.cpp:85-173,208-234   option applicability, PAL config
+//   src/abi/c_api_decoder.cpp:96-143           crop overrides and validation
+//   src/abi/c_api_decoder.cpp:183-332          per-frame window (decodeSourceFrame)
+//   src/abi/c_api_decoder.cpp:602-1049         commit
+//   src/abi/c_api_decoder.cpp:1072-1152        sync and async decode
+//   src/decoders/source_field.cpp:45-171       window loading, black padding
+//   src/output/output_writer.cpp:195-239       output geometry / padding
+//   src/abi/c_api_frame.cpp:24-238             frame accessors
+//
+// The chroma decode itself runs on the GPU through the device layer
+// (tpal_device.cpp); this file owns no arithmetic on samples.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.h"
+#include "handles.h"
+#include "tpal_device_internal.h"
+
+namespace {
+
+using chd::fail;
+
+enum class OptType { F64, I32, Bool, Str };
+
+constexpr int kMaxGpuWorkers = 8;   // host workers feeding one GPU in async decode
+constexpr int kCacheSlots = 32;     // packed fields resident per worker (window needs 14)
+
+bool is_pal_kind(chd_decoder_kind_t k)
+{
+    return k == CHD_DEC_PAL_2D || k == CHD_DEC_TRANSFORM_2D || k == CHD_DEC_TRANSFORM_3D;
+}
+bool is_comb_kind(chd_decoder_kind_t k)
+{
+    return k == CHD_DEC_NTSC_1D || k == CHD_DEC_NTSC_2D || k == CHD_DEC_NTSC_3D || k == CHD_DEC_NTSC_3D_NO_ADAPT ||
+           k == CHD_DEC_NN_TRANSFORM3D;
+}
+bool is_ldzeug_kind(chd_decoder_kind_t k)
+{
+    return k == CHD_DEC_LDZEUG_COLOR_CNN || k == CHD_DEC_LDZEUG_LUMA_SEP || k == CHD_DEC_LDZEUG_LUMA_SEP_FRAME;
+}
+
+// registry::optionApplies (registry.cpp:113-173), plus this build's
+// transform_mask / transform_mask_weights for CHD_DEC_TRANSFORM_3D.
+bool option_applies(chd_decoder_kind_t kind, const std::string &n, OptType t)
+{
+    if (t == OptType::I32 && (n == "padding_multiple" || n == "thread_count")) return true;
+    if (t == OptType::Str && (n == "output_format" || n == "output_clamp" || n == "color_difference_precision" ||
+                              n == "broadcast_scaling_precision"))
+        return true;
+    if (t == OptType::Bool && (n == "output_y4m_headers" || n == "reverse_field_order")) return true;
+    if (t == OptType::I32 && (n == "first_active_frame_line" || n == "last_active_frame_line" ||
+                              n == "first_active_sample" || n == "last_active_sample"))
+        return true;
+    const bool comb = is_comb_kind(kind) || kind == CHD_DEC_AUTO;
+    const bool pal = is_pal_kind(kind) || kind == CHD_DEC_AUTO;
+    const bool ldzeug = is_ldzeug_kind(kind);
+    const bool nn3d = kind == CHD_DEC_NN_TRANSFORM3D;
+    const bool secam = kind == CHD_DEC_SECAM;
+    const bool hvd = kind == CHD_DEC_HVD_2D || kind == CHD_DEC_HVD_3D;
+    const bool tpal = kind == CHD_DEC_TRANSFORM_2D || kind == CHD_DEC_TRANSFORM_3D;
+    if (t == OptType::F64 && n == "chroma_gain") return comb || pal || ldzeug || secam || hvd;
+    if (t == OptType::F64 && n == "chroma_phase_deg") return comb || pal || ldzeug || hvd;
+    if (t == OptType::F64 && n == "luma_nr_level") return comb || pal || kind == CHD_DEC_MONO;
+    if (t == OptType::F64 && n == "chroma_nr_level") return comb;
+    if (t == OptType::Bool && n == "phase_compensation") return comb || ldzeug;
+    if (t == OptType::F64 && n == "comb_adapt_threshold") return kind == CHD_DEC_NTSC_3D;
+    if (t == OptType::F64 && n == "comb_chroma_weight") return kind == CHD_DEC_NTSC_3D;
+    if (t == OptType::Bool && n == "comb_show_map") return kind == CHD_DEC_NTSC_3D;
+    if (t == OptType::Str && (n == "chroma_ident_mode" || n == "chroma_ident_manual")) return secam;
+    if (t == OptType::F64 && (n == "chroma_click_nr_level" || n == "chroma_click_env_dip_db" ||
+                              n == "chroma_click_freq_overshoot"))
+        return secam;
+    if (t == OptType::F64 && n == "transform_threshold") return tpal;
+    if (t == OptType::Str && n == "transform_thresholds_file") return tpal;
+    if (t == OptType::Str && (n == "transform_mask" || n == "transform_mask_weights"))
+        return kind == CHD_DEC_TRANSFORM_3D;
+    if (t == OptType::I32 && n == "hvd_cg_iterations") return hvd;
+    if (t == OptType::F64 && n == "hvd_temporal_strength") return kind == CHD_DEC_HVD_3D;
+    if (t == OptType::F64 && n == "nn_input_magnitude_scale") return nn3d;
+    if (t == OptType::Bool && n == "nn_chroma_bandpass")
+        return kind == CHD_DEC_LDZEUG_LUMA_SEP || kind == CHD_DEC_LDZEUG_LUMA_SEP_FRAME;
+    return false;
+}
+
+std::string hip_err(hipError_t e) { return std::string(hipGetErrorName(e)) + " (" + hipGetErrorString(e) + ")"; }
+
+// One worker: its own plan, stream, device field cache and staging, the
+// equivalent of the reference's per-worker decoder instance (handles.h:136-148).
+struct Worker {
+    std::mutex mu;
+    chd_tpal_plan *plan = nullptr;
+    hipStream_t stream = nullptr;
+    uint16_t *d_slots = nullptr;
+    std::vector<int64_t> slot_key;      // field number * 2 + shifted, -1 = empty
+    std::vector<uint64_t> slot_tick;
+    uint64_t tick = 0;
+    uint16_t *h_stage = nullptr;        // pinned: 14 fields
+    void *d_out = nullptr;
+    int device = 0;
+
+    ~Worker()
+    {
+        if (plan) {
+            hipSetDevice(device);
+            if (stream) (void)hipStreamSynchronize(stream);
+            chd_tpal_plan_free(plan);
+        }
+        if (d_slots) (void)hipFree(d_slots);
+        if (d_out) (void)hipFree(d_out);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace
+
+struct chd_decoder {
+    chd_video_t *video = nullptr;
+    chd_decoder_kind_t kind = CHD_DEC_AUTO;
+    std::unordered_map<std::string, double> f64;
+    std::unordered_map<std::string, int32_t> i32;
+    std::unordered_map<std::string, bool> boolean;
+    std::unordered_map<std::string, std::string> str;
+    bool committed = false;
+
+    // committed state
+    chd_decoder_kind_t resolved = CHD_DEC_AUTO;
+    bool reverseFieldOrder = false;
+    chd::VideoParams vp;
+    chd_pixel_format_t format = CHD_PIXEL_YUV444P16;
+    int32_t outW = 0, outH = 0, padL = 0, padT = 0, planes = 3;
+    int32_t lookBehind = 0, lookAhead = 0;
+    int32_t threadCount = 1;
+    chd_tpal_params_t params{};
+    std::vector<double> thresholds;
+    std::mutex workersMu;
+    std::vector<std::unique_ptr<Worker>> workers;
+
+    bool firstFieldFirst() const { return video->meta.isFirstFieldFirst != reverseFieldOrder; }
+};
+
+namespace {
+
+template <typename T>
+chd_status_t set_opt(chd_decoder_t *d, const char *name, OptType t, T value,
+                     std::unordered_map<std::string, T> chd_decoder::*dst)
+{
+    if (d == nullptr || name == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_decoder_set_option: null argument");
+    if (d->committed) return fail(CHD_E_INVALID_ARG, "chd_decoder_set_option: decoder already committed");
+    if (!option_applies(d->kind, name, t))
+        return fail(CHD_E_INVALID_ARG,
+                    std::string("chd_decoder_set_option: option \"") + name + "\" not valid for this decoder kind");
+    (d->*dst)[name] = std::move(value);
+    return CHD_OK;
+}
+
+chd_status_t worker_init(chd_decoder_t *d, Worker &w, const char *who)
+{
+    if (w.plan != nullptr) return CHD_OK;
+    chd_tpal_params_t p = d->params;
+    p.max_batch_frames = 1;
+    chd_status_t st = tpal_plan_create_impl(&p, &w.plan, who);
+    if (st != CHD_OK) return st;
+    w.device = p.device;
+    const tpal::Geom &g = w.plan->g;
+    hipError_t e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&w.d_slots, (size_t)kCacheSlots * g.field_elems * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(&w.d_out, chd_tpal_frame_bytes(w.plan));
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&w.h_stage, (size_t)14 * g.fw * g.fh * sizeof(uint16_t), 0);
+    if (e == hipSuccess)
+        e = hipMemsetD16Async((hipDeviceptr_t)w.d_slots, (unsigned short)d->vp.black16bIre,
+                              (size_t)kCacheSlots * g.field_elems, w.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+    if (e != hipSuccess) return fail(CHD_E_OOM, std::string(who) + ": device allocation failed: " + hip_err(e));
+    w.slot_key.assign(kCacheSlots, -1);
+    w.slot_tick.assign(kCacheSlots, 0);
+    return CHD_OK;
+}
+
+Worker *get_worker(chd_decoder_t *d, size_t idx)
+{
+    std::lock_guard<std::mutex> lk(d->workersMu);
+    while (d->workers.size() <= idx) d->workers.push_back(std::make_unique<Worker>());
+    return d->workers[idx].get();
+}
+
+// Make `number` (1-based) resident in the worker's field cache and return its
+// packed device pointer.  `pinned` marks slots the current window uses.
+const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool shifted, int stage_idx,
+                            std::vector<int> &pinned, chd_status_t &st)
+{
+    const tpal::Geom &g = w.plan->g;
+    const int64_t key = (int64_t)number * 2 + (shifted ? 1 : 0);
+    w.tick++;
+    for (int s = 0; s < kCacheSlots; ++s) {
+        if (w.slot_key[s] == key) {
+            w.slot_tick[s] = w.tick;
+            pinned.push_back(s);
+            return w.d_slots + (size_t)s * g.field_elems;
+        }
+    }
+    int victim = -1;
+    for (int s = 0; s < kCacheSlots; ++s) {
+        if (std::find(pinned.begin(), pinned.end(), s) != pinned.end()) continue;
+        if (victim < 0 || w.slot_tick[s] < w.slot_tick[victim]) victim = s;
+    }
+    uint16_t *stage = w.h_stage + (size_t)stage_idx * g.fw * g.fh;
+    if (!d->video->tbc.read_field(number, stage)) {
+        st = fail(CHD_E_IO, "chd_decode_frame: failed to read field " + std::to_string(number));
+        return nullptr;
+    }
+    uint16_t *dst = w.d_slots + (size_t)victim * g.field_elems + g.lmargin;
+    hipError_t e;
+    if (!shifted) {
+        e = hipMemcpy2DAsync(dst, g.pitch * 2, stage, g.fw * 2, g.fw * 2, g.fh, hipMemcpyHostToDevice, w.stream);
+    } else {
+        // SourceField::shiftSecondField (source_field.cpp:51-66): drop the
+        // first two samples of the field, append two black samples.
+        e = hipMemcpy2DAsync(dst, g.pitch * 2, stage + 2, g.fw * 2, g.fw * 2, g.fh - 1, hipMemcpyHostToDevice,
+                             w.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dst + (size_t)(g.fh - 1) * g.pitch, stage + (size_t)(g.fh - 1) * g.fw + 2,
+                               (g.fw - 2) * 2, hipMemcpyHostToDevice, w.stream);
+        if (e == hipSuccess)
+            e = hipMemsetD16Async((hipDeviceptr_t)(dst + (size_t)(g.fh - 1) * g.pitch + g.fw - 2),
+                                  (unsigned short)d->vp.black16bIre, 2, w.stream);
+    }
+    if (e != hipSuccess) {
+        st = fail(CHD_E_INTERNAL, "chd_decode_frame: field upload failed: " + hip_err(e));
+        return nullptr;
+    }
+    w.slot_key[victim] = key;
+    w.slot_tick[victim] = w.tick;
+    pinned.push_back(victim);
+    return w.d_slots + (size_t)victim * g.field_elems;
+}
+
+// decodeFrameLocked (c_api_decoder.cpp:183-332, 461-515) on worker `w`.
+chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd_frame_t **out)
+{
+    chd::Metadata &meta = d->video->meta;
+    const bool fff = d->firstFieldFirst();
+    const int32_t numFrames = meta.numberOfFrames(fff);
+    if (frame_index < 0 || frame_index >= numFrames)
+        return fail(CHD_E_OUT_OF_RANGE, "chd_decode_frame: frame_index out of range");
+    chd_status_t st = worker_init(d, w, "chd_decode_frame");
+    if (st != CHD_OK) return st;
+    hipSetDevice(w.device);
+
+    // SourceField::loadFields (source_field.cpp:68-171)
+    const int32_t behind = d->lookBehind, ahead = d->lookAhead;
+    const int32_t nwin = 2 * (behind + 1 + ahead);
+    const int32_t firstFrameNumber = (int32_t)frame_index + 1;
+    const int32_t numFields = meta.numberOfFields();
+    const int32_t numSource = d->video->tbc.available_fields;
+    const bool shiftSecond = (d->vp.system == chd::System::PAL || d->vp.system == chd::System::PAL_M) &&
+                             d->vp.isSubcarrierLocked;
+    TpalWindow win;
+    std::memset(&win, 0, sizeof(win));
+    std::vector<int> pinned;
+    int32_t frameNumber = firstFrameNumber - behind;
+    for (int32_t i = 0; i < nwin; i += 2, ++frameNumber) {
+        bool blank = frameNumber < 1 || frameNumber > numFrames;
+        int32_t mframe = blank ? 1 : frameNumber;
+        int32_t f1 = meta.fieldNumber(mframe, false, fff), f2 = meta.fieldNumber(mframe, true, fff);
+        auto metaSafe = [&](int32_t a, int32_t b) { return a >= 1 && b >= 1 && a <= numFields && b <= numFields; };
+        bool metaBad = !metaSafe(f1, f2);
+        const bool srcBad = numSource != -1 && (f1 > numSource || f2 > numSource);
+        if (!blank && (metaBad || srcBad)) {
+            chd::log_message(CHD_LOG_WARN, "chd_decode_frame: frame " + std::to_string(frameNumber) +
+                                               " is outside the readable fields; decoding it as black");
+            blank = true;
+            mframe = 1;
+            f1 = meta.fieldNumber(mframe, false, fff);
+            f2 = meta.fieldNumber(mframe, true, fff);
+            metaBad = !metaSafe(f1, f2);
+        }
+        if (!metaBad) {
+            win.offset[i] = meta.field(f1).isFirstField ? 0 : 1;
+            win.offset[i + 1] = meta.field(f2).isFirstField ? 0 : 1;
+        } else {
+            win.offset[i] = 0;
+            win.offset[i + 1] = 1;
+            blank = true;
+        }
+        if (blank) {
+            win.field[i] = nullptr;
+            win.field[i + 1] = nullptr;
+        } else {
+            win.field[i] = cache_field(d, w, f1, false, i, pinned, st);
+            if (st != CHD_OK) return st;
+            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, i + 1, pinned, st);
+            if (st != CHD_OK) return st;
+        }
+    }
+    // The decoded frame's own fields are always real: a black padding frame
+    // can only appear in the look-around.
+    win.out = w.d_out;
+    st = tpal_decode_windows(w.plan, &win, 1, w.stream);
+    if (st != CHD_OK) return st;
+
+    auto frame = std::make_unique<chd_frame>();
+    frame->format = d->format;
+    frame->width = d->outW;
+    frame->height = d->outH;
+    frame->planes = d->planes;
+    const size_t n = (size_t)d->planes * d->outW * d->outH;
+    hipError_t e;
+    if (d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS) {
+        frame->f32.resize(n);
+        e = hipMemcpyAsync(frame->f32.data(), w.d_out, n * sizeof(float), hipMemcpyDeviceToHost, w.stream);
+    } else {
+        frame->u16.resize(n);
+        e = hipMemcpyAsync(frame->u16.data(), w.d_out, n * sizeof(uint16_t), hipMemcpyDeviceToHost, w.stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+    if (e != hipSuccess) return fail(CHD_E_INTERNAL, "chd_decode_frame: GPU decode failed: " + hip_err(e));
+    frame->info.format = d->format;
+    frame->info.width = d->outW;
+    frame->info.height = d->outH;
+    frame->info.num_planes = d->planes;
+    frame->info.frame_index = frame_index;
+    *out = frame.release();
+    return CHD_OK;
+}
+
+int parse_format(const std::string &n, chd_pixel_format_t &f)
+{
+    // 1 = supported, 0 = known but not produced by this build, -1 = unknown
+    if (n == "yuv444p16" || n == "YUV444P16") { f = CHD_PIXEL_YUV444P16; return 1; }
+    if (n == "yuv444ps" || n == "YUV444PS") { f = CHD_PIXEL_YUV444PS; return 1; }
+    if (n == "gray16" || n == "GRAY16") { f = CHD_PIXEL_GRAY16; return 1; }
+    if (n == "grays" || n == "GRAYS") { f = CHD_PIXEL_GRAYS; return 1; }
+    if (n == "rgb48" || n == "RGB48") { f = CHD_PIXEL_RGB48; return 0; }
+    if (n == "rgbs" || n == "RGBS") { f = CHD_PIXEL_RGBS; return 0; }
+    if (n == "yuv440p16" || n == "YUV440P16") { f = CHD_PIXEL_YUV440P16; return 0; }
+    if (n == "yuv440ps" || n == "YUV440PS") { f = CHD_PIXEL_YUV440PS; return 0; }
+    return -1;
+}
+
+int parse_clamp(const std::string &n)
+{
+    if (n == "none" || n == "NONE") return CHD_CLAMP_NONE;
+    if (n == "legal_rgb_sdr" || n == "LEGAL_RGB_SDR") return CHD_CLAMP_LEGAL_RGB_SDR;
+    if (n == "legal_rgb_hdr" || n == "LEGAL_RGB_HDR") return CHD_CLAMP_LEGAL_RGB_HDR;
+    if (n == "legal_ycbcr_bt601" || n == "LEGAL_YCBCR_BT601") return CHD_CLAMP_LEGAL_YCBCR_BT601;
+    return -1;
+}
+
+// color_conversion.cpp:16-67
+bool color_scales(const std::string &cdp, const std::string &bsp, double &ecb, double &ecr)
+{
+    double kr, kb, ur, vr;
+    if (cdp.empty() || cdp == "modern") { kr = 0.299; kb = 0.114; }
+    else if (cdp == "classic") { kr = 0.30; kb = 0.11; }
+    else return false;
+    if (bsp.empty() || bsp == "scientific") { ur = 0.49211104112248356308804691718185; vr = 0.87728321993817866838972487283129; }
+    else if (bsp == "modern") { ur = 0.492111; vr = 0.877283; }
+    else if (bsp == "classic") { ur = 0.493; vr = 0.877; }
+    else return false;
+    ecb = 1.0 / (2.0 * (1.0 - kb) * ur);
+    ecr = 1.0 / (2.0 * (1.0 - kr) * vr);
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+chd_status_t chd_decoder_create(chd_video_t *v, chd_decoder_kind_t kind, chd_decoder_t **out)
+{
+    if (v == nullptr || out == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_decoder_create: null argument");
+    auto d = std::make_unique<chd_decoder>();
+    d->video = v;
+    d->kind = kind;
+    *out = d.release();
+    return CHD_OK;
+}
+
+void chd_decoder_free(chd_decoder_t *d) { delete d; }
+
+chd_status_t chd_decoder_set_option_f64(chd_decoder_t *d, const char *name, double v)
+{
+    return set_opt(d, name, OptType::F64, v, &chd_decoder::f64);
+}
+chd_status_t chd_decoder_set_option_i32(chd_decoder_t *d, const char *name, int32_t v)
+{
+    return set_opt(d, name, OptType::I32, v, &chd_decoder::i32);
+}
+chd_status_t chd_decoder_set_option_bool(chd_decoder_t *d, const char *name, int v)
+{
+    return set_opt<bool>(d, name, OptType::Bool, v != 0, &chd_decoder::boolean);
+}
+chd_status_t chd_decoder_set_option_str(chd_decoder_t *d, const char *name, const char *v)
+{
+    if (v == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_decoder_set_option_str: null argument (value)");
+    return set_opt<std::string>(d, name, OptType::Str, v, &chd_decoder::str);
+}
+
+chd_status_t chd_decoder_has_option(const chd_decoder_t *d, const char *name)
+{
+    if (d == nullptr || name == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_decoder_has_option: null argument");
+    const bool any = option_applies(d->kind, name, OptType::F64) || option_applies(d->kind, name, OptType::I32) ||
+                     option_applies(d->kind, name, OptType::Bool) || option_applies(d->kind, name, OptType::Str);
+    return any ? CHD_OK : CHD_E_INVALID_ARG;
+}
+
+chd_status_t chd_decoder_commit(chd_decoder_t *d)
+{
+    const std::string fn = "chd_decoder_commit";
+    if (d == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_decoder_commit: null argument");
+    if (d->committed) return CHD_OK;
+    if (d->video == nullptr) return fail(CHD_E_INVALID_ARG, fn + ": video has no source");
+    const chd::Metadata &meta = d->video->meta;
+    const chd::System system = meta.vp.system;
+
+    // registry::resolveAuto (registry.cpp:96-107)
+    chd_decoder_kind_t resolved = d->kind;
+    if (resolved == CHD_DEC_AUTO) {
+        resolved = (system == chd::System::NTSC) ? CHD_DEC_NTSC_2D
+                 : (system == chd::System::SECAM) ? CHD_DEC_SECAM : CHD_DEC_PAL_2D;
+    }
+
+    int32_t padding = 1;
+    if (auto it = d->i32.find("padding_multiple"); it != d->i32.end()) {
+        padding = it->second;
+        if (padding < 1 || padding > 1024)
+            return fail(CHD_E_INVALID_ARG, fn + ": padding_multiple must be between 1 (no padding) and 1024");
+    }
+    bool y4m = false;
+    if (auto it = d->boolean.find("output_y4m_headers"); it != d->boolean.end()) y4m = it->second;
+
+    chd_pixel_format_t fmt = CHD_PIXEL_YUV444P16;
+    int fmt_ok = 1;
+    if (auto it = d->str.find("output_format"); it != d->str.end()) {
+        fmt_ok = parse_format(it->second, fmt);
+        if (fmt_ok < 0) return fail(CHD_E_INVALID_ARG, fn + ": unknown output_format \"" + it->second + "\"");
+    }
+    int clamp = CHD_CLAMP_NONE;
+    if (auto it = d->str.find("output_clamp"); it != d->str.end()) {
+        clamp = parse_clamp(it->second);
+        if (clamp < 0) return fail(CHD_E_INVALID_ARG, fn + ": unknown output_clamp \"" + it->second + "\"");
+    }
+    std::string cdp, bsp;
+    if (auto it = d->str.find("color_difference_precision"); it != d->str.end()) {
+        cdp = it->second;
+        if (cdp != "classic" && cdp != "modern")
+            return fail(CHD_E_INVALID_ARG, fn + ": unknown color_difference_precision \"" + cdp + "\"");
+    }
+    if (auto it = d->str.find("broadcast_scaling_precision"); it != d->str.end()) {
+        bsp = it->second;
+        if (bsp != "classic" && bsp != "modern" && bsp != "scientific")
+            return fail(CHD_E_INVALID_ARG, fn + ": unknown broadcast_scaling_precision \"" + bsp + "\"");
+    }
+    {
+        const bool is440 = fmt == CHD_PIXEL_YUV440P16 || fmt == CHD_PIXEL_YUV440PS;
+        const bool lumaOnly = fmt == CHD_PIXEL_GRAY16 || fmt == CHD_PIXEL_GRAYS;
+        if (system == chd::System::SECAM && !is440 && !lumaOnly)
+            return fail(CHD_E_INVALID_ARG, fn + ": SECAM sources decode to 4:4:0 chroma or luma-only");
+        if (is440 && system != chd::System::SECAM)
+            return fail(CHD_E_INVALID_ARG, fn + ": 4:4:0 output is produced only by line-sequential (SECAM) decodes");
+        if (is440 && y4m) return fail(CHD_E_INVALID_ARG, fn + ": output_y4m_headers does not support 4:4:0 output");
+    }
+
+    // crop overrides and validation (c_api_decoder.cpp:96-143, 829-837)
+    chd::VideoParams vp = meta.vp;
+    if (auto it = d->i32.find("first_active_frame_line"); it != d->i32.end()) vp.firstActiveFrameLine = it->second;
+    if (auto it = d->i32.find("last_active_frame_line"); it != d->i32.end()) vp.lastActiveFrameLine = it->second;
+    if (auto it = d->i32.find("first_active_sample"); it != d->i32.end()) vp.activeVideoStart = it->second;
+    if (auto it = d->i32.find("last_active_sample"); it != d->i32.end()) vp.activeVideoEnd = it->second + 1;
+    if (vp.activeVideoStart < 0 || vp.activeVideoEnd > vp.fieldWidth || vp.activeVideoEnd <= vp.activeVideoStart)
+        return fail(CHD_E_INVALID_ARG, fn + ": active sample crop must satisfy 0 <= first_active_sample "
+                                            "<= last_active_sample < field_width");
+    if (vp.firstActiveFrameLine < 0 || vp.lastActiveFrameLine >= (vp.fieldHeight * 2) - 1 ||
+        vp.lastActiveFrameLine < vp.firstActiveFrameLine)
+        return fail(CHD_E_INVALID_ARG, fn + ": active frame-line crop must satisfy 0 <= first_active_frame_line "
+                                            "<= last_active_frame_line < (2 * field_height) - 1");
+    bool reverse = false;
+    if (auto it = d->boolean.find("reverse_field_order"); it != d->boolean.end()) reverse = it->second;
+
+    // OutputWriter::updateConfiguration (output_writer.cpp:195-239)
+    const int32_t aw = vp.activeVideoEnd - vp.activeVideoStart;
+    const int32_t ah = vp.lastActiveFrameLine - vp.firstActiveFrameLine + 1;
+    int32_t ow = aw, oh = ah, padL = 0, padR = 0, padT = 0, padB = 0;
+    if (padding > 1) {
+        while (ow % padding != 0) {
+            if (ow % 2 == 0) padR++; else padL++;
+            ow = padL + aw + padR;
+        }
+        while (oh % padding != 0) {
+            if (oh % 2 == 0) padB++; else padT++;
+            oh = padT + ah + padB;
+        }
+    }
+
+    d->reverseFieldOrder = reverse;
+    d->vp = vp;
+    d->format = fmt;
+    d->outW = ow;
+    d->outH = oh;
+    d->padL = padL;
+    d->padT = padT;
+    d->planes = (fmt == CHD_PIXEL_GRAY16 || fmt == CHD_PIXEL_GRAYS) ? 1 : 3;
+
+    if (d->kind == CHD_DEC_NONE) {
+        d->resolved = CHD_DEC_NONE;
+        d->lookBehind = d->lookAhead = 0;
+        d->threadCount = 0;
+        d->committed = true;
+        return CHD_OK;
+    }
+
+    if (resolved != CHD_DEC_TRANSFORM_2D && resolved != CHD_DEC_TRANSFORM_3D)
+        return fail(CHD_E_DECODER_UNKNOWN,
+                    fn + ": unknown or unsupported decoder kind (this build decodes CHD_DEC_TRANSFORM_2D/3D)");
+    if (fmt_ok == 0)
+        return fail(CHD_E_UNSUPPORTED, fn + ": output_format not produced by this build "
+                                            "(yuv444p16, yuv444ps, gray16, grays are)");
+    // PalDecoder::configure (pal_decoder.cpp:38-51)
+    if (system != chd::System::PAL && system != chd::System::PAL_M)
+        return fail(CHD_E_DECODER_INCOMPATIBLE, fn + ": This decoder is for PAL video sources only");
+
+    int32_t threads = 0;
+    if (auto it = d->i32.find("thread_count"); it != d->i32.end()) threads = it->second;
+    if (threads <= 0) {
+        const unsigned hc = std::thread::hardware_concurrency();
+        threads = hc == 0 ? 2 : (int32_t)hc;
+    }
+
+    double luma_nr = 0.0;
+    if (auto it = d->f64.find("luma_nr_level"); it != d->f64.end()) luma_nr = it->second;
+    if (luma_nr > 0.0) return fail(CHD_E_UNSUPPORTED, fn + ": luma_nr_level > 0 is not supported by this build");
+    std::string mask = "threshold";
+    if (auto it = d->str.find("transform_mask"); it != d->str.end()) mask = it->second;
+    if (mask != "threshold" && mask != "learned")
+        return fail(CHD_E_INVALID_ARG, fn + ": unknown transform_mask \"" + mask + "\"");
+    if (mask == "learned")
+        return fail(CHD_E_UNSUPPORTED, fn + ": transform_mask \"learned\" is not available in this build");
+
+    // PalColour::Configuration + TransformPal::updateConfiguration
+    double threshold = 0.4;
+    if (auto it = d->f64.find("transform_threshold"); it != d->f64.end()) threshold = it->second;
+    std::vector<double> thr;
+    if (auto it = d->str.find("transform_thresholds_file"); it != d->str.end() && !it->second.empty()) {
+        // registry.cpp:85-92: a file that cannot be opened leaves the
+        // thresholds uniform; values are whitespace separated.
+        std::ifstream in(it->second);
+        if (in.is_open()) {
+            double v;
+            while (in >> v) thr.push_back(v);
+        }
+        const size_t want = (resolved == CHD_DEC_TRANSFORM_3D) ? 768 : 80;
+        if (!thr.empty() && thr.size() != want)
+            return fail(CHD_E_INVALID_ARG, fn + ": transform_thresholds_file holds " + std::to_string(thr.size()) +
+                                               " values, the decoder needs " + std::to_string(want));
+    }
+    double ecb = 0.0, ecr = 0.0;
+    color_scales(cdp, bsp, ecb, ecr);
+
+    chd_tpal_params_t &p = d->params;
+    std::memset(&p, 0, sizeof(p));
+    p.kind = resolved;
+    p.field_width = vp.fieldWidth;
+    p.field_height = vp.fieldHeight;
+    p.active_video_start = vp.activeVideoStart;
+    p.active_video_end = vp.activeVideoEnd;
+    p.first_active_frame_line = vp.firstActiveFrameLine;
+    p.last_active_frame_line = vp.lastActiveFrameLine;
+    p.colour_burst_start = vp.colourBurstStart;
+    p.colour_burst_end = vp.colourBurstEnd;
+    p.black_16b_ire = vp.black16bIre;
+    p.white_16b_ire = vp.white16bIre;
+    p.fsc_hz = vp.fSC;
+    p.sample_rate_hz = vp.sampleRate;
+    p.threshold = threshold;
+    d->thresholds = thr;
+    p.thresholds = d->thresholds.empty() ? nullptr : d->thresholds.data();
+    p.chroma_gain = 1.0;
+    p.chroma_phase_deg = 0.0;
+    if (auto it = d->f64.find("chroma_gain"); it != d->f64.end()) p.chroma_gain = it->second;
+    if (auto it = d->f64.find("chroma_phase_deg"); it != d->f64.end()) p.chroma_phase_deg = it->second;
+    p.ecb_from_u = ecb;
+    p.ecr_from_v = ecr;
+    p.pixel_format = fmt;
+    p.clamp = clamp;
+    p.out_width = ow;
+    p.out_height = oh;
+    p.pad_left = padL;
+    p.pad_top = padT;
+    p.max_batch_frames = 1;
+    p.device = 0;
+
+    d->resolved = resolved;
+    d->lookBehind = (resolved == CHD_DEC_TRANSFORM_3D) ? 2 : 0;  // (HALFZTILE + 1) / 2
+    d->lookAhead = (resolved == CHD_DEC_TRANSFORM_3D) ? 4 : 0;   // (ZTILE - 1 + 1) / 2
+    d->threadCount = threads;
+
+    // Build worker 0 now so a missing GPU fails here, not at first decode.
+    Worker *w0 = get_worker(d, 0);
+    chd_status_t st = worker_init(d, *w0, "chd_decoder_commit");
+    if (st != CHD_OK) {
+        std::lock_guard<std::mutex> lk(d->workersMu);
+        d->workers.clear();
+        return st;
+    }
+    d->committed = true;
+    return CHD_OK;
+}
+
+chd_status_t chd_decoder_get_output_info(const chd_decoder_t *d, chd_output_info_t *out)
+{
+    if (d == nullptr || out == nullptr)
+        return fail(CHD_E_INVALID_ARG, "chromadec: chd_decoder_get_output_info: null argument");
+    if (!d->committed) return fail(CHD_E_INVALID_ARG, "chd_decoder_get_output_info: chd_decoder_commit not called");
+    out->format = d->format;
+    out->width = d->outW;
+    out->height = d->outH;
+    out->num_planes = d->planes;
+    out->num_frames = d->video->meta.numberOfFrames(d->firstFieldFirst());
+    return CHD_OK;
+}
+
+chd_status_t chd_decode_frame(chd_decoder_t *d, int64_t frame_index, chd_frame_t **out)
+{
+    if (d == nullptr || out == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_decode_frame: null argument");
+    *out = nullptr;
+    if (!d->committed) return fail(CHD_E_INVALID_ARG, "chd_decode_frame: chd_decoder_commit not called");
+    if (d->resolved == CHD_DEC_NONE)
+        return fail(CHD_E_DECODER_INCOMPATIBLE, "chd_decode_frame: decoder kind is CHD_DEC_NONE (geometry/dropout only)");
+    Worker *w = get_worker(d, 0);
+    std::lock_guard<std::mutex> lk(w->mu);
+    return decode_locked(d, *w, frame_index, out);
+}
+
+chd_status_t chd_decode_frames_async(chd_decoder_t *d, const int64_t *indices, size_t n, chd_frame_done_cb cb,
+                                      void *user, chd_cancel_t *cancel)
+{
+    if (d == nullptr || cb == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_decode_frames_async: null argument");
+    if (n > 0 && indices == nullptr)
+        return fail(CHD_E_INVALID_ARG, "chromadec: chd_decode_frames_async: null argument (indices)");
+    if (!d->committed) return fail(CHD_E_INVALID_ARG, "chd_decode_frames_async: chd_decoder_commit not called");
+    if (d->resolved == CHD_DEC_NONE)
+        return fail(CHD_E_DECODER_INCOMPATIBLE,
+                    "chd_decode_frames_async: decoder kind is CHD_DEC_NONE (geometry/dropout only)");
+    if (n == 0) return CHD_OK;
+    const size_t W = std::min<size_t>({n, (size_t)std::max(1, d->threadCount), (size_t)kMaxGpuWorkers});
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> threads;
+    threads.reserve(W);
+    for (size_t wi = 0; wi < W; ++wi) {
+        Worker *w = get_worker(d, wi);
+        threads.emplace_back([d, w, indices, n, cb, user, cancel, &next]() {
+            for (;;) {
+                const size_t i = next.fetch_add(1, std::memory_order_relaxed);
+                if (i >= n) return;
+                const int64_t idx = indices[i];
+                if (cancel != nullptr && cancel->requested.load(std::memory_order_acquire)) {
+                    cb(user, CHD_E_CANCELLED, idx, nullptr);
+                    continue;
+                }
+                chd_frame_t *frame = nullptr;
+                chd_status_t rc;
+                {
+                    std::lock_guard<std::mutex> lk(w->mu);
+                    rc = decode_locked(d, *w, idx, &frame);
+                }
+                cb(user, rc, idx, frame);
+            }
+        });
+    }
+    for (auto &t : threads) t.join();
+    return CHD_OK;
+}
+
+// ---- frames (c_api_frame.cpp) ----------------------------------------------
+
+chd_status_t chd_frame_get_info(const chd_frame_t *f, chd_frame_info_t *out)
+{
+    if (f == nullptr || out == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_frame_get_info: null argument");
+    *out = f->info;
+    return CHD_OK;
+}
+
+chd_status_t chd_frame_get_plane_info(const chd_frame_t *f, chd_plane_t p, chd_plane_info_t *out)
+{
+    if (f == nullptr || out == nullptr) return fail(CHD_E_INVALID_ARG, "chromadec: chd_frame_get_plane_info: null argument");
+    bool valid = false;
+    switch (f->format) {
+    case CHD_PIXEL_YUV444P16:
+    case CHD_PIXEL_YUV444PS:
+        valid = p == CHD_PLANE_Y || p == CHD_PLANE_CB || p == CHD_PLANE_CR;
+        break;
+    case CHD_PIXEL_GRAY16:
+    case CHD_PIXEL_GRAYS: valid = p == CHD_PLANE_Y; break;
+    default: break;
+    }
+    if (!valid) return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_info: plane not valid for the frame's pixel format");
+    out->width = f->width;
+    out->height = f->height;
+    out->first_frame_row = 0;
+    return CHD_OK;
+}
+
+chd_status_t chd_frame_get_plane(const chd_frame_t *f, chd_plane_t p, const void **out_data, ptrdiff_t *out_stride)
+{
+    if (f == nullptr || out_data == nullptr || out_stride == nullptr)
+        return fail(CHD_E_INVALID_ARG, "chromadec: chd_frame_get_plane: null argument");
+    *out_data = nullptr;
+    *out_stride = 0;
+    const size_t ps = (size_t)f->width * f->height;
+    switch (f->format) {
+    case CHD_PIXEL_YUV444P16:
+        if (p != CHD_PLANE_Y && p != CHD_PLANE_CB && p != CHD_PLANE_CR)
+            return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: plane not valid for YUV444P16");
+        *out_data = f->u16.data() + ps * (size_t)p;
+        *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(uint16_t);
+        return CHD_OK;
+    case CHD_PIXEL_GRAY16:
+        if (p != CHD_PLANE_Y) return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: GRAY16 only has CHD_PLANE_Y");
+        *out_data = f->u16.data();
+        *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(uint16_t);
+        return CHD_OK;
+    case CHD_PIXEL_YUV444PS:
+    case CHD_PIXEL_GRAYS:
+        return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: use chd_frame_get_plane_float for float pixel formats");
+    default: break;
+    }
+    return fail(CHD_E_INTERNAL, "chd_frame_get_plane: unknown pixel format");
+}
+
+chd_status_t chd_frame_get_plane_float(const chd_frame_t *f, chd_plane_t p, const float **out_data,
+                                        ptrdiff_t *out_stride)
+{
+    if (f == nullptr || out_data == nullptr || out_stride == nullptr)
+        return fail(CHD_E_INVALID_ARG, "chromadec: chd_frame_get_plane_float: null argument");
+    if (f->format != CHD_PIXEL_YUV444PS && f->format != CHD_PIXEL_GRAYS)
+        return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: frame is not a float pixel format");
+    if (f->format == CHD_PIXEL_GRAYS && p != CHD_PLANE_Y)
+        return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: CHD_PIXEL_GRAYS only has CHD_PLANE_Y");
+    if (p != CHD_PLANE_Y && p != CHD_PLANE_CB && p != CHD_PLANE_CR)
+        return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: Y'CbCr float frame plane must be Y, Cb, or Cr");
+    *out_data = f->f32.data() + (size_t)f->width * f->height * (size_t)p;
+    *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(float);
+    return CHD_OK;
+}
+
+void chd_frame_free(chd_frame_t *f) { delete f; }
+
+}  // extern "C"

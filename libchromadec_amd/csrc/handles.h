@@ -1,0 +1,88 @@
+// ABI handle definitions and the host-side source model.
+//
+// Reference counterparts: src/abi/handles.h:49-250 (handles),
+// src/metadata/core.h:56-118 (VideoParameters), src/metadata/core.cpp:
+// 1092-1160 (frame pairing), src/reader/tbc_source.cpp (field reads).
+#pragma once
+
+#include <chromadec/chromadec.h>
+#include <chromadec/tpal_device.h>
+
+#include <atomic>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace chd {
+
+enum class System { PAL = 0, NTSC = 1, PAL_M = 2, SECAM = 3 };
+
+struct VideoParams {
+    int32_t numberOfSequentialFields = -1;
+    System system = System::NTSC;
+    bool isSubcarrierLocked = false;
+    bool isWidescreen = false;
+    int32_t colourBurstStart = -1, colourBurstEnd = -1;
+    int32_t activeVideoStart = -1, activeVideoEnd = -1;  // half-open
+    int32_t white16bIre = -1, black16bIre = -1, blanking16bIre = -1;
+    int32_t fieldWidth = -1, fieldHeight = -1;
+    double sampleRate = -1.0;
+    double fSC = -1.0;
+    int32_t firstActiveFrameLine = -1, lastActiveFrameLine = -1;  // inclusive
+};
+
+struct DropOuts {
+    std::vector<int32_t> startx, endx, fieldLine;  // fieldLine is 1-based
+    bool empty() const { return startx.empty(); }
+};
+
+struct FieldMeta {
+    bool isFirstField = false;
+    DropOuts dropOuts;
+};
+
+struct Metadata {
+    VideoParams vp;
+    std::vector<FieldMeta> fields;
+    bool isFirstFieldFirst = true;
+
+    int32_t numberOfFields() const { return (int32_t)fields.size(); }
+    const FieldMeta &field(int32_t number) const;          // 1-based; blank record if out of range
+    int32_t numberOfFrames(bool firstFieldFirst) const;     // core.cpp:1092-1107
+    int32_t fieldNumber(int32_t frame, bool second, bool firstFieldFirst) const;  // core.cpp:1117-1170
+};
+
+// Reads fields of a headerless little-endian uint16 .tbc file.
+struct TbcFile {
+    int fd = -1;
+    int64_t field_samples = 0;
+    int32_t available_fields = 0;
+    ~TbcFile();
+    bool open(const std::string &path, int64_t samples_per_field, std::string &err);
+    bool read_field(int32_t number /*1-based*/, uint16_t *dst) const;
+};
+
+chd_status_t read_ld_json(const std::string &path, Metadata &meta, std::string &err);
+
+}  // namespace chd
+
+struct chd_video {
+    std::string path;
+    chd::Metadata meta;
+    chd::TbcFile tbc;
+};
+
+struct chd_cancel {
+    std::atomic<bool> requested{false};
+};
+
+struct chd_frame {
+    chd_frame_info_t info{};
+    chd_pixel_format_t format = CHD_PIXEL_YUV444P16;
+    int32_t width = 0, height = 0, planes = 0;
+    std::vector<float> f32;      // float formats: planes x height x width
+    std::vector<uint16_t> u16;   // integer formats
+};

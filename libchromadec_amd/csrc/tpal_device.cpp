@@ -1,0 +1,559 @@
+// Device-resident Transform-PAL decode: plans, tables, job lists, launches.
+//
+// Job construction keeps the reference's per-frame Z lattice
+// (TransformPal3D::filterFields at transform_pal_3d.cpp:151 anchors its Z
+// tiles at the start of the window chd_decode_frame loads, i.e. at absolute
+// fields 2f-4 and 2f for frame f).  A slab of 8 fields starting at field 2k
+// therefore serves frame k (its planes z = 0,1) and frame k+2 (z = 4,5); a
+// run of F frames needs F+2 slabs instead of the 2F the reference computes.
+#include <chromadec/decoder.h>
+#include <chromadec/tpal_device.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "tpal_device_internal.h"
+#include "tpal_internal.h"
+
+using tpal::Geom;
+
+namespace {
+
+constexpr int kMaxRunsPerChunk = 16;
+
+std::string hip_msg(const char *what, hipError_t e)
+{
+    return std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+}
+
+double window(int n, int N) { return 0.5 - 0.5 * std::cos((2.0 * M_PI * (n + 0.5)) / N); }
+
+tpal::Tables make_tables()
+{
+    tpal::Tables t;
+    std::memset(&t, 0, sizeof(t));
+    for (int z = 0; z < 8; ++z) { t.wz_d[z] = window(z, 8); t.wz[z] = (float)t.wz_d[z]; }
+    for (int y = 0; y < 32; ++y) { t.wy_d[y] = window(y, 32); t.wy[y] = (float)t.wy_d[y]; }
+    for (int x = 0; x < 16; ++x) { t.wx_d[x] = window(x, 16); t.wx[x] = (float)t.wx_d[x]; }
+    for (int y = 0; y < 16; ++y) { t.wy2_d[y] = window(y, 16); t.wy2[y] = (float)t.wy2_d[y]; }
+    for (int x = 0; x < 32; ++x) { t.wx2_d[x] = window(x, 32); t.wx2[x] = (float)t.wx2_d[x]; }
+    for (int j = 0; j < 32; ++j) {
+        t.cos32_d[j] = std::cos(2.0 * M_PI * j / 32.0);
+        t.sin32_d[j] = std::sin(2.0 * M_PI * j / 32.0);
+    }
+    for (int k = 0; k < 5; ++k)
+        for (int n = 0; n < 16; ++n) {
+            const double a = 2.0 * M_PI * (k + 2) * n / 16.0;
+            t.dx_re[k][n] = (float)(t.wx_d[n] * std::cos(a));
+            t.dx_im[k][n] = (float)(-t.wx_d[n] * std::sin(a));
+            t.ix_re[k][n] = (float)(2.0 / 4096.0 * std::cos(a));
+            t.ix_im[k][n] = (float)(-2.0 / 4096.0 * std::sin(a));
+        }
+    for (int k = 0; k < 9; ++k)
+        for (int n = 0; n < 32; ++n) {
+            const double a = 2.0 * M_PI * (k + 4) * n / 32.0;
+            t.dx2_re[k][n] = (float)(t.wx2_d[n] * std::cos(a));
+            t.dx2_im[k][n] = (float)(-t.wx2_d[n] * std::sin(a));
+            t.ix2_re[k][n] = (float)(2.0 / 512.0 * std::cos(a));
+            t.ix2_im[k][n] = (float)(-2.0 / 512.0 * std::sin(a));
+        }
+    return t;
+}
+
+std::mutex g_tables_mu;
+std::vector<bool> g_tables_ready;
+
+chd_status_t ensure_tables(int device)
+{
+    std::lock_guard<std::mutex> lk(g_tables_mu);
+    if ((int)g_tables_ready.size() <= device) g_tables_ready.resize(device + 1, false);
+    if (g_tables_ready[device]) return CHD_OK;
+    const tpal::Tables t = make_tables();
+    const hipError_t e = tpal::upload_tables(t);
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_plan_create: table upload", e));
+    g_tables_ready[device] = true;
+    return CHD_OK;
+}
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **out, const char *who)
+{
+    const std::string W = who;
+    if (p == nullptr || out == nullptr) return chd::fail(CHD_E_INVALID_ARG, W + ": null argument");
+    if (p->kind != CHD_DEC_TRANSFORM_2D && p->kind != CHD_DEC_TRANSFORM_3D)
+        return chd::fail(CHD_E_DECODER_UNKNOWN, W + ": kind must be CHD_DEC_TRANSFORM_2D or CHD_DEC_TRANSFORM_3D");
+    if (p->pixel_format != CHD_PIXEL_YUV444PS && p->pixel_format != CHD_PIXEL_YUV444P16 &&
+        p->pixel_format != CHD_PIXEL_GRAYS && p->pixel_format != CHD_PIXEL_GRAY16)
+        return chd::fail(CHD_E_UNSUPPORTED, W + ": pixel format not supported by the GPU decoder");
+    if (p->field_width <= 0 || p->field_height <= 0 || p->active_video_start < 0 ||
+        p->active_video_end > p->field_width || p->active_video_end <= p->active_video_start ||
+        p->first_active_frame_line < 0 || p->last_active_frame_line < p->first_active_frame_line ||
+        p->last_active_frame_line >= 2 * p->field_height - 1 || p->colour_burst_start < 0 ||
+        p->colour_burst_end > p->field_width || p->colour_burst_end <= p->colour_burst_start ||
+        p->sample_rate_hz <= 0.0 || p->white_16b_ire <= p->black_16b_ire)
+        return chd::fail(CHD_E_INVALID_ARG, W + ": invalid video geometry");
+
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return chd::fail(CHD_E_UNSUPPORTED, W + ": no HIP device available (the Transform PAL decoder runs on the GPU)");
+    if (p->device < 0 || p->device >= ndev) return chd::fail(CHD_E_INVALID_ARG, W + ": device ordinal out of range");
+    e = hipSetDevice(p->device);
+    if (e != hipSuccess) return chd::fail(CHD_E_UNSUPPORTED, hip_msg((W + ": hipSetDevice").c_str(), e));
+
+    auto plan = new chd_tpal_plan();
+    plan->kind = p->kind;
+    plan->device = p->device;
+    Geom &g = plan->g;
+    std::memset(&g, 0, sizeof(g));
+    g.fw = p->field_width;
+    g.fh = p->field_height;
+    g.avs = p->active_video_start;
+    g.ave = p->active_video_end;
+    g.aw = g.ave - g.avs;
+    g.first = p->first_active_frame_line;
+    g.last = p->last_active_frame_line;
+    g.cbs = p->colour_burst_start;
+    g.cbe = p->colour_burst_end;
+    g.black = (float)p->black_16b_ire;
+    // packed field layout: tile origins (avs - 8 + 8k, avs - 16 + 16k) 8-aligned
+    g.lmargin = 16 + ((8 - (g.avs % 8)) % 8);
+    g.pitch = ((g.lmargin + g.fw + 32) + 63) / 64 * 64;
+    g.field_elems = (int64_t)g.fh * g.pitch;
+    g.cpitch = ((g.aw + 3 * tpal::kColOrigin) + 31) / 32 * 32;
+    g.part_plane = (int64_t)g.fh * g.cpitch;
+    g.n_ty3 = (g.last - (g.first - tpal::kYT / 2)) / (tpal::kYT / 2) + 1;
+    g.n_tx3 = ceil_div(g.aw + tpal::kXT / 2, tpal::kXT / 2);
+    g.n_strips3 = ceil_div(g.n_tx3, tpal::kStrip3);
+    g.n_tx2 = ceil_div(g.aw + tpal::kXT2 / 2, tpal::kXT2 / 2);
+    g.n_strips2 = ceil_div(g.n_tx2, tpal::kStrip2);
+    plan->n_ty2_max = 0;
+    plan->n_lblocks = 0;
+    for (int off = 0; off < 2; ++off) {
+        const int firstFL = (g.first + 1 - off) / 2;
+        const int lastFL = ((g.last - off) / 2) + 1;
+        const int nty = ceil_div(lastFL - (firstFL - tpal::kYT2 / 2), tpal::kYT2 / 2);
+        plan->n_ty2_max = std::max(plan->n_ty2_max, nty);
+        plan->n_lblocks = std::max(plan->n_lblocks, ceil_div(std::max(lastFL - firstFL, 0), tpal::kDemodLines));
+    }
+    g.burst_floor = (float)(130000.0 / 128);
+    g.out_w = p->out_width;
+    g.out_h = p->out_height;
+    g.pad_left = p->pad_left;
+    g.pad_top = p->pad_top;
+    g.pixel_format = p->pixel_format;
+    g.clamp_mode = p->clamp;
+    if (g.out_w < g.aw + g.pad_left || g.out_h < (g.last - g.first + 1) + g.pad_top || g.pad_left < 0 || g.pad_top < 0) {
+        delete plan;
+        return chd::fail(CHD_E_INVALID_ARG, W + ": output frame smaller than the active picture");
+    }
+    const double range = (double)p->white_16b_ire - (double)p->black_16b_ire;
+    g.ey_scale = (float)(1.0 / range);
+    g.ecb_scale = (float)(p->ecb_from_u / range);
+    g.ecr_scale = (float)(p->ecr_from_v / range);
+    const double theta = (p->chroma_phase_deg * M_PI) / 180.0;
+    g.chroma_gain_cos = (float)(std::cos(theta) * p->chroma_gain);
+    g.chroma_gain_sin = (float)(std::sin(theta) * p->chroma_gain);
+
+    // PalColour::buildLookUpTables (palcolour.cpp:150-277)
+    const double bw = 1100000.0 / 0.93;
+    const double ca = 0.5 * p->sample_rate_hz / bw;
+    const int fs = std::max(1, (int)ca);
+    if (fs > tpal::kMaxFilter) {
+        delete plan;
+        return chd::fail(CHD_E_UNSUPPORTED, W + ": sample rate needs a chroma filter wider than the GPU kernel supports");
+    }
+    g.filter_size = fs;
+    std::vector<float> cfilt((tpal::kMaxFilter + 1) * 4, 0.0f);
+    {
+        std::vector<double> cf((fs + 1) * 4, 0.0);
+        double cdiv = 0;
+        for (int f = 0; f <= fs; ++f) {
+            const double fc = std::min(ca, (double)f);
+            const double ff = std::min(ca, std::sqrt((double)f * f + 2 * 2));
+            const double fff = std::min(ca, std::sqrt((double)f * f + 4 * 4));
+            const double ffff = std::min(ca, std::sqrt((double)f * f + 6 * 6));
+            const int d = (f == 0) ? 2 : 1;
+            cf[f * 4 + 0] = (1 + std::cos(M_PI * fc / ca)) / d;
+            cf[f * 4 + 2] = (1 + std::cos(M_PI * ff / ca)) / d;
+            cf[f * 4 + 1] = (1 + std::cos(M_PI * fff / ca)) / d;
+            cf[f * 4 + 3] = (1 + std::cos(M_PI * ffff / ca)) / d;
+            cdiv += 2 * (cf[f * 4 + 0] + 2 * cf[f * 4 + 2] + 2 * cf[f * 4 + 1] + 2 * cf[f * 4 + 3]);
+        }
+        for (int i = 0; i < (fs + 1) * 4; ++i) cfilt[i] = (float)(cf[i] / cdiv);
+    }
+    std::vector<float> sine(g.fw), cosine(g.fw);
+    for (int i = 0; i < g.fw; ++i) {
+        const double rad = 2 * M_PI * i * p->fsc_hz / p->sample_rate_hz;
+        // PAL-M swaps the references (palcolour.cpp:158-174)
+        sine[i] = (float)(g.fh != 263 ? std::sin(rad) : std::cos(rad));
+        cosine[i] = (float)(g.fh != 263 ? std::cos(rad) : std::sin(rad));
+    }
+
+    // thresholds (transform_pal.cpp:53-77): squared, z/y/x order
+    const int nthr = (p->kind == CHD_DEC_TRANSFORM_3D) ? 768 : 80;
+    std::vector<double> thr64(nthr);
+    std::vector<float> thr32(nthr);
+    for (int i = 0; i < nthr; ++i) {
+        const double t = p->thresholds ? p->thresholds[i] : p->threshold;
+        thr64[i] = t * t;
+        thr32[i] = (float)thr64[i];
+    }
+
+    chd_status_t st = ensure_tables(p->device);
+    if (st != CHD_OK) { delete plan; return st; }
+
+    plan->max_frames = p->max_batch_frames > 0 ? p->max_batch_frames : 64;
+    // workspace units: 3D slabs (frames + 2 per run), 2D fields
+    if (p->kind == CHD_DEC_TRANSFORM_3D) {
+        plan->max_units = plan->max_frames + 2 * std::min(kMaxRunsPerChunk, plan->max_frames);
+        plan->unit_floats = 8 * (size_t)g.part_plane;
+    } else {
+        plan->max_units = 2 * plan->max_frames;
+        plan->unit_floats = 2 * (size_t)g.part_plane;
+    }
+    auto dmalloc = [&](void **ptr, size_t bytes) -> bool {
+        const hipError_t r = hipMalloc(ptr, bytes);
+        if (r != hipSuccess) { *ptr = nullptr; return false; }
+        return true;
+    };
+    bool ok = true;
+    ok = ok && dmalloc((void **)&plan->d_thr32, nthr * sizeof(float));
+    ok = ok && dmalloc((void **)&plan->d_thr64, nthr * sizeof(double));
+    ok = ok && dmalloc((void **)&plan->d_sine, g.fw * sizeof(float));
+    ok = ok && dmalloc((void **)&plan->d_cosine, g.fw * sizeof(float));
+    ok = ok && dmalloc((void **)&plan->d_cfilt, cfilt.size() * sizeof(float));
+    ok = ok && dmalloc((void **)&plan->d_partials, plan->max_units * plan->unit_floats * sizeof(float));
+    ok = ok && dmalloc((void **)&plan->d_slab_jobs, plan->max_units * sizeof(tpal::SlabJob));
+    ok = ok && dmalloc((void **)&plan->d_field_jobs, plan->max_units * sizeof(tpal::FieldJob2D));
+    ok = ok && dmalloc((void **)&plan->d_frame_jobs, plan->max_frames * sizeof(tpal::FrameJob));
+    ok = ok && hipHostMalloc((void **)&plan->h_slab_jobs, plan->max_units * sizeof(tpal::SlabJob), 0) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&plan->h_field_jobs, plan->max_units * sizeof(tpal::FieldJob2D), 0) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&plan->h_frame_jobs, plan->max_frames * sizeof(tpal::FrameJob), 0) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&plan->staging_done, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&plan->ev[i]) == hipSuccess;
+    if (!ok) {
+        chd_tpal_plan_free(plan);
+        return chd::fail(CHD_E_OOM, W + ": device allocation failed");
+    }
+    ok = ok && hipMemcpy(plan->d_thr32, thr32.data(), nthr * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(plan->d_thr64, thr64.data(), nthr * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(plan->d_sine, sine.data(), g.fw * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(plan->d_cosine, cosine.data(), g.fw * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(plan->d_cfilt, cfilt.data(), cfilt.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        chd_tpal_plan_free(plan);
+        return chd::fail(CHD_E_INTERNAL, W + ": device upload failed");
+    }
+    *out = plan;
+    return CHD_OK;
+}
+
+// Enqueue one chunk: copy job lists, launch the tile and demod kernels.
+static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hipStream_t s)
+{
+    const Geom &g = plan->g;
+    hipError_t e;
+    if (plan->kind == CHD_DEC_TRANSFORM_3D) {
+        e = hipMemcpyAsync(plan->d_slab_jobs, plan->h_slab_jobs, n_units * sizeof(tpal::SlabJob),
+                           hipMemcpyHostToDevice, s);
+    } else {
+        e = hipMemcpyAsync(plan->d_field_jobs, plan->h_field_jobs, n_units * sizeof(tpal::FieldJob2D),
+                           hipMemcpyHostToDevice, s);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(plan->d_frame_jobs, plan->h_frame_jobs, n_frames * sizeof(tpal::FrameJob),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: job upload", e));
+    const bool border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
+    if (border) {
+        for (int i = 0; i < n_frames; ++i) tpal::launch_fill_border(g, plan->h_frame_jobs[i].out, 1, s);
+    }
+    if (plan->timing) hipEventRecord(plan->ev[0], s);
+    if (plan->kind == CHD_DEC_TRANSFORM_3D)
+        tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s);
+    else
+        tpal::launch_tpal2d(g, plan->d_field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
+    if (plan->timing) hipEventRecord(plan->ev[1], s);
+    tpal::launch_demod(g, plan->d_frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
+                       plan->n_lblocks, s);
+    if (plan->timing) hipEventRecord(plan->ev[2], s);
+    e = hipGetLastError();
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: kernel launch", e));
+    if (plan->timing) {
+        // accumulate kernel time of this chunk (waits for it; timing mode only)
+        hipEventSynchronize(plan->ev[2]);
+        float t0 = 0, t1 = 0;
+        hipEventElapsedTime(&t0, plan->ev[0], plan->ev[1]);
+        hipEventElapsedTime(&t1, plan->ev[1], plan->ev[2]);
+        plan->last_tile_ms += t0;
+        plan->last_demod_ms += t1;
+        plan->n_tile_launches += 1;
+        plan->n_demod_launches += 1;
+    }
+    return CHD_OK;
+}
+
+static void wait_staging(chd_tpal_plan *plan)
+{
+    if (plan->staging_pending) {
+        hipEventSynchronize(plan->staging_done);
+        plan->staging_pending = false;
+    }
+}
+
+static int64_t frame_bytes(const chd_tpal_plan *plan)
+{
+    const Geom &g = plan->g;
+    const int planes = (g.pixel_format == CHD_PIXEL_GRAY16 || g.pixel_format == CHD_PIXEL_GRAYS) ? 1 : 3;
+    const int es = (g.pixel_format == CHD_PIXEL_YUV444PS || g.pixel_format == CHD_PIXEL_GRAYS) ? 4 : 2;
+    return (int64_t)planes * g.out_w * g.out_h * es;
+}
+
+chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n, hipStream_t s)
+{
+    // Each window is decoded on its own lattice (two slabs / two fields).
+    const Geom &g = plan->g;
+    hipSetDevice(plan->device);
+    plan->last_tile_ms = plan->last_demod_ms = 0;
+    plan->n_tile_launches = plan->n_demod_launches = 0;
+    const int per_chunk = std::max(1, std::min(plan->max_frames, plan->max_units / 2));
+    for (int base = 0; base < n; base += per_chunk) {
+        const int cnt = std::min(per_chunk, n - base);
+        wait_staging(plan);
+        for (int i = 0; i < cnt; ++i) {
+            const TpalWindow &win = w[base + i];
+            tpal::FrameJob &fj = plan->h_frame_jobs[i];
+            std::memset(&fj, 0, sizeof(fj));
+            fj.out = win.out;
+            if (plan->kind == CHD_DEC_TRANSFORM_3D) {
+                // window fields 0..13, start 4: slabs at window offsets 0 and 4
+                for (int sl = 0; sl < 2; ++sl) {
+                    tpal::SlabJob &sj = plan->h_slab_jobs[2 * i + sl];
+                    for (int z = 0; z < 8; ++z) {
+                        sj.field[z] = win.field[4 * sl + z];
+                        sj.offset[z] = win.offset[4 * sl + z];
+                    }
+                    sj.partial = plan->d_partials + (size_t)(2 * i + sl) * plan->unit_floats;
+                }
+                for (int fsel = 0; fsel < 2; ++fsel) {
+                    fj.field[fsel] = win.field[4 + fsel];
+                    fj.offset[fsel] = win.offset[4 + fsel];
+                    const float *a = plan->h_slab_jobs[2 * i].partial + (2 + fsel) * 2 * g.part_plane;
+                    const float *b = plan->h_slab_jobs[2 * i + 1].partial + fsel * 2 * g.part_plane;
+                    fj.chroma[fsel][0] = a;
+                    fj.chroma[fsel][1] = a + g.part_plane;
+                    fj.chroma[fsel][2] = b;
+                    fj.chroma[fsel][3] = b + g.part_plane;
+                }
+                fj.nchroma = 4;
+            } else {
+                for (int fsel = 0; fsel < 2; ++fsel) {
+                    tpal::FieldJob2D &jj = plan->h_field_jobs[2 * i + fsel];
+                    jj.field = win.field[fsel];
+                    jj.offset = win.offset[fsel];
+                    jj.partial = plan->d_partials + (size_t)(2 * i + fsel) * plan->unit_floats;
+                    fj.field[fsel] = win.field[fsel];
+                    fj.offset[fsel] = win.offset[fsel];
+                    fj.chroma[fsel][0] = jj.partial;
+                    fj.chroma[fsel][1] = jj.partial + g.part_plane;
+                }
+                fj.nchroma = 2;
+            }
+        }
+        chd_status_t st = run_chunk(plan, 2 * cnt, cnt, s);
+        plan->staging_pending = true;
+        if (st != CHD_OK) return st;
+    }
+    return CHD_OK;
+}
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+chd_status_t chd_tpal_plan_create(const chd_tpal_params_t *params, chd_tpal_plan_t **out)
+{
+    return tpal_plan_create_impl(params, out, "chd_tpal_plan_create");
+}
+
+void chd_tpal_plan_free(chd_tpal_plan_t *plan)
+{
+    if (plan == nullptr) return;
+    hipSetDevice(plan->device);
+    if (plan->staging_pending) hipEventSynchronize(plan->staging_done);
+    hipDeviceSynchronize();
+    hipFree(plan->d_thr32);
+    hipFree(plan->d_thr64);
+    hipFree(plan->d_sine);
+    hipFree(plan->d_cosine);
+    hipFree(plan->d_cfilt);
+    hipFree(plan->d_partials);
+    hipFree(plan->d_slab_jobs);
+    hipFree(plan->d_field_jobs);
+    hipFree(plan->d_frame_jobs);
+    if (plan->h_slab_jobs) hipHostFree(plan->h_slab_jobs);
+    if (plan->h_field_jobs) hipHostFree(plan->h_field_jobs);
+    if (plan->h_frame_jobs) hipHostFree(plan->h_frame_jobs);
+    if (plan->staging_done) hipEventDestroy(plan->staging_done);
+    for (int i = 0; i < 4; ++i)
+        if (plan->ev[i]) hipEventDestroy(plan->ev[i]);
+    delete plan;
+}
+
+size_t chd_tpal_packed_field_elems(const chd_tpal_plan_t *plan)
+{
+    return plan ? (size_t)plan->g.field_elems : 0;
+}
+
+size_t chd_tpal_frame_bytes(const chd_tpal_plan_t *plan) { return plan ? (size_t)frame_bytes(plan) : 0; }
+
+chd_status_t chd_tpal_pack_fields(chd_tpal_plan_t *plan, const uint16_t *d_src, int64_t n_fields,
+                                  uint16_t *d_packed, void *stream)
+{
+    if (plan == nullptr || (n_fields > 0 && (d_src == nullptr || d_packed == nullptr)) || n_fields < 0)
+        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_pack_fields: null argument");
+    hipSetDevice(plan->device);
+    tpal::launch_pack(plan->g, d_src, (int64_t)plan->g.fw * plan->g.fh, d_packed, (int)n_fields,
+                      (hipStream_t)stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_pack_fields", e));
+    return CHD_OK;
+}
+
+chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs, void *d_out,
+                             void *stream)
+{
+    if (plan == nullptr || (n_seqs > 0 && (seqs == nullptr || d_out == nullptr)) || n_seqs < 0)
+        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_decode: null argument");
+    for (int i = 0; i < n_seqs; ++i) {
+        const chd_tpal_sequence_t &q = seqs[i];
+        if (q.count < 0 || q.first_frame < 0 || q.first_frame + q.count > q.n_frames ||
+            (q.count > 0 && q.d_fields == nullptr))
+            return chd::fail(CHD_E_OUT_OF_RANGE, "chd_tpal_decode: frame run outside its sequence");
+    }
+    hipSetDevice(plan->device);
+    const hipStream_t s = (hipStream_t)stream;
+    const Geom &g = plan->g;
+    const int64_t fb = frame_bytes(plan);
+    const bool is3d = plan->kind == CHD_DEC_TRANSFORM_3D;
+    plan->last_tile_ms = plan->last_demod_ms = 0;
+    plan->n_tile_launches = plan->n_demod_launches = 0;
+
+    int units = 0, frames = 0, runs = 0;
+    int64_t out_index = 0;
+    auto flush = [&]() -> chd_status_t {
+        if (frames == 0) return CHD_OK;
+        chd_status_t st = run_chunk(plan, units, frames, s);
+        plan->staging_pending = true;
+        units = frames = runs = 0;
+        return st;
+    };
+    auto field_ptr = [&](const chd_tpal_sequence_t &q, int64_t fi) -> const uint16_t * {
+        return (fi >= 0 && fi < 2 * q.n_frames) ? q.d_fields + fi * g.field_elems : nullptr;
+    };
+    wait_staging(plan);
+    for (int si = 0; si < n_seqs; ++si) {
+        const chd_tpal_sequence_t &q = seqs[si];
+        int64_t f = q.first_frame;
+        const int64_t fend = q.first_frame + q.count;
+        while (f < fend) {
+            // frames this run can take in the current chunk
+            const int extra = is3d ? 2 : 0;
+            const int unit_per_frame = is3d ? 1 : 2;
+            int room_units = plan->max_units - units - extra;
+            int room = std::min(plan->max_frames - frames, room_units / unit_per_frame);
+            if (runs >= kMaxRunsPerChunk || room <= 0) {
+                if (frames == 0) return chd::fail(CHD_E_INTERNAL, "chd_tpal_decode: workspace too small");
+                chd_status_t st = flush();
+                if (st != CHD_OK) return st;
+                wait_staging(plan);
+                continue;
+            }
+            const int cnt = (int)std::min<int64_t>(room, fend - f);
+            const int unit0 = units;
+            if (is3d) {
+                // slabs k = f-2 .. f+cnt-1
+                for (int k = 0; k < cnt + 2; ++k) {
+                    tpal::SlabJob &sj = plan->h_slab_jobs[units + k];
+                    const int64_t slab = f - 2 + k;
+                    for (int z = 0; z < 8; ++z) {
+                        sj.field[z] = field_ptr(q, 2 * slab + z);
+                        sj.offset[z] = z & 1;
+                    }
+                    sj.partial = plan->d_partials + (size_t)(units + k) * plan->unit_floats;
+                }
+                units += cnt + 2;
+            } else {
+                for (int k = 0; k < 2 * cnt; ++k) {
+                    tpal::FieldJob2D &jj = plan->h_field_jobs[units + k];
+                    jj.field = field_ptr(q, 2 * f + k);
+                    jj.offset = k & 1;
+                    jj.partial = plan->d_partials + (size_t)(units + k) * plan->unit_floats;
+                }
+                units += 2 * cnt;
+            }
+            for (int i = 0; i < cnt; ++i) {
+                tpal::FrameJob &fj = plan->h_frame_jobs[frames + i];
+                std::memset(&fj, 0, sizeof(fj));
+                fj.out = (char *)d_out + (out_index + i) * fb;
+                for (int fsel = 0; fsel < 2; ++fsel) {
+                    fj.field[fsel] = field_ptr(q, 2 * (f + i) + fsel);
+                    fj.offset[fsel] = fsel;
+                    if (is3d) {
+                        // slab f-2 (its z = 4,5) then slab f (z = 0,1)
+                        const float *a = plan->h_slab_jobs[unit0 + i].partial + (2 + fsel) * 2 * g.part_plane;
+                        const float *b = plan->h_slab_jobs[unit0 + i + 2].partial + fsel * 2 * g.part_plane;
+                        fj.chroma[fsel][0] = a;
+                        fj.chroma[fsel][1] = a + g.part_plane;
+                        fj.chroma[fsel][2] = b;
+                        fj.chroma[fsel][3] = b + g.part_plane;
+                    } else {
+                        const float *a = plan->h_field_jobs[unit0 + 2 * i + fsel].partial;
+                        fj.chroma[fsel][0] = a;
+                        fj.chroma[fsel][1] = a + g.part_plane;
+                    }
+                }
+                fj.nchroma = is3d ? 4 : 2;
+            }
+            frames += cnt;
+            out_index += cnt;
+            runs += 1;
+            f += cnt;
+        }
+    }
+    return flush();
+}
+
+chd_status_t chd_tpal_set_timing(chd_tpal_plan_t *plan, int enable)
+{
+    if (plan == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_set_timing: null argument");
+    plan->timing = enable != 0;
+    return CHD_OK;
+}
+
+chd_status_t chd_tpal_last_timing(chd_tpal_plan_t *plan, double *tile_ms, double *demod_ms, int32_t *n_tile,
+                                  int32_t *n_demod)
+{
+    if (plan == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_last_timing: null argument");
+    if (tile_ms) *tile_ms = plan->last_tile_ms;
+    if (demod_ms) *demod_ms = plan->last_demod_ms;
+    if (n_tile) *n_tile = plan->n_tile_launches;
+    if (n_demod) *n_demod = plan->n_demod_launches;
+    return CHD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// Internal view of a Transform-PAL device plan, shared by the device API
+// (tpal_device.cpp) and the ABI decoder (decoder.cpp).
+#pragma once
+
+#include <chromadec/tpal_device.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "tpal_internal.h"
+
+struct chd_tpal_plan {
+    tpal::Geom g{};
+    int kind = 0;
+    int device = 0;
+    int n_ty2_max = 0;
+    int n_lblocks = 0;
+    float *d_thr32 = nullptr;
+    double *d_thr64 = nullptr;
+    float *d_sine = nullptr, *d_cosine = nullptr, *d_cfilt = nullptr;
+    int max_frames = 0;
+    int max_units = 0;          // slabs (3D) or fields (2D) per chunk
+    size_t unit_floats = 0;
+    float *d_partials = nullptr;
+    tpal::SlabJob *d_slab_jobs = nullptr;
+    tpal::FieldJob2D *d_field_jobs = nullptr;
+    tpal::FrameJob *d_frame_jobs = nullptr;
+    tpal::SlabJob *h_slab_jobs = nullptr;       // pinned staging
+    tpal::FieldJob2D *h_field_jobs = nullptr;
+    tpal::FrameJob *h_frame_jobs = nullptr;
+    hipEvent_t staging_done = nullptr;
+    bool staging_pending = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timing = false;
+    double last_tile_ms = 0, last_demod_ms = 0;
+    int n_tile_launches = 0, n_demod_launches = 0;
+};
+
+// One chd_decode_frame window: the packed fields the reference's
+// SourceField::loadFields would hand to PalDecoder::decodeFrames
+// (14 for 3D with the frame at 4,5; 2 for 2D), nullptr for black padding.
+struct TpalWindow {
+    const uint16_t *field[14];
+    int32_t offset[14];
+    void *out;
+};
+
+chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **out, const char *who);
+chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n, hipStream_t s);

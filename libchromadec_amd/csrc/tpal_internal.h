@@ -1,0 +1,106 @@
+// Host/device shared layout of the Transform-PAL GPU pipeline.
+//
+// Device memory layout (all HBM):
+//   * Packed field store: a field is field_height rows of `pitch` uint16,
+//     sample x of a row at column x + lmargin.  lmargin makes every tile
+//     origin 16-byte aligned; margins hold black so reads that run off the
+//     stored row see black.
+//   * Slab partials (3D): one slab = 8 consecutive fields starting at an
+//     even field 2k (frame k).  Its inverse transform feeds output fields
+//     z = 0,1 (frame k) and z = 4,5 (frame k+2), so a slab partial holds
+//     4 planes x 2 row classes (tile row parity) x field_height rows x
+//     `cpitch` floats, column c = x - active_video_start + kColOrigin.
+//   * Field partials (2D): 2 row classes per field, same row layout.
+//   * Output frames: planes x out_height x out_width (float or uint16).
+#pragma once
+
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+namespace tpal {
+
+constexpr int kZT = 8, kYT = 32, kXT = 16;  // 3D tile (fields, frame lines, samples)
+constexpr int kYT2 = 16, kXT2 = 32;          // 2D tile (field lines, samples)
+constexpr int kStrip3 = 8;                   // owned 3D tiles per workgroup strip
+constexpr int kStrip2 = 4;                   // owned 2D tiles per lane-group strip
+constexpr int kDemodLines = 8;               // field lines per demod workgroup
+constexpr int kMaxFilter = 15;               // max PalColour filterSize supported
+constexpr int kColOrigin = 16;               // partial column of sample active_video_start
+
+struct Geom {
+    int32_t fw, fh;          // field width / height
+    int32_t avs, ave, aw;    // active samples [avs, ave), aw = ave - avs
+    int32_t first, last;     // active frame lines, inclusive
+    int32_t cbs, cbe;        // colour burst [cbs, cbe)
+    float black;             // black16bIre
+    int32_t pitch;           // packed row pitch (uint16 elements)
+    int32_t lmargin;         // packed column of sample 0
+    int64_t field_elems;     // packed field size (elements)
+    int32_t cpitch;          // partial row pitch (floats)
+    int64_t part_plane;      // fh * cpitch
+    // 3D lattice
+    int32_t n_ty3, n_tx3, n_strips3;
+    // 2D lattice (per field parity: first line differs)
+    int32_t n_tx2, n_strips2;
+    // demod
+    int32_t filter_size;
+    float burst_floor;       // 130000 / 128
+    // output
+    int32_t out_w, out_h, pad_left, pad_top;
+    int32_t pixel_format;    // chd_pixel_format_t
+    int32_t clamp_mode;
+    float ey_scale, ecb_scale, ecr_scale;
+    float chroma_gain_cos, chroma_gain_sin;  // gain * cos/sin(phase)
+};
+
+struct SlabJob {
+    const uint16_t *field[kZT];   // nullptr = black field
+    int32_t offset[kZT];          // 0 = first field of its frame, 1 = second
+    float *partial;               // 4 * 2 * part_plane floats
+};
+
+struct FieldJob2D {
+    const uint16_t *field;        // nullptr = black
+    int32_t offset;
+    float *partial;               // 2 * part_plane floats
+};
+
+struct FrameJob {
+    const uint16_t *field[2];
+    int32_t offset[2];
+    const float *chroma[2][4];    // per field: partial planes summed in order
+    int32_t nchroma;
+    void *out;                    // planes x out_h x out_w
+};
+
+// Constant tables of the tile transforms (device __constant__ copy c_tab).
+struct Tables {
+    // 3D: windowed forward DFT of 16 samples to x bins 2..6, and the inverse
+    // c2r kernel (2/4096) * exp(+i...) for the same bins.
+    float dx_re[5][16], dx_im[5][16];
+    float ix_re[5][16], ix_im[5][16];
+    float wz[8], wy[32], wx[16];
+    // 2D: bins 4..12 of a 32-sample window, inverse scaled by 2/512.
+    float dx2_re[9][32], dx2_im[9][32];
+    float ix2_re[9][32], ix2_im[9][32];
+    float wy2[16], wx2[32];
+    // FP64 recheck tables.
+    double wz_d[8], wy_d[32], wx_d[16], wy2_d[16], wx2_d[32];
+    double cos32_d[32], sin32_d[32];
+};
+
+
+// Launchers (tpal_kernels.hip).
+hipError_t upload_tables(const Tables &t);
+void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
+                   hipStream_t s);
+void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
+                   const double *thr64, hipStream_t s);
+void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
+                  const float *cfilt, int n_lblocks, hipStream_t s);
+void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
+                 hipStream_t s);
+void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s);
+
+}  // namespace tpal

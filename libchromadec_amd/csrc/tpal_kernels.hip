@@ -1,0 +1,934 @@
+// Transform-PAL on gfx950: tile transforms, carrier-mirror mask, overlap-add,
+// PAL demodulation and output conversion.
+//
+// The 3D tile transform follows TransformPal3D::filterFields
+// (src/decoders/palcolour/transform_pal_3d.cpp:123-333): a raised-cosine
+// windowed 8x32x16 tile, forward real FFT, keep a bin and its mirror about
+// the U/V carrier when their magnitudes agree within the threshold, inverse
+// FFT, overlap-add at half-tile hops.  It is computed here in FP32 with an
+// exact reformulation that skips all work the output cannot see:
+//   * black16bIre is subtracted from every sample.  The window's spectrum
+//     occupies bins {0, +-1} per axis and the mask passes only x bins 2..6,
+//     so this changes nothing the mask lets through, while the black-filled
+//     lines of the other field become exact zeros.
+//   * Each z plane holds data on one line parity p(z) only, so its 32-point
+//     vertical transform is a 16-point transform of the data lines times a
+//     parity twiddle.  The inverse likewise folds 32 -> 16 bins per plane
+//     and yields only the lines of that plane's field.
+//   * Only x bins 2..6 are formed (direct windowed DFT of 16 samples) and
+//     only the output planes the frame lattice uses are inverted.
+// The magnitude comparison runs in FP32; a comparison whose outcome FP32
+// rounding could change (margin below a bound on the FP32 error of the two
+// bins) is decided again from FP64 bins summed directly from the samples,
+// so the keep/discard decisions match an FP64 evaluation.
+//
+// The demodulator follows PalColour::detectBurst and
+// PalColour::decodeLine<double, true> (src/decoders/palcolour/palcolour.cpp:
+// 344-409, 456-646) and the output stage OutputWriter::convertToFloat /
+// convertLine (src/output/output_writer.cpp:402-486, 670-727).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tpal_internal.h"
+
+namespace tpal {
+
+// ---------------------------------------------------------------------------
+// Small complex helper and compile-time twiddles (multiples of 2*pi/32).
+// ---------------------------------------------------------------------------
+
+struct c32 {
+    float x, y;
+};
+__device__ __forceinline__ c32 cadd(c32 a, c32 b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ c32 csub(c32 a, c32 b) { return {a.x - b.x, a.y - b.y}; }
+
+__device__ constexpr float kCosOct[9] = {
+    1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+    0.70710678118654757f, 0.55557023301960229f, 0.38268343236508984f,
+    0.19509032201612833f, 0.0f};
+
+__device__ __forceinline__ float cos32(int k)
+{
+    k &= 31;
+    return k <= 8 ? kCosOct[k] : (k <= 16 ? -kCosOct[16 - k] : (k <= 24 ? -kCosOct[k - 16] : kCosOct[32 - k]));
+}
+__device__ __forceinline__ float sin32(int k) { return cos32(k + 24); }
+
+// v * exp(SIGN * 2*pi*i * t / 32), with the quarter turns done exactly.
+template <int SIGN>
+__device__ __forceinline__ c32 twiddle(c32 b, int t)
+{
+    t &= 31;
+    if (t == 0) return b;
+    if (t == 8) return SIGN < 0 ? c32{b.y, -b.x} : c32{-b.y, b.x};
+    if (t == 16) return c32{-b.x, -b.y};
+    if (t == 24) return SIGN < 0 ? c32{-b.y, b.x} : c32{b.y, -b.x};
+    const float c = cos32(t), s = SIGN * sin32(t);
+    return c32{b.x * c - b.y * s, b.x * s + b.y * c};
+}
+
+template <int N>
+__device__ __forceinline__ void bitrev_perm(c32 (&v)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        int j = 0;
+#pragma unroll
+        for (int b = 0; (1 << b) < N; ++b)
+            if (i & (1 << b)) j |= (N >> 1) >> b;
+        if (i < j) {
+            c32 t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+    }
+}
+
+// In-register radix-2 FFT, unnormalised.  SIGN = -1 forward, +1 inverse.
+template <int N, int SIGN>
+__device__ __forceinline__ void fft(c32 (&v)[N])
+{
+    bitrev_perm(v);
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < N; i += len) {
+#pragma unroll
+            for (int k = 0; k < len / 2; ++k) {
+                const c32 a = v[i + k];
+                const c32 w = twiddle<SIGN>(v[i + k + len / 2], k * (32 / len));
+                v[i + k] = cadd(a, w);
+                v[i + k + len / 2] = csub(a, w);
+            }
+        }
+    }
+}
+
+// Constant tables, filled once per device by the host (tpal_device.cpp).
+__constant__ Tables c_tab;
+
+// Error bound factor for FP32 bins: |bin error| <= kErrK * 2^-24 * L1(tile).
+constexpr float kErrK = 64.0f;
+constexpr float kU = 5.9604645e-08f;  // 2^-24
+
+// Wave-wide reductions (64 lanes).
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Keep/discard decision with an FP32 uncertainty test.  Returns 1 keep,
+// 0 discard, -1 undecidable in FP32.  a/b are the bin values, e the bin error
+// bound, th the squared threshold (transform_pal_3d.cpp:313-327).
+__device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
+{
+    const float ma = a.x * a.x + a.y * a.y;
+    const float mb = b.x * b.x + b.y * b.y;
+    const float ra = sqrtf(ma), rb = sqrtf(mb);
+    // |m_hat - m| <= 2 r e + e^2 (plus rounding of the products themselves)
+    const float da = 2.0f * ra * e + e * e + 4.0f * kU * ma;
+    const float db = 2.0f * rb * e + e * e + 4.0f * kU * mb;
+    const float d1 = da + db * th + 4.0f * kU * (ma + mb * th);  // for ma vs mb*th
+    const float d2 = db + da * th + 4.0f * kU * (mb + ma * th);  // for mb vs ma*th
+    const float g1 = ma - mb * th, g2 = mb - ma * th;
+    if (fabsf(g1) <= d1 || fabsf(g2) <= d2) return -1;
+    return (g1 < 0.0f || g2 < 0.0f) ? 0 : 1;
+}
+
+__device__ __forceinline__ int decide64(double ar, double ai, double br, double bi, double th)
+{
+    const double ma = ar * ar + ai * ai, mb = br * br + bi * bi;
+    return (ma < mb * th || mb < ma * th) ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// 3D tile kernel.  One 64-lane workgroup = one strip of kStrip3 (+1 leading)
+// tiles of one tile row of one slab.
+// ---------------------------------------------------------------------------
+
+struct Lds3 {
+    c32 A[8][5][16];          // x-DFT rows (z, k, m); reused by the y inverse
+    c32 S[5][32][8];          // spectrum (k, ky, kz)
+    uint8_t keep[5][32][8];
+    uint16_t flag[768];
+    int nflag;
+};
+
+__device__ __forceinline__ void load16(const uint16_t *p, float black, float (&s)[16])
+{
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    const uint4 u0 = q[0], u1 = q[1];
+    const uint32_t w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s[2 * i] = (float)(w[i] & 0xFFFFu) - black;
+        s[2 * i + 1] = (float)(w[i] >> 16) - black;
+    }
+}
+
+__global__ void __launch_bounds__(64)
+tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__restrict__ thr32,
+             const double *__restrict__ thr64)
+{
+    __shared__ Lds3 L;
+    const int lane = threadIdx.x;
+    const int per_job = g.n_ty3 * g.n_strips3;
+    const int job_id = blockIdx.x / per_job;
+    const int rem = blockIdx.x - job_id * per_job;
+    const int ty = rem / g.n_strips3;
+    const int strip = rem - ty * g.n_strips3;
+    const SlabJob &job = jobs[job_id];
+
+    const int tileY = g.first - kYT / 2 + (kYT / 2) * ty;
+    const int startY = max(g.first - tileY, 0);
+    const int endY = min(g.last - tileY + 1, kYT);
+    int par[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) par[z] = (job.offset[z] - tileY) & 1;
+
+    const int tx_own0 = strip * kStrip3;
+    const int tx_end = min(tx_own0 + kStrip3, g.n_tx3);
+    const int tx_begin = max(tx_own0 - 1, 0);
+
+    // inverse-stage lane role: output plane zsel (z = 0,1,4,5), line m
+    const int zsel = lane >> 4, om = lane & 15;
+    const int oz = (zsel < 2) ? zsel : zsel + 2;
+    const int opar = (job.offset[oz] - tileY) & 1;
+    const int oy = 2 * om + opar;
+    const bool orow_ok = (oy >= startY) && (oy < endY);
+    float *orow = job.partial + (int64_t)(zsel * 2 + (ty & 1)) * g.part_plane +
+                  (int64_t)((tileY + oy) >> 1) * g.cpitch;
+    float carry[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) carry[i] = 0.0f;
+
+    for (int tx = tx_begin; tx < tx_end; ++tx) {
+        const int tileX = g.avs - kXT / 2 + (kXT / 2) * tx;
+
+        // ---- forward x: two rows per lane -> x bins 2..6 ----------------
+        float l1 = 0.0f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = lane + 64 * h;
+            const int z = r >> 4, m = r & 15;
+            const int y = 2 * m + par[z];
+            const uint16_t *fp = job.field[z];
+            float s[16];
+            if (fp != nullptr && y >= startY && y < endY) {
+                load16(fp + (int64_t)((tileY + y) >> 1) * g.pitch + (tileX + g.lmargin), g.black, s);
+            } else {
+#pragma unroll
+                for (int n = 0; n < 16; ++n) s[n] = 0.0f;
+            }
+            const float rw = c_tab.wz[z] * c_tab.wy[y];
+            float acc = 0.0f;
+#pragma unroll
+            for (int n = 0; n < 16; ++n) acc += fabsf(s[n]) * c_tab.wx[n];
+            l1 += acc * rw;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                float re = 0.0f, im = 0.0f;
+#pragma unroll
+                for (int n = 0; n < 16; ++n) {
+                    re += s[n] * c_tab.dx_re[k][n];
+                    im += s[n] * c_tab.dx_im[k][n];
+                }
+                L.A[z][k][m] = c32{re * rw, im * rw};
+            }
+        }
+        const float err = kErrK * kU * wave_sum(l1);
+        if (lane == 0) L.nflag = 0;
+        __syncthreads();
+
+        // ---- vertical: 16-point DFT of the data lines + parity twiddle ---
+        if (lane < 40) {
+            const int z = lane / 5, k = lane - 5 * (lane / 5);
+            c32 v[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) v[m] = L.A[z][k][m];
+            fft<16, -1>(v);
+            const bool odd = par[z] != 0;
+#pragma unroll
+            for (int ky = 0; ky < 16; ++ky) {
+                c32 x0 = v[ky];
+                if (odd) x0 = twiddle<-1>(x0, ky);
+                L.S[k][ky][z] = x0;
+                L.S[k][ky + 16][z] = odd ? c32{-x0.x, -x0.y} : x0;
+            }
+        }
+        __syncthreads();
+
+        // ---- temporal: 8-point DFT per (k, ky); clear the keep mask ------
+        for (int c = lane; c < 160; c += 64) {
+            const int k = c >> 5, ky = c & 31;
+            c32 v[8];
+#pragma unroll
+            for (int z = 0; z < 8; ++z) v[z] = L.S[k][ky][z];
+            fft<8, -1>(v);
+#pragma unroll
+            for (int z = 0; z < 8; ++z) L.S[k][ky][z] = v[z];
+        }
+        {
+            uint32_t *kw = reinterpret_cast<uint32_t *>(&L.keep[0][0][0]);
+            for (int i = lane; i < 1280 / 4; i += 64) kw[i] = 0u;
+        }
+        __syncthreads();
+
+        // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
+        for (int v = lane; v < 768; v += 64) {
+            const int z = v / 96, r2 = v - 96 * z;
+            const int y = r2 / 3, x = 2 + (r2 - 3 * y);
+            const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
+            if (x == xr && y == yr && z == zr) {
+                L.keep[x - 2][y][z] = 1;
+                continue;
+            }
+            const int d = decide32(L.S[x - 2][y][z], L.S[xr - 2][yr][zr], err, thr32[v]);
+            if (d > 0) {
+                L.keep[x - 2][y][z] = 1;
+                L.keep[xr - 2][yr][zr] = 1;
+            } else if (d < 0) {
+                const int slot = atomicAdd(&L.nflag, 1);
+                L.flag[slot] = (uint16_t)v;
+            }
+        }
+        __syncthreads();
+
+        // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
+        const int nflag = L.nflag;
+        if (nflag > 0) {
+            // Reload this lane's two rows (exact integers) for the direct DFT.
+            float s[2][16];
+            int zz[2], yy[2];
+            bool ok[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int r = lane + 64 * h;
+                zz[h] = r >> 4;
+                yy[h] = 2 * (r & 15) + par[zz[h]];
+                const uint16_t *fp = job.field[zz[h]];
+                ok[h] = fp != nullptr && yy[h] >= startY && yy[h] < endY;
+                if (ok[h]) {
+                    load16(fp + (int64_t)((tileY + yy[h]) >> 1) * g.pitch + (tileX + g.lmargin), g.black, s[h]);
+                } else {
+#pragma unroll
+                    for (int n = 0; n < 16; ++n) s[h][n] = 0.0f;
+                }
+            }
+            for (int f = 0; f < nflag; ++f) {
+                const int v = L.flag[f];
+                const int z = v / 96, r2 = v - 96 * z;
+                const int y = r2 / 3, x = 2 + (r2 - 3 * y);
+                const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
+                double sa[2] = {0.0, 0.0}, sb[2] = {0.0, 0.0};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!ok[h]) continue;
+                    const double rw = c_tab.wz_d[zz[h]] * c_tab.wy_d[yy[h]];
+                    for (int n = 0; n < 16; ++n) {
+                        const double xv = (double)s[h][n] * rw * c_tab.wx_d[n];
+                        const int ja = (4 * z * zz[h] + y * yy[h] + 2 * x * n) & 31;
+                        const int jb = (4 * zr * zz[h] + yr * yy[h] + 2 * xr * n) & 31;
+                        sa[0] += xv * c_tab.cos32_d[ja];
+                        sa[1] -= xv * c_tab.sin32_d[ja];
+                        sb[0] += xv * c_tab.cos32_d[jb];
+                        sb[1] -= xv * c_tab.sin32_d[jb];
+                    }
+                }
+                const double ar = wave_sum_d(sa[0]), ai = wave_sum_d(sa[1]);
+                const double br = wave_sum_d(sb[0]), bi = wave_sum_d(sb[1]);
+                if (lane == 0 && decide64(ar, ai, br, bi, thr64[v])) {
+                    L.keep[x - 2][y][z] = 1;
+                    L.keep[xr - 2][yr][zr] = 1;
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- masked inverse z: outputs z = 0,1,4,5 ------------------------
+        for (int c = lane; c < 160; c += 64) {
+            const int k = c >> 5, ky = c & 31;
+            c32 v[8];
+#pragma unroll
+            for (int z = 0; z < 8; ++z) {
+                const c32 t = L.S[k][ky][z];
+                v[z] = L.keep[k][ky][z] ? t : c32{0.0f, 0.0f};
+            }
+            fft<8, +1>(v);
+            L.S[k][ky][0] = v[0];
+            L.S[k][ky][1] = v[1];
+            L.S[k][ky][2] = v[4];
+            L.S[k][ky][3] = v[5];
+        }
+        __syncthreads();
+
+        // ---- inverse y: fold 32 -> 16 bins for the plane's line parity ----
+        if (lane < 20) {
+            const int zs = lane / 5, k = lane - 5 * (lane / 5);
+            const int z = (zs < 2) ? zs : zs + 2;
+            const bool odd = par[z] != 0;
+            c32 v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const c32 lo = L.S[k][j][zs], hi = L.S[k][j + 16][zs];
+                c32 t = odd ? csub(lo, hi) : cadd(lo, hi);
+                if (odd) t = twiddle<+1>(t, j);
+                v[j] = t;
+            }
+            fft<16, +1>(v);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) L.A[zs][k][m] = v[m];
+        }
+        __syncthreads();
+
+        // ---- inverse x (c2r on bins 2..6) and overlap-add ----------------
+        {
+            c32 C[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) C[k] = L.A[zsel][k][om];
+            float out[16];
+#pragma unroll
+            for (int n = 0; n < 16; ++n) {
+                float a = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) a += C[k].x * c_tab.ix_re[k][n] + C[k].y * c_tab.ix_im[k][n];
+                out[n] = a;
+            }
+            const bool owned = tx >= tx_own0;
+            if (owned && orow_ok) {
+                // columns tileX .. tileX+7 are complete (tiles tx-1 and tx)
+                const int c0 = tileX - g.avs + kColOrigin;  // partial column of tileX
+                float4 *dst = reinterpret_cast<float4 *>(orow + c0);
+                float v8[8];
+#pragma unroll
+                for (int n = 0; n < 8; ++n) {
+                    const int col = tileX + n;
+                    v8[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
+                }
+                dst[0] = make_float4(v8[0], v8[1], v8[2], v8[3]);
+                dst[1] = make_float4(v8[4], v8[5], v8[6], v8[7]);
+                if (tx == g.n_tx3 - 1) {
+#pragma unroll
+                    for (int n = 0; n < 8; ++n) {
+                        const int col = tileX + 8 + n;
+                        v8[n] = (col >= g.avs && col < g.ave) ? out[8 + n] : 0.0f;
+                    }
+                    dst[2] = make_float4(v8[0], v8[1], v8[2], v8[3]);
+                    dst[3] = make_float4(v8[4], v8[5], v8[6], v8[7]);
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < 8; ++n) carry[n] = out[8 + n];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2D tile kernel (TransformPal2D, transform_pal_2d.cpp:107-287).  One
+// 64-lane workgroup = 4 groups of 16 lanes; group q runs its own strip of
+// kStrip2 (+1 leading) tiles of one tile row of one field, one line per lane.
+// ---------------------------------------------------------------------------
+
+struct Lds2 {
+    c32 A[4][9][16];          // per group: (k, line) after x DFT / before x inverse
+    uint8_t keep[4][9][16];
+    uint16_t flag[4][80];
+    int nflag[4];
+};
+
+__device__ __forceinline__ void load32(const uint16_t *p, float black, float (&s)[32])
+{
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const uint4 u = q[h];
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            s[8 * h + 2 * i] = (float)(w[i] & 0xFFFFu) - black;
+            s[8 * h + 2 * i + 1] = (float)(w[i] >> 16) - black;
+        }
+    }
+}
+
+__device__ __forceinline__ float group_sum16(float v)
+{
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double group_sum16_d(double v)
+{
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(64)
+tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
+             const float *__restrict__ thr32, const double *__restrict__ thr64)
+{
+    __shared__ Lds2 L;
+    const int lane = threadIdx.x;
+    const int q = lane >> 4, ln = lane & 15;
+    const int per_job = n_ty_max * g.n_strips2 / 4 + 1;  // unused bound guard
+    (void)per_job;
+    const int groups_per_row = (g.n_strips2 + 3) / 4;
+    const int per_field = n_ty_max * groups_per_row;
+    const int job_id = blockIdx.x / per_field;
+    const int rem = blockIdx.x - job_id * per_field;
+    const int ty = rem / groups_per_row;
+    const int strip = (rem - ty * groups_per_row) * 4 + q;
+    const FieldJob2D &job = jobs[job_id];
+
+    // SourceField::getFirstActiveLine / getLastActiveLine (source_field.h:86-91)
+    const int firstFL = (g.first + 1 - job.offset) / 2;
+    const int lastFL = ((g.last - job.offset) / 2) + 1;
+    const int n_ty = (lastFL - (firstFL - kYT2 / 2) + kYT2 / 2 - 1) / (kYT2 / 2);
+    const bool row_live = ty < n_ty;
+    const int tileY = firstFL - kYT2 / 2 + (kYT2 / 2) * ty;
+    const int startY = max(firstFL - tileY, 0);
+    const int endY = min(lastFL - tileY, kYT2);
+
+    const int tx_own0 = strip * kStrip2;
+    const bool strip_live = row_live && strip < g.n_strips2;
+    const int tx_end = strip_live ? min(tx_own0 + kStrip2, g.n_tx2) : 0;
+    const int tx_begin = strip_live ? max(tx_own0 - 1, 0) : 0;
+    // all groups iterate the same number of steps (the longest strip)
+    const int steps = kStrip2 + 1;
+
+    const bool row_ok = (ln >= startY) && (ln < endY);
+    float *orow = job.partial + (int64_t)(ty & 1) * g.part_plane + (int64_t)(tileY + ln) * g.cpitch;
+    float carry[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) carry[i] = 0.0f;
+
+    for (int step = 0; step < steps; ++step) {
+        const int tx = tx_begin + step;
+        const bool live = strip_live && tx < tx_end;
+        const int tileX = g.avs - kXT2 / 2 + (kXT2 / 2) * tx;
+
+        // ---- forward x: one line per lane -> bins 4..12 -------------------
+        float s[32];
+        const bool have = live && job.field != nullptr && row_ok;
+        if (have) {
+            load32(job.field + (int64_t)(tileY + ln) * g.pitch + (tileX + g.lmargin), g.black, s);
+        } else {
+#pragma unroll
+            for (int n = 0; n < 32; ++n) s[n] = 0.0f;
+        }
+        const float rw = c_tab.wy2[ln];
+        float l1 = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 32; ++n) l1 += fabsf(s[n]) * c_tab.wx2[n];
+        const float err = kErrK * kU * group_sum16(l1 * rw);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            float re = 0.0f, im = 0.0f;
+#pragma unroll
+            for (int n = 0; n < 32; ++n) {
+                re += s[n] * c_tab.dx2_re[k][n];
+                im += s[n] * c_tab.dx2_im[k][n];
+            }
+            L.A[q][k][ln] = c32{re * rw, im * rw};
+        }
+        if (ln == 0) L.nflag[q] = 0;
+        __syncthreads();
+
+        // ---- vertical 16-point DFT: lanes 0..8 of each group ---------------
+        if (ln < 9) {
+            c32 v[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) v[m] = L.A[q][ln][m];
+            fft<16, -1>(v);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) L.A[q][ln][m] = v[m];
+        }
+        for (int i = ln; i < 9 * 16; i += 16) (&L.keep[q][0][0])[i] = 0;
+        __syncthreads();
+
+        // ---- mask: 80 visits per group (transform_pal_2d.cpp applyFilter) --
+        for (int v = ln; v < 80; v += 16) {
+            const int y = v / 5, x = 4 + (v - 5 * y);
+            const int yr = (24 - y) & 15, xr = 16 - x;
+            if (x == xr && y == yr) {
+                L.keep[q][x - 4][y] = 1;
+                continue;
+            }
+            const int d = decide32(L.A[q][x - 4][y], L.A[q][xr - 4][yr], err, thr32[v]);
+            if (d > 0) {
+                L.keep[q][x - 4][y] = 1;
+                L.keep[q][xr - 4][yr] = 1;
+            } else if (d < 0) {
+                const int slot = atomicAdd(&L.nflag[q], 1);
+                L.flag[q][slot] = (uint16_t)v;
+            }
+        }
+        __syncthreads();
+
+        // FP64 re-decision; the loop count is wave-uniform (max over groups).
+        int nmax = 0;
+#pragma unroll
+        for (int gq = 0; gq < 4; ++gq) nmax = max(nmax, L.nflag[gq]);
+        for (int f = 0; f < nmax; ++f) {
+            const bool mine = f < L.nflag[q];
+            const int v = mine ? L.flag[q][f] : 0;
+            const int y = v / 5, x = 4 + (v - 5 * y);
+            const int yr = (24 - y) & 15, xr = 16 - x;
+            double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+            if (mine && have) {
+                const double rwd = c_tab.wy2_d[ln];
+                for (int n = 0; n < 32; ++n) {
+                    const double xv = (double)s[n] * rwd * c_tab.wx2_d[n];
+                    // phase index in 32ths: ky*line*2 (16-point) + kx*n (32-point)
+                    const int ja = (2 * y * ln + x * n) & 31;
+                    const int jb = (2 * yr * ln + xr * n) & 31;
+                    a0 += xv * c_tab.cos32_d[ja];
+                    a1 -= xv * c_tab.sin32_d[ja];
+                    b0 += xv * c_tab.cos32_d[jb];
+                    b1 -= xv * c_tab.sin32_d[jb];
+                }
+            }
+            a0 = group_sum16_d(a0); a1 = group_sum16_d(a1);
+            b0 = group_sum16_d(b0); b1 = group_sum16_d(b1);
+            if (mine && ln == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
+                L.keep[q][x - 4][y] = 1;
+                L.keep[q][xr - 4][yr] = 1;
+            }
+        }
+        __syncthreads();
+
+        // ---- masked inverse vertical ----------------------------------------
+        if (ln < 9) {
+            c32 v[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) v[m] = L.keep[q][ln][m] ? L.A[q][ln][m] : c32{0.0f, 0.0f};
+            fft<16, +1>(v);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) L.A[q][ln][m] = v[m];
+        }
+        __syncthreads();
+
+        // ---- inverse x and overlap-add ----------------------------------------
+        c32 C[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) C[k] = L.A[q][k][ln];
+        float out[32];
+#pragma unroll
+        for (int n = 0; n < 32; ++n) {
+            float a = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a += C[k].x * c_tab.ix2_re[k][n] + C[k].y * c_tab.ix2_im[k][n];
+            out[n] = a;
+        }
+        if (live && tx >= tx_own0 && row_ok) {
+            const int c0 = tileX - g.avs + kColOrigin;
+            float4 *dst = reinterpret_cast<float4 *>(orow + c0);
+            float v16[16];
+#pragma unroll
+            for (int n = 0; n < 16; ++n) {
+                const int col = tileX + n;
+                v16[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) dst[i] = make_float4(v16[4 * i], v16[4 * i + 1], v16[4 * i + 2], v16[4 * i + 3]);
+            if (tx == g.n_tx2 - 1) {
+#pragma unroll
+                for (int n = 0; n < 16; ++n) {
+                    const int col = tileX + 16 + n;
+                    v16[n] = (col >= g.avs && col < g.ave) ? out[16 + n] : 0.0f;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    dst[4 + i] = make_float4(v16[4 * i], v16[4 * i + 1], v16[4 * i + 2], v16[4 * i + 3]);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 16; ++n) carry[n] = out[16 + n];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Demodulation + output.  One 256-thread workgroup = kDemodLines field lines
+// of one field of one frame.
+// ---------------------------------------------------------------------------
+
+constexpr int kDemodThreads = 256;            // = output columns per segment
+constexpr int kSegCols = kDemodThreads + 2 * 16;  // staged columns incl. halo
+
+struct LdsD {
+    float chroma[kDemodLines + 6][kSegCols];
+    float mn[8][kSegCols];
+    float bp[kDemodLines], bq[kDemodLines], vsw[kDemodLines];
+};
+
+__device__ __forceinline__ float h273_round(float x) { return copysignf(floorf(fabsf(x) + 0.5f), x); }
+
+// Grid: jobs x 2 fields x line blocks x column segments.
+__global__ void __launch_bounds__(kDemodThreads)
+pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
+          const float *__restrict__ cosine, const float *__restrict__ cfilt /* [kMaxFilter+1][4] */,
+          int n_lblocks, int n_segs)
+{
+    __shared__ LdsD L;
+    const int tid = threadIdx.x;
+    int rem = blockIdx.x;
+    const int seg = rem % n_segs; rem /= n_segs;
+    const int lb = rem % n_lblocks; rem /= n_lblocks;
+    const int fsel = rem & 1;
+    const int job_id = rem >> 1;
+    const FrameJob &job = jobs[job_id];
+    const int off = job.offset[fsel];
+    const uint16_t *comp = job.field[fsel];
+    const int firstLine = (g.first + 1 - off) / 2;
+    const int lastLine = ((g.last - off) / 2) + 1;
+    const int n0 = firstLine + lb * kDemodLines;
+    if (n0 >= lastLine) return;
+    const int nl = min(kDemodLines, lastLine - n0);
+    const int fs = g.filter_size;
+    // PalColour::decodeLine fill window (palcolour.cpp:555-556)
+    const int fillStart = max(0, g.avs - fs);
+    const int fillEnd = min(g.ave + fs + 1, g.fw);
+    const int endPos = min(g.ave, g.fw);
+    // this segment's output columns and its staged window (halo 16 >= fs)
+    const int i0 = g.avs + seg * kDemodThreads;
+    const int s0 = i0 - 16;  // staged column 0
+
+    // ---- stage chroma rows n0-3 .. n0+nl+2 (zero outside the active area) ---
+    const int nrows = nl + 6;
+    for (int idx = tid; idx < nrows * kSegCols; idx += kDemodThreads) {
+        const int r = idx / kSegCols, cc = idx - r * kSegCols;
+        const int line = n0 - 3 + r;
+        const int col = s0 + cc;
+        float v = 0.0f;
+        if (line >= firstLine && line < lastLine && col >= g.avs && col < g.ave) {
+            const int64_t o = (int64_t)line * g.cpitch + (col - g.avs + kColOrigin);
+            const int nc = job.nchroma;
+            v = job.chroma[fsel][0][o];
+            if (nc > 1) v += job.chroma[fsel][1][o];
+            if (nc > 2) v += job.chroma[fsel][2][o];
+            if (nc > 3) v += job.chroma[fsel][3][o];
+        }
+        L.chroma[r][cc] = v;
+    }
+
+    // ---- burst detection per line (palcolour.cpp:344-409) -------------------
+    {
+        const int wave = tid >> 6, lane = tid & 63;
+        const int blen = g.cbe - g.cbs;
+        for (int li = wave; li < nl; li += kDemodThreads / 64) {
+            const int number = n0 + li;
+            float bp = 0, bq = 0, bpo = 0, bqo = 0;
+            for (int i = g.cbs + lane; i < g.cbe; i += 64) {
+                auto px = [&](int ln) -> float {
+                    if (ln < 0 || ln >= g.fh) return 0.0f;  // blackLine (zeros)
+                    if (comp == nullptr) return g.black;
+                    return (float)comp[(int64_t)ln * g.pitch + i + g.lmargin];
+                };
+                const float in0 = px(number), in1 = px(number - 1), in2 = px(number + 1);
+                const float in3 = px(number - 2), in4 = px(number + 2);
+                const float a = (in0 - (in3 + in4) * 0.5f) * 0.5f;
+                const float b = (in2 - in1) * 0.5f;
+                bp += a * sine[i];
+                bq += a * cosine[i];
+                bpo += b * sine[i];
+                bqo += b * cosine[i];
+            }
+            bp = wave_sum(bp) / blen;
+            bq = wave_sum(bq) / blen;
+            bpo = wave_sum(bpo) / blen;
+            bqo = wave_sum(bqo) / blen;
+            if (lane == 0) {
+                const float vs = (((bp - bpo) * (bp - bpo) + (bq - bqo) * (bq - bqo)) < (bp * bp + bq * bq) * 2) ? 1.0f : -1.0f;
+                float lbp = (bp - bqo) * 0.5f, lbq = (bq + bpo) * 0.5f;
+                const float norm = fmaxf(sqrtf(lbp * lbp + lbq * lbq), g.burst_floor);
+                lbp /= norm;
+                lbq /= norm;
+                // gain / phase adjustment (palcolour.cpp:321-325)
+                L.bp[li] = (lbp * g.chroma_gain_cos - lbq * g.chroma_gain_sin);
+                L.bq[li] = (lbp * g.chroma_gain_sin + lbq * g.chroma_gain_cos);
+                L.vsw[li] = vs;
+            }
+        }
+    }
+    __syncthreads();
+
+    const bool is_float = (g.pixel_format == 1 /*YUV444PS*/ || g.pixel_format == 5 /*GRAYS*/);
+    const bool luma_only = (g.pixel_format == 4 /*GRAY16*/ || g.pixel_format == 5 /*GRAYS*/);
+    const int64_t plane = (int64_t)g.out_w * g.out_h;
+    float eyLo = -INFINITY, eyHi = INFINITY, ecLo = -INFINITY, ecHi = INFINITY;
+    float yLo = 0.0f, yHi = 65535.0f, cLo = 0.0f, cHi = 65535.0f;
+    if (g.clamp_mode == 1) {
+        eyLo = 0.0f; eyHi = 1.0f; ecLo = -0.5f; ecHi = 0.5f;
+        yLo = 16.0f * 256.0f; yHi = 235.0f * 256.0f; cLo = 16.0f * 256.0f; cHi = 240.0f * 256.0f;
+    } else if (g.clamp_mode == 3) {
+        eyLo = -15.0f / 219.0f; eyHi = 238.75f / 219.0f; ecLo = -127.0f / 224.0f; ecHi = 126.75f / 224.0f;
+        yLo = 256.0f; yHi = 254.75f * 256.0f; cLo = 256.0f; cHi = 254.75f * 256.0f;
+    }
+
+    const int i = i0 + tid;              // this thread's output column
+    const bool col_live = i < endPos;
+    for (int li = 0; li < nl; ++li) {
+        const int number = n0 + li;
+        const int r0 = li + 3;  // staged row of line `number`
+        // m/n products (palcolour.cpp:557-567) over the staged columns
+        for (int cc = tid; cc < kSegCols; cc += kDemodThreads) {
+            const int col = s0 + cc;
+            float S = 0.0f, Cn = 0.0f;
+            if (col >= fillStart && col < fillEnd) { S = sine[col]; Cn = cosine[col]; }
+            const float c0 = L.chroma[r0][cc];
+            const float c1 = L.chroma[r0 - 1][cc], c2 = L.chroma[r0 + 1][cc];
+            const float c3 = L.chroma[r0 - 2][cc], c4 = L.chroma[r0 + 2][cc];
+            const float c5 = L.chroma[r0 - 3][cc], c6 = L.chroma[r0 + 3][cc];
+            L.mn[0][cc] = c0 * S;
+            L.mn[2][cc] = c1 * S - c2 * S;
+            L.mn[1][cc] = -c3 * S - c4 * S;
+            L.mn[3][cc] = -c5 * S + c6 * S;
+            L.mn[4][cc] = c0 * Cn;
+            L.mn[6][cc] = c1 * Cn - c2 * Cn;
+            L.mn[5][cc] = -c3 * Cn - c4 * Cn;
+            L.mn[7][cc] = -c5 * Cn + c6 * Cn;
+        }
+        __syncthreads();
+        if (col_live) {
+            const float bp = L.bp[li], bq = L.bq[li], vsw = L.vsw[li];
+            float PU = 0, QU = 0, PV = 0, QV = 0;
+            for (int b = 0; b <= fs; ++b) {
+                // taps clamp into the populated halo (palcolour.cpp:587-588)
+                const int l = max(fillStart, i - b) - s0;
+                const int r = min(fillEnd - 1, i + b) - s0;
+                const float c0 = cfilt[b * 4 + 0], c1 = cfilt[b * 4 + 1];
+                const float c2 = cfilt[b * 4 + 2], c3 = cfilt[b * 4 + 3];
+                const float m0 = L.mn[0][r] + L.mn[0][l], m1 = L.mn[1][r] + L.mn[1][l];
+                const float m2 = L.mn[2][r] + L.mn[2][l], m3 = L.mn[3][r] + L.mn[3][l];
+                const float q0 = L.mn[4][r] + L.mn[4][l], q1 = L.mn[5][r] + L.mn[5][l];
+                const float q2 = L.mn[6][r] + L.mn[6][l], q3 = L.mn[7][r] + L.mn[7][l];
+                const float common_p = m0 * c0 + m1 * c1;
+                const float common_q = q0 * c0 + q1 * c1;
+                PU += common_p + q2 * c2 + q3 * c3;
+                QU += common_q - m2 * c2 - m3 * c3;
+                PV += common_p - q2 * c2 - q3 * c3;
+                QV += common_q + m2 * c2 + m3 * c3;
+            }
+            const float compv = (comp != nullptr) ? (float)comp[(int64_t)number * g.pitch + i + g.lmargin] : g.black;
+            const float Y = compv - L.chroma[r0][i - s0];
+            const float U = -(PU * bp + QU * bq) * 2.0f;
+            const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
+            const int frame_line = number * 2 + off;
+            const int orow = frame_line - g.first + g.pad_top;
+            const int64_t o = (int64_t)orow * g.out_w + g.pad_left + (i - g.avs);
+            const float ey = (Y - g.black) * g.ey_scale;
+            const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
+            if (is_float) {
+                float *out = reinterpret_cast<float *>(job.out);
+                out[o] = fminf(fmaxf(ey, eyLo), eyHi);
+                if (!luma_only) {
+                    out[plane + o] = fminf(fmaxf(ecb, ecLo), ecHi);
+                    out[2 * plane + o] = fminf(fmaxf(ecr, ecLo), ecHi);
+                }
+            } else {
+                uint16_t *out = reinterpret_cast<uint16_t *>(job.out);
+                out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), yLo), yHi);
+                if (!luma_only) {
+                    out[plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecb + 128.0f * 256.0f), cLo), cHi);
+                    out[2 * plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecr + 128.0f * 256.0f), cLo), cHi);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Packing and border fill.
+// ---------------------------------------------------------------------------
+
+// Copy .tbc-layout fields (fh x fw uint16, contiguous) into the packed store.
+__global__ void pack_fields(const Geom g, const uint16_t *__restrict__ src, int64_t src_field_elems,
+                            uint16_t *__restrict__ dst, int n_fields)
+{
+    const int64_t total = (int64_t)n_fields * g.fh * g.pitch;
+    const uint16_t blk = (uint16_t)g.black;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = i / g.field_elems;
+        const int64_t r = i - f * g.field_elems;
+        const int line = (int)(r / g.pitch);
+        const int col = (int)(r - (int64_t)line * g.pitch) - g.lmargin;
+        dst[i] = (col >= 0 && col < g.fw) ? src[f * src_field_elems + (int64_t)line * g.fw + col] : blk;
+    }
+}
+
+// Fill whole output frames with the black border value (only needed when
+// padding adds a border around the active picture).
+__global__ void fill_border(const Geom g, void *out, int64_t n_frames)
+{
+    const bool is_float = (g.pixel_format == 1 || g.pixel_format == 5);
+    const int planes = (g.pixel_format == 4 || g.pixel_format == 5) ? 1 : 3;
+    const int64_t plane = (int64_t)g.out_w * g.out_h;
+    const int64_t total = n_frames * planes * plane;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)((i / plane) % planes);
+        if (is_float) {
+            reinterpret_cast<float *>(out)[i] = 0.0f;
+        } else {
+            reinterpret_cast<uint16_t *>(out)[i] = (p == 0) ? (uint16_t)(16 * 256) : (uint16_t)(128 * 256);
+        }
+    }
+}
+
+}  // namespace tpal
+
+// Launch wrappers (C++ linkage, used by tpal_device.cpp).
+namespace tpal {
+
+hipError_t upload_tables(const Tables &t) { return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof(Tables)); }
+
+void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
+                   hipStream_t s)
+{
+    const int blocks = n_jobs * g.n_ty3 * g.n_strips3;
+    if (blocks > 0) hipLaunchKernelGGL(tpal3d_tiles, dim3(blocks), dim3(64), 0, s, g, jobs, thr32, thr64);
+}
+
+void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
+                   const double *thr64, hipStream_t s)
+{
+    const int groups_per_row = (g.n_strips2 + 3) / 4;
+    const int blocks = n_jobs * n_ty_max * groups_per_row;
+    if (blocks > 0)
+        hipLaunchKernelGGL(tpal2d_tiles, dim3(blocks), dim3(64), 0, s, g, jobs, n_ty_max, thr32, thr64);
+}
+
+void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
+                  const float *cfilt, int n_lblocks, hipStream_t s)
+{
+    const int n_segs = (g.aw + kDemodThreads - 1) / kDemodThreads;
+    const int blocks = n_jobs * 2 * n_lblocks * n_segs;
+    if (blocks > 0)
+        hipLaunchKernelGGL(pal_demod, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt,
+                           n_lblocks, n_segs);
+}
+
+void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
+                 hipStream_t s)
+{
+    if (n_fields > 0) hipLaunchKernelGGL(pack_fields, dim3(1024), dim3(256), 0, s, g, src, src_field_elems, dst, n_fields);
+}
+
+void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s)
+{
+    if (n_frames > 0) hipLaunchKernelGGL(fill_border, dim3(1024), dim3(256), 0, s, g, out, n_frames);
+}
+
+}  // namespace tpal
