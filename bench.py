@@ -1,0 +1,174 @@
+"""Transform-PAL 3D throughput on one MI355X.
+
+Workload (the GPU configuration the headline metric is quoted on): a 64-frame
+synthetic PAL sequence, 4fSC composite stored as 1135x313 uint16 fields,
+seeded moving Gaussian-blurred colour patches over a static bar background
+with 40 dB SNR noise; CHD_DEC_TRANSFORM_3D, threshold 0.4, yuv444ps output.
+One step decodes all 64 frames with the device API (chd_tpal_decode), fields
+already resident in HBM, output planes written to HBM.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the 3D tile
+transform), timed with HIP events on its stream; `cpu_baseline` times the
+FP64 oracle (a port of the reference algorithm) on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_FRAMES = 64
+ALGO_IN = 2 * 1135 * 313 * 2            # composite bytes per frame
+ALGO_OUT_PS = 3 * 922 * 576 * 4         # yuv444ps planes per frame
+NOMINAL_GFLOP_PER_FRAME = 1.396         # 4329 tiles x 261.6 kFLOP + demod
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def make_workload(n_frames: int, seed: int = 2) -> np.ndarray:
+    from libchromadec_amd import synth
+    cache = os.path.join("/tmp", f"chd_bench_c2_{n_frames}_{seed}.npy")
+    if os.path.exists(cache):
+        return np.load(cache)
+    scene = synth.PatchScene(seed=seed)
+    fields = synth.encode_sequence([scene.frame(f) for f in range(n_frames)],
+                                   synth.EncodeOptions(snr_db=40, rng=np.random.default_rng(seed)))
+    try:
+        np.save(cache, fields)
+    except OSError:
+        pass
+    return fields
+
+
+def cpu_baseline(fields: np.ndarray, n_sample: int):
+    from oracle import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    p = orc.pal_params()
+    frames = list(range(n_sample))
+    t0 = time.perf_counter()
+    orc.decode_sequence(p, 8, fields, frames, orc.uniform_thresholds(8), 0, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{n_sample} frames of the same 64-frame sequence, FP64 oracle "
+                      f"(reference-equivalent TransformPal3D+PalColour+OutputWriter, radix-2 FFT), "
+                      f"one frame per OpenMP thread, {dt:.1f} s wall"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=16, help="frames for the CPU baseline (0 = skip)")
+    ap.add_argument("--no-parity", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the decode shards by frame across GPUs with no "
+              "exchange (see DESIGN.md), but only --gpus 1 is run here", file=sys.stderr)
+        return 2
+
+    import torch
+    from libchromadec_amd import abi
+
+    fields = make_workload(N_FRAMES)
+    dev = torch.device("cuda:0")
+    plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, pixel_format=abi.CHD_PIXEL_YUV444PS,
+                                            max_batch_frames=N_FRAMES))
+    src = torch.from_numpy(fields.view(np.int16)).to(dev)
+    packed = torch.empty((fields.shape[0], plan.packed_field_elems), dtype=torch.int16, device=dev)
+    plan.pack(src.data_ptr(), fields.shape[0], packed.data_ptr())
+    del src
+    out = torch.empty((N_FRAMES, plan.frame_bytes // 4), dtype=torch.float32, device=dev)
+    seqs = [(packed.data_ptr(), N_FRAMES, 0, N_FRAMES)]
+    stream = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        plan.decode(seqs, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.decode(seqs, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    fps = N_FRAMES * args.steps / elapsed
+
+    # kernel durations: HIP events around each launch on the decode stream
+    plan.set_timing(True)
+    tile_ms, demod_ms, n_tile = 0.0, 0.0, 0
+    for _ in range(max(3, min(args.steps, 10))):
+        plan.decode(seqs, out.data_ptr(), stream)
+        a, b, nt, _ = plan.last_timing()
+        tile_ms += a
+        demod_ms += b
+        n_tile += nt
+    plan.set_timing(False)
+    torch.cuda.synchronize()
+    tile_avg = tile_ms / n_tile
+    demod_avg = demod_ms / n_tile
+    frames_per_launch = N_FRAMES
+    algo_bytes = (ALGO_IN + ALGO_OUT_PS) * frames_per_launch
+    achieved = algo_bytes / (tile_avg * 1e-3) / 1e9
+    step_kernel_ms = tile_avg + demod_avg
+    achieved_tflops = NOMINAL_GFLOP_PER_FRAME * frames_per_launch / (step_kernel_ms * 1e-3) / 1e3
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "tpal3d_tiles_c2_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    parity = None
+    if not args.no_parity:
+        from oracle import oracle as orc
+        want = orc.decode_frame(orc.pal_params(), 8, fields, 1, orc.uniform_thresholds(8), 0)
+        got = out[1].view(3, 576, 922).cpu().numpy()
+        parity = float(np.abs(got.astype(np.float64) - want).max())
+
+    res = {
+        "metric": "PAL frames/s, Transform-3D (plain threshold), yuv444ps",
+        "value": round(fps, 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64-frame synthetic PAL sequence (moving patches over bars, 40 dB SNR), "
+                               "1135x313 u16 fields, CHD_DEC_TRANSFORM_3D threshold 0.4, yuv444ps, "
+                               "fields resident in HBM",
+                   "frames_per_step": N_FRAMES, "tile": "8x32x16", "threshold": 0.4},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "tpal3d_tiles", "avg_launch_ms": round(tile_avg, 4),
+                     "algorithmic_bytes_per_launch": algo_bytes,
+                     "demod_kernel_avg_launch_ms": round(demod_avg, 4),
+                     "frac_whole_decode": round(algo_bytes / (step_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "valu": {"nominal_gflop_per_frame": NOMINAL_GFLOP_PER_FRAME,
+                              "achieved_tflops_nominal": round(achieved_tflops, 2),
+                              "peak_tflops": FP32_PEAK_TFLOPS,
+                              "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)}},
+        "parity_max_abs_vs_oracle_frame1": parity,
+    }
+    if args.cpu_sample > 0:
+        res["cpu_baseline"] = cpu_baseline(fields, args.cpu_sample)
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

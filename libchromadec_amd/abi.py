@@ -142,11 +142,24 @@ SIGNATURES = {
 }
 
 
+def _share_hip_runtime_with_torch():
+    """PyTorch-ROCm wheels carry their own libamdhip64, loaded by the
+    unversioned name "libamdhip64.so".  If this library (which needs
+    "libamdhip64.so.7") loaded /opt/rocm's copy first, torch would load a
+    second HIP runtime and find no GPU.  Importing torch first makes the
+    dynamic linker reuse torch's runtime (same SONAME) for this library."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load the in-tree library (raises if it has not been built)."""
     global _lib
     with _lock:
         if _lib is None:
+            _share_hip_runtime_with_torch()
             if not os.path.exists(LIB_PATH):
                 raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                                    "(or make -C libchromadec_amd)")

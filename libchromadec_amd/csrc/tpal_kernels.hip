@@ -108,8 +108,11 @@ __device__ __forceinline__ void fft(c32 (&v)[N])
 // Constant tables, filled once per device by the host (tpal_device.cpp).
 __constant__ Tables c_tab;
 
-// Error bound factor for FP32 bins: |bin error| <= kErrK * 2^-24 * L1(tile).
-constexpr float kErrK = 64.0f;
+// Error bound for FP32 bins: |bin error| <= kErrK * 2^-24 * L1(windowed tile).
+// The worst case of the 16-term x DFT plus 4 + 3 radix-2 stages and the
+// twiddles is about 25 u L1; measured errors on synthetic PAL stay below 1 u L1.
+constexpr float kErrK = 32.0f;   // 3D tile
+constexpr float kErrK2 = 48.0f;  // 2D tile: 32-term x DFT + 4 stages
 constexpr float kU = 5.9604645e-08f;  // 2^-24
 
 // Wave-wide reductions (64 lanes).
@@ -531,7 +534,7 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
         float l1 = 0.0f;
 #pragma unroll
         for (int n = 0; n < 32; ++n) l1 += fabsf(s[n]) * c_tab.wx2[n];
-        const float err = kErrK * kU * group_sum16(l1 * rw);
+        const float err = kErrK2 * kU * group_sum16(l1 * rw);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             float re = 0.0f, im = 0.0f;

@@ -1,0 +1,216 @@
+"""GPU parity: chd_decode_frame (HIP kernels) against the FP64 oracle.
+
+Tolerances: float planes within 1e-4 of full scale
+(E'Y 0..1, E'Cb/E'Cr +-0.5), yuv444p16 within 1 LSB.  Every call goes
+through the C ABI in libchromadec_amd/lib/libchromadec.so.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from libchromadec_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+F32_TOL = 1e-4
+U16_TOL = 1
+
+
+def _oracle(orc, kind, fields, frame, fmt, **geo):
+    thr = geo.pop("thr", None)
+    p = orc.pal_params(**geo)
+    t = orc.uniform_thresholds(kind) if thr is None else thr
+    return orc.decode_frame(p, kind, fields, frame, t, 0 if fmt == "yuv444ps" else 1)
+
+
+def _check(got, want, fmt):
+    d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+    tol = F32_TOL if fmt in ("yuv444ps", "grays") else U16_TOL
+    assert d.max() <= tol, f"max {d.max()} at {np.unravel_index(d.argmax(), d.shape)}"
+    return d.max()
+
+
+@pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
+def test_transform3d_full_frames(orc, patch_capture, fmt):
+    path, fields = patch_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format=fmt).commit()
+    for f in (0, 2, 5):  # black look-behind, interior, black look-ahead
+        got = dec.decode(f)
+        assert got.shape == (3, 576, 922)
+        _check(got, _oracle(orc, 8, fields, f, fmt), fmt)
+
+
+@pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
+def test_transform2d_full_frames(orc, bars_capture, fmt):
+    path, fields = bars_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_2D, output_format=fmt).commit()
+    for f in range(3):
+        _check(dec.decode(f), _oracle(orc, 7, fields, f, fmt), fmt)
+
+
+@pytest.mark.parametrize("kind", [7, 8])
+@pytest.mark.parametrize("crop", [dict(first_active_frame_line=101, last_active_frame_line=170,
+                                       first_active_sample=333, last_active_sample=470),
+                                  dict(first_active_frame_line=0, last_active_frame_line=624,
+                                       first_active_sample=0, last_active_sample=1134)])
+def test_crop_padding_clamp(orc, patch_capture, kind, crop):
+    path, fields = patch_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444ps", padding_multiple=16, output_clamp="legal_rgb_sdr",
+                      **crop).commit()
+    oi = dec.output_info()
+    got = dec.decode(3)
+    fl, ll = crop["first_active_frame_line"], crop["last_active_frame_line"]
+    fs, ls = crop["first_active_sample"], crop["last_active_sample"]
+    aw, ah = ls - fs + 1, ll - fl + 1
+    assert oi.width % 16 == 0 and oi.height % 16 == 0 and oi.width >= aw and oi.height >= ah
+    # OutputWriter::updateConfiguration padding order (output_writer.cpp:214-238)
+    pl = pt = 0
+    w, h, pr, pb = aw, ah, 0, 0
+    while w % 16:
+        if w % 2 == 0: pr += 1
+        else: pl += 1
+        w = pl + aw + pr
+    while h % 16:
+        if h % 2 == 0: pb += 1
+        else: pt += 1
+        h = pt + ah + pb
+    want = _oracle(orc, kind, fields, 3, "yuv444ps", first_line=fl, last_line=ll, avs=fs, ave=ls + 1,
+                   clamp_mode=1, pad_left=pl, pad_top=pt, out_width=w, out_height=h)
+    _check(got, want, "yuv444ps")
+
+
+@pytest.mark.parametrize("kind", [7, 8])
+@pytest.mark.parametrize("t", [0.0, 0.25, 1.5])
+def test_threshold_values(orc, patch_capture, kind, t):
+    path, fields = patch_capture
+    geo = dict(first_line=260, last_line=331, avs=520, ave=680)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444ps", transform_threshold=t,
+                      first_active_frame_line=260, last_active_frame_line=331,
+                      first_active_sample=520, last_active_sample=679).commit()
+    _check(dec.decode(2), _oracle(orc, kind, fields, 2, "yuv444ps", thr=orc.uniform_thresholds(kind, t), **geo),
+           "yuv444ps")
+
+
+@pytest.mark.parametrize("kind", [7, 8])
+def test_thresholds_file(orc, patch_capture, tmp_path, kind):
+    path, fields = patch_capture
+    n = 768 if kind == 8 else 80
+    vals = np.random.default_rng(9).uniform(0.1, 0.7, n)
+    tf = tmp_path / "thr.txt"
+    tf.write_text("\n".join(f"{x:.17g}" for x in vals))
+    geo = dict(first_line=260, last_line=331, avs=520, ave=680)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444p16", transform_thresholds_file=str(tf),
+                      first_active_frame_line=260, last_active_frame_line=331,
+                      first_active_sample=520, last_active_sample=679).commit()
+    _check(dec.decode(4), _oracle(orc, kind, fields, 4, "yuv444p16", thr=vals * vals, **geo), "yuv444p16")
+    # a missing file leaves the thresholds uniform (registry.cpp:85-92)
+    dec2 = abi.Decoder(v, kind, output_format="yuv444p16", transform_thresholds_file=str(tmp_path / "none.txt"),
+                       first_active_frame_line=260, last_active_frame_line=331,
+                       first_active_sample=520, last_active_sample=679).commit()
+    _check(dec2.decode(4), _oracle(orc, kind, fields, 4, "yuv444p16", **geo), "yuv444p16")
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0.4 0.4")
+    dec3 = abi.Decoder(v, kind, transform_thresholds_file=str(bad))
+    with pytest.raises(abi.ChdError) as e:
+        dec3.commit()
+    assert e.value.name == "CHD_E_INVALID_ARG"
+
+
+def test_gain_phase_gray(orc, patch_capture):
+    path, fields = patch_capture
+    geo = dict(first_line=200, last_line=263, avs=400, ave=528)
+    crop = dict(first_active_frame_line=200, last_active_frame_line=263, first_active_sample=400,
+                last_active_sample=527)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps", chroma_gain=1.7,
+                      chroma_phase_deg=-12.5, **crop).commit()
+    _check(dec.decode(1), _oracle(orc, 8, fields, 1, "yuv444ps", chroma_gain=1.7, chroma_phase_deg=-12.5, **geo),
+           "yuv444ps")
+    g = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="grays", **crop).commit().decode(1)
+    assert g.shape == (1, 64, 128)
+    _check(g, _oracle(orc, 8, fields, 1, "yuv444ps", **geo)[:1], "grays")
+    g16 = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_2D, output_format="gray16", **crop).commit().decode(1)
+    _check(g16, _oracle(orc, 7, fields, 1, "yuv444p16", **geo)[:1], "gray16")
+
+
+def test_out_of_range_and_reuse(patch_capture):
+    import ctypes as C
+    path, _ = patch_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps").commit()
+    f = C.c_void_p()
+    assert abi.lib().chd_decode_frame(dec.h, 6, C.byref(f)) == 13
+    assert abi.lib().chd_decode_frame(dec.h, -1, C.byref(f)) == 13
+    a = dec.decode(3)
+    dec.decode(0)
+    dec.decode(5)
+    b = dec.decode(3)  # field cache reuse and eviction leave results unchanged
+    assert np.array_equal(a, b)
+    # a second commit is a no-op; options after commit are rejected
+    dec.commit()
+    with pytest.raises(abi.ChdError):
+        dec.set_option("transform_threshold", 0.3)
+
+
+@pytest.mark.parametrize("kind", [7, 8])
+def test_async_matches_sync(patch_capture, kind):
+    path, _ = patch_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444p16", thread_count=3).commit()
+    res = dec.decode_async([5, 0, 3, 1, 4, 2, 3])
+    assert sorted(res) == [0, 1, 2, 3, 4, 5]
+    for i in range(6):
+        assert np.array_equal(res[i], dec.decode(i))
+    import ctypes as C
+    c = C.c_void_p()
+    abi.lib().chd_cancel_create(C.byref(c))
+    abi.lib().chd_cancel_request(c)
+    res2 = dec.decode_async([0, 1], cancel=c)
+    assert res2 == {0: "CHD_E_CANCELLED", 1: "CHD_E_CANCELLED"}
+    abi.lib().chd_cancel_free(c)
+
+
+@pytest.mark.parametrize("kind", [7, 8])
+def test_device_api_matches_abi_bitwise(patch_capture, kind):
+    """chd_tpal_decode shares each 8-field slab between frames f and f+2;
+    the result must be bit-identical to per-frame chd_decode_frame."""
+    import torch
+    path, fields = patch_capture
+    dev = torch.device("cuda:0")
+    plan = abi.TpalPlan(abi.pal_tpal_params(kind=kind, pixel_format=abi.CHD_PIXEL_YUV444PS, max_batch_frames=4))
+    src = torch.from_numpy(fields.view(np.int16)).to(dev)
+    packed = torch.empty((fields.shape[0], plan.packed_field_elems), dtype=torch.int16, device=dev)
+    plan.pack(src.data_ptr(), fields.shape[0], packed.data_ptr())
+    out = torch.zeros((6, 3, 576, 922), dtype=torch.float32, device=dev)
+    # two runs: frames 0..3 and 4..5 (the first run also spans two chunks of 4)
+    plan.decode([(packed.data_ptr(), 6, 0, 4), (packed.data_ptr(), 6, 4, 2)], out.data_ptr())
+    torch.cuda.synchronize()
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444ps").commit()
+    host = out.cpu().numpy()
+    for f in range(6):
+        assert np.array_equal(host[f], dec.decode(f)), f
+
+
+def test_truncated_capture_decodes_black_neighbours(orc, patch_capture, tmp_path):
+    """Fields the sidecar lists but the file lacks read as black frames
+    (source_field.cpp:104-132)."""
+    import shutil
+    path, fields = patch_capture
+    short = tmp_path / "short.tbc"
+    with open(path, "rb") as a, open(short, "wb") as b:
+        b.write(a.read(fields[0].nbytes * 10))  # 5 of 6 frames
+    shutil.copy(path + ".json", str(short) + ".json")
+    v = abi.Video(str(short))
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps").commit()
+    got = dec.decode(3)
+    f2 = fields.copy()
+    f2[10:] = 0x4000
+    f2 = f2[:12]
+    _check(got, _oracle(orc, 8, f2, 3, "yuv444ps"), "yuv444ps")
