@@ -108,11 +108,14 @@ __device__ __forceinline__ void fft(c32 (&v)[N])
 // Constant tables, filled once per device by the host (tpal_device.cpp).
 __constant__ Tables c_tab;
 
-// Error bound for FP32 bins: |bin error| <= kErrK * 2^-24 * L1(windowed tile).
-// The worst case of the 16-term x DFT plus 4 + 3 radix-2 stages and the
-// twiddles is about 25 u L1; measured errors on synthetic PAL stay below 1 u L1.
-constexpr float kErrK = 32.0f;   // 3D tile
-constexpr float kErrK2 = 48.0f;  // 2D tile: 32-term x DFT + 4 stages
+// Error bound for FP32 bins: |bin error| <= kErrK * 2^-24 * L1, with L1 the
+// sum of |windowed, black-subtracted samples| of the tile.  A worst-case
+// rounding analysis of the folded x DFT, the radix-2 stages and the twiddles
+// gives about 34 u L1 (3D) and 45 u L1 (2D); the largest error measured over
+// 384 000 bins of synthetic PAL tiles (noise, zone plates, bars) was
+// 0.69 u L1.  The bound used keeps a factor of >10 over every measured error.
+constexpr float kErrK = 8.0f;    // 3D tile
+constexpr float kErrK2 = 12.0f;  // 2D tile (32-term x DFT)
 constexpr float kU = 5.9604645e-08f;  // 2^-24
 
 // Wave-wide reductions (64 lanes).
@@ -129,20 +132,21 @@ __device__ __forceinline__ double wave_sum_d(double v)
     return v;
 }
 
-// Keep/discard decision with an FP32 uncertainty test.  Returns 1 keep,
-// 0 discard, -1 undecidable in FP32.  a/b are the bin values, e the bin error
-// bound, th the squared threshold (transform_pal_3d.cpp:313-327).
+// Keep/discard decision with an FP32 uncertainty test
+// (transform_pal_3d.cpp:313-327: discard when |a|^2 < |b|^2 t^2 or
+// |b|^2 < |a|^2 t^2).  Returns 1 keep, 0 discard, -1 when FP32 error `e` on
+// the bins could change the outcome.  |m_hat - m| <= 2|a|e + e^2 with
+// |a| <= |a.x| + |a.y|, plus the rounding of the products themselves.
 __device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
 {
     const float ma = a.x * a.x + a.y * a.y;
     const float mb = b.x * b.x + b.y * b.y;
-    const float ra = sqrtf(ma), rb = sqrtf(mb);
-    // |m_hat - m| <= 2 r e + e^2 (plus rounding of the products themselves)
-    const float da = 2.0f * ra * e + e * e + 4.0f * kU * ma;
-    const float db = 2.0f * rb * e + e * e + 4.0f * kU * mb;
-    const float d1 = da + db * th + 4.0f * kU * (ma + mb * th);  // for ma vs mb*th
-    const float d2 = db + da * th + 4.0f * kU * (mb + ma * th);  // for mb vs ma*th
+    const float da = e * (2.0f * (fabsf(a.x) + fabsf(a.y)) + e);
+    const float db = e * (2.0f * (fabsf(b.x) + fabsf(b.y)) + e);
     const float g1 = ma - mb * th, g2 = mb - ma * th;
+    const float r1 = 4.0f * kU * (ma + mb * th), r2 = 4.0f * kU * (mb + ma * th);
+    const float d1 = fmaf(db, th, da) + r1;
+    const float d2 = fmaf(da, th, db) + r2;
     if (fabsf(g1) <= d1 || fabsf(g2) <= d2) return -1;
     return (g1 < 0.0f || g2 < 0.0f) ? 0 : 1;
 }
@@ -158,13 +162,42 @@ __device__ __forceinline__ int decide64(double ar, double ai, double br, double 
 // tiles of one tile row of one slab.
 // ---------------------------------------------------------------------------
 
+// LDS layout.  Rows are padded so that the lanes of a wave, which walk
+// different rows in step, hit different banks: A rows are 17 complex
+// (136 B) apart, S columns 9 complex (72 B) apart, and each k block of S is
+// shifted by 2 complex.  A (x-DFT rows / inverse-y output) and S (spectrum)
+// are never live at the same time, so they share storage.
+constexpr int kApad = 17;
+constexpr int kSpad = 9;
+constexpr int kSk = 32 * kSpad + 2;
+
 struct Lds3 {
-    c32 A[8][5][16];          // x-DFT rows (z, k, m); reused by the y inverse
-    c32 S[5][32][8];          // spectrum (k, ky, kz)
-    uint8_t keep[5][32][8];
-    uint16_t flag[768];
-    int nflag;
+    union {
+        c32 A[40][kApad];     // forward: row z*5+k holds the 16 data lines m
+                              // inverse: row zs*5+k holds the 16 output lines
+        c32 S[5 * kSk];       // spectrum S[k*kSk + ky*kSpad + kz]
+    } u;
+    uint32_t keep[40];        // bit (k*32+ky)*8+kz: bin kept by the mask
+    uint32_t flag[24];        // bit v: visit v undecidable in FP32
+    double cs[32], sn[32];    // cos/sin(2 pi j / 32) for the FP64 re-check
 };
+
+__device__ __forceinline__ c32 &Sat(Lds3 &L, int k, int ky, int kz) { return L.u.S[k * kSk + ky * kSpad + kz]; }
+__device__ __forceinline__ void set_keep(Lds3 &L, int k, int ky, int kz)
+{
+    const int b = (k * 32 + ky) * 8 + kz;
+    atomicOr(&L.keep[b >> 5], 1u << (b & 31));
+}
+
+// acc + x * c for a coefficient known at compile time after unrolling; the
+// exact 0 / +-1 cases cost nothing.
+__device__ __forceinline__ float cmac(float acc, float x, float c)
+{
+    if (c == 0.0f) return acc;
+    if (c == 1.0f) return acc + x;
+    if (c == -1.0f) return acc - x;
+    return fmaf(x, c, acc);
+}
 
 __device__ __forceinline__ void load16(const uint16_t *p, float black, float (&s)[16])
 {
@@ -175,6 +208,87 @@ __device__ __forceinline__ void load16(const uint16_t *p, float black, float (&s
     for (int i = 0; i < 8; ++i) {
         s[2 * i] = (float)(w[i] & 0xFFFFu) - black;
         s[2 * i + 1] = (float)(w[i] >> 16) - black;
+    }
+}
+
+// Windowed 16-sample real DFT to bins 2..6 (X[j] = bin j+2), folded by the
+// half-period symmetries of the bins: even bins see u[n] + u[n+8], odd bins
+// u[n] - u[n+8]; bins 2 and 6 fold again over n+4.  Returns sum |u[n]|.
+__device__ __forceinline__ float xdft16(const float (&s)[16], c32 (&X)[5])
+{
+    float u[16], l1 = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        u[n] = s[n] * c_tab.wx[n];
+        l1 += fabsf(u[n]);
+    }
+    float e[8], o[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        e[n] = u[n] + u[n + 8];
+        o[n] = u[n] - u[n + 8];
+    }
+    // bin 4: sum e[n] (-i)^n
+    X[2] = c32{(e[0] - e[2]) + (e[4] - e[6]), (e[3] - e[1]) + (e[7] - e[5])};
+    // bins 2 and 6: g[n] = e[n] - e[n+4]
+    const float g0 = e[0] - e[4], g1 = e[1] - e[5], g2 = e[2] - e[6], g3 = e[3] - e[7];
+    const float r = 0.70710678118654752f;
+    const float a = (g1 - g3) * r, b = (g1 + g3) * r;
+    X[0] = c32{g0 + a, -(g2 + b)};
+    X[4] = c32{g0 - a, g2 - b};
+    // bins 3 and 5
+#pragma unroll
+    for (int j = 1; j <= 3; j += 2) {
+        const int k = j + 2;
+        float re = 0.0f, im = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            re = cmac(re, o[n], cos32(2 * k * n));
+            im = cmac(im, o[n], -sin32(2 * k * n));
+        }
+        X[j] = c32{re, im};
+    }
+    return l1;
+}
+
+// The same row DFT in FP64 for the re-check (bins 2..6 of the window).
+__device__ __forceinline__ void xdft16_d(const float (&s)[16], double rw, double (&re)[5], double (&im)[5])
+{
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { re[j] = 0.0; im[j] = 0.0; }
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        const double u = (double)s[n] * (rw * c_tab.wx_d[n]);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const int ph = (2 * (j + 2) * n) & 31;
+            re[j] = fma(u, c_tab.cos32_d[ph], re[j]);
+            im[j] = fma(-u, c_tab.sin32_d[ph], im[j]);
+        }
+    }
+}
+
+// c2r over bins 2..6 (C[j] = bin j+2, pre-scaled): out[n] = 2 Re sum C e^{+i..}.
+// Even bins repeat every 8 samples, odd bins change sign.
+__device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
+{
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        float E = 0.0f, O = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const int k = j + 2;
+            const float c = cos32(2 * k * n), s = sin32(2 * k * n);
+            if (k & 1) {
+                O = cmac(O, C[j].x, c);
+                O = cmac(O, C[j].y, -s);
+            } else {
+                E = cmac(E, C[j].x, c);
+                E = cmac(E, C[j].y, -s);
+            }
+        }
+        out[n] = E + O;
+        out[n + 8] = E - O;
     }
 }
 
@@ -191,6 +305,11 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
     const int strip = rem - ty * g.n_strips3;
     const SlabJob &job = jobs[job_id];
 
+    if (lane < 32) {
+        L.cs[lane] = c_tab.cos32_d[lane];
+        L.sn[lane] = c_tab.sin32_d[lane];
+    }
+
     const int tileY = g.first - kYT / 2 + (kYT / 2) * ty;
     const int startY = max(g.first - tileY, 0);
     const int endY = min(g.last - tileY + 1, kYT);
@@ -202,7 +321,25 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
     const int tx_end = min(tx_own0 + kStrip3, g.n_tx3);
     const int tx_begin = max(tx_own0 - 1, 0);
 
-    // inverse-stage lane role: output plane zsel (z = 0,1,4,5), line m
+    // forward-stage role: rows (z, m) = lane and lane + 64
+    const uint16_t *rowp[2];
+    float roww[2];
+    int rz[2], ry[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h;
+        rz[h] = r >> 4;
+        ry[h] = 2 * (r & 15) + par[rz[h]];
+        const uint16_t *fp = job.field[rz[h]];
+        const bool ok = fp != nullptr && ry[h] >= startY && ry[h] < endY;
+        rowp[h] = ok ? fp + (int64_t)((tileY + ry[h]) >> 1) * g.pitch + g.lmargin : nullptr;
+        roww[h] = c_tab.wz[rz[h]] * c_tab.wy[ry[h]];
+    }
+    // mask-stage role: bins y = lane & 31; lanes 0-31 visit x = 2 (all z) and
+    // x = 3 (z < 4), lanes 32-63 visit x = 3 (z >= 4) and x = 4 (all z)
+    const int vy = lane & 31, vhalf = lane >> 5;
+
+    // inverse-stage role: output plane zsel (z = 0,1,4,5), line om
     const int zsel = lane >> 4, om = lane & 15;
     const int oz = (zsel < 2) ? zsel : zsel + 2;
     const int opar = (job.offset[oz] - tileY) & 1;
@@ -221,137 +358,133 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
         float l1 = 0.0f;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int r = lane + 64 * h;
-            const int z = r >> 4, m = r & 15;
-            const int y = 2 * m + par[z];
-            const uint16_t *fp = job.field[z];
             float s[16];
-            if (fp != nullptr && y >= startY && y < endY) {
-                load16(fp + (int64_t)((tileY + y) >> 1) * g.pitch + (tileX + g.lmargin), g.black, s);
+            if (rowp[h] != nullptr) {
+                load16(rowp[h] + tileX, g.black, s);
             } else {
 #pragma unroll
                 for (int n = 0; n < 16; ++n) s[n] = 0.0f;
             }
-            const float rw = c_tab.wz[z] * c_tab.wy[y];
-            float acc = 0.0f;
+            c32 X[5];
+            l1 += xdft16(s, X) * roww[h];
+            const int m = (lane + 64 * h) & 15;
 #pragma unroll
-            for (int n = 0; n < 16; ++n) acc += fabsf(s[n]) * c_tab.wx[n];
-            l1 += acc * rw;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                float re = 0.0f, im = 0.0f;
-#pragma unroll
-                for (int n = 0; n < 16; ++n) {
-                    re += s[n] * c_tab.dx_re[k][n];
-                    im += s[n] * c_tab.dx_im[k][n];
-                }
-                L.A[z][k][m] = c32{re * rw, im * rw};
-            }
+            for (int j = 0; j < 5; ++j) L.u.A[rz[h] * 5 + j][m] = c32{X[j].x * roww[h], X[j].y * roww[h]};
         }
         const float err = kErrK * kU * wave_sum(l1);
-        if (lane == 0) L.nflag = 0;
+        if (lane < 40) L.keep[lane] = 0u;
+        if (lane < 24) L.flag[lane] = 0u;
         __syncthreads();
 
         // ---- vertical: 16-point DFT of the data lines + parity twiddle ---
-        if (lane < 40) {
-            const int z = lane / 5, k = lane - 5 * (lane / 5);
+        {
             c32 v[16];
+            const bool act = lane < 40;
+            if (act) {
 #pragma unroll
-            for (int m = 0; m < 16; ++m) v[m] = L.A[z][k][m];
-            fft<16, -1>(v);
-            const bool odd = par[z] != 0;
+                for (int m = 0; m < 16; ++m) v[m] = L.u.A[lane][m];
+            }
+            __syncthreads();  // A and S share storage
+            if (act) {
+                const int z = lane / 5, k = lane - 5 * (lane / 5);
+                fft<16, -1>(v);
+                const bool odd = par[z] != 0;
 #pragma unroll
-            for (int ky = 0; ky < 16; ++ky) {
-                c32 x0 = v[ky];
-                if (odd) x0 = twiddle<-1>(x0, ky);
-                L.S[k][ky][z] = x0;
-                L.S[k][ky + 16][z] = odd ? c32{-x0.x, -x0.y} : x0;
+                for (int ky = 0; ky < 16; ++ky) {
+                    c32 x0 = v[ky];
+                    if (odd) x0 = twiddle<-1>(x0, ky);
+                    Sat(L, k, ky, z) = x0;
+                    Sat(L, k, ky + 16, z) = odd ? c32{-x0.x, -x0.y} : x0;
+                }
             }
         }
         __syncthreads();
 
-        // ---- temporal: 8-point DFT per (k, ky); clear the keep mask ------
+        // ---- temporal: 8-point DFT per (k, ky) ----------------------------
         for (int c = lane; c < 160; c += 64) {
             const int k = c >> 5, ky = c & 31;
             c32 v[8];
 #pragma unroll
-            for (int z = 0; z < 8; ++z) v[z] = L.S[k][ky][z];
+            for (int z = 0; z < 8; ++z) v[z] = Sat(L, k, ky, z);
             fft<8, -1>(v);
 #pragma unroll
-            for (int z = 0; z < 8; ++z) L.S[k][ky][z] = v[z];
-        }
-        {
-            uint32_t *kw = reinterpret_cast<uint32_t *>(&L.keep[0][0][0]);
-            for (int i = lane; i < 1280 / 4; i += 64) kw[i] = 0u;
+            for (int z = 0; z < 8; ++z) Sat(L, k, ky, z) = v[z];
         }
         __syncthreads();
 
         // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
-        for (int v = lane; v < 768; v += 64) {
-            const int z = v / 96, r2 = v - 96 * z;
-            const int y = r2 / 3, x = 2 + (r2 - 3 * y);
+        // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v]
+#pragma unroll 4
+        for (int i = 0; i < 12; ++i) {
+            int x, z;
+            if (vhalf == 0) { x = (i < 8) ? 2 : 3; z = (i < 8) ? i : i - 8; }
+            else { x = (i < 4) ? 3 : 4; z = (i < 4) ? i + 4 : i - 4; }
+            const int y = vy;
             const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
+            const int v = (z * 32 + y) * 3 + (x - 2);
             if (x == xr && y == yr && z == zr) {
-                L.keep[x - 2][y][z] = 1;
+                set_keep(L, x - 2, y, z);
                 continue;
             }
-            const int d = decide32(L.S[x - 2][y][z], L.S[xr - 2][yr][zr], err, thr32[v]);
+            const int d = decide32(Sat(L, x - 2, y, z), Sat(L, xr - 2, yr, zr), err, thr32[v]);
             if (d > 0) {
-                L.keep[x - 2][y][z] = 1;
-                L.keep[xr - 2][yr][zr] = 1;
+                set_keep(L, x - 2, y, z);
+                set_keep(L, xr - 2, yr, zr);
             } else if (d < 0) {
-                const int slot = atomicAdd(&L.nflag, 1);
-                L.flag[slot] = (uint16_t)v;
+                atomicOr(&L.flag[v >> 5], 1u << (v & 31));
             }
         }
         __syncthreads();
 
         // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
-        const int nflag = L.nflag;
-        if (nflag > 0) {
-            // Reload this lane's two rows (exact integers) for the direct DFT.
-            float s[2][16];
-            int zz[2], yy[2];
-            bool ok[2];
+        // Each lane sums its two rows' FP64 row DFTs, turned by the (ky, kz)
+        // phase of the bin; a wave reduction gives the exact bin.
+        bool any_flag = false;
+        if (lane < 24) any_flag = L.flag[lane] != 0u;
+        if (__any(any_flag)) {
+            double dre[2][5], dim[2][5];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int r = lane + 64 * h;
-                zz[h] = r >> 4;
-                yy[h] = 2 * (r & 15) + par[zz[h]];
-                const uint16_t *fp = job.field[zz[h]];
-                ok[h] = fp != nullptr && yy[h] >= startY && yy[h] < endY;
-                if (ok[h]) {
-                    load16(fp + (int64_t)((tileY + yy[h]) >> 1) * g.pitch + (tileX + g.lmargin), g.black, s[h]);
+                float s[16];
+                if (rowp[h] != nullptr) {
+                    load16(rowp[h] + tileX, g.black, s);
                 } else {
 #pragma unroll
-                    for (int n = 0; n < 16; ++n) s[h][n] = 0.0f;
+                    for (int n = 0; n < 16; ++n) s[n] = 0.0f;
                 }
+                xdft16_d(s, c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]], dre[h], dim[h]);
             }
-            for (int f = 0; f < nflag; ++f) {
-                const int v = L.flag[f];
-                const int z = v / 96, r2 = v - 96 * z;
-                const int y = r2 / 3, x = 2 + (r2 - 3 * y);
-                const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
-                double sa[2] = {0.0, 0.0}, sb[2] = {0.0, 0.0};
+            for (int w = 0; w < 24; ++w) {
+                uint32_t bits = L.flag[w];
+                while (bits != 0u) {
+                    const int v = w * 32 + __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    const int z = v / 96, r2 = v - 96 * z;
+                    const int y = r2 / 3, x = 2 + (r2 - 3 * y);
+                    const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
+                    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (!ok[h]) continue;
-                    const double rw = c_tab.wz_d[zz[h]] * c_tab.wy_d[yy[h]];
-                    for (int n = 0; n < 16; ++n) {
-                        const double xv = (double)s[h][n] * rw * c_tab.wx_d[n];
-                        const int ja = (4 * z * zz[h] + y * yy[h] + 2 * x * n) & 31;
-                        const int jb = (4 * zr * zz[h] + yr * yy[h] + 2 * xr * n) & 31;
-                        sa[0] += xv * c_tab.cos32_d[ja];
-                        sa[1] -= xv * c_tab.sin32_d[ja];
-                        sb[0] += xv * c_tab.cos32_d[jb];
-                        sb[1] -= xv * c_tab.sin32_d[jb];
+                    for (int h = 0; h < 2; ++h) {
+                        // phase of bin (kz, ky) at row (z_h, y_h): 4 kz z + ky y (32ths)
+                        const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
+                        const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
+                        const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
+                        const double xr_ = dre[h][x - 2], xi_ = dim[h][x - 2];
+                        const double yr_ = dre[h][xr - 2], yi_ = dim[h][xr - 2];
+                        // (xr + i xi)(ca - i sa)
+                        a0 += xr_ * ca + xi_ * sa;
+                        a1 += xi_ * ca - xr_ * sa;
+                        b0 += yr_ * cb + yi_ * sb;
+                        b1 += yi_ * cb - yr_ * sb;
                     }
-                }
-                const double ar = wave_sum_d(sa[0]), ai = wave_sum_d(sa[1]);
-                const double br = wave_sum_d(sb[0]), bi = wave_sum_d(sb[1]);
-                if (lane == 0 && decide64(ar, ai, br, bi, thr64[v])) {
-                    L.keep[x - 2][y][z] = 1;
-                    L.keep[xr - 2][yr][zr] = 1;
+                    a0 = wave_sum_d(a0);
+                    a1 = wave_sum_d(a1);
+                    b0 = wave_sum_d(b0);
+                    b1 = wave_sum_d(b1);
+                    if (lane == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
+                        set_keep(L, x - 2, y, z);
+                        set_keep(L, xr - 2, yr, zr);
+                    }
                 }
             }
             __syncthreads();
@@ -360,56 +493,61 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
         // ---- masked inverse z: outputs z = 0,1,4,5 ------------------------
         for (int c = lane; c < 160; c += 64) {
             const int k = c >> 5, ky = c & 31;
+            const uint32_t kb = (L.keep[c >> 2] >> ((c & 3) * 8)) & 0xFFu;
             c32 v[8];
 #pragma unroll
             for (int z = 0; z < 8; ++z) {
-                const c32 t = L.S[k][ky][z];
-                v[z] = L.keep[k][ky][z] ? t : c32{0.0f, 0.0f};
+                const c32 t = Sat(L, k, ky, z);
+                v[z] = ((kb >> z) & 1u) ? t : c32{0.0f, 0.0f};
             }
             fft<8, +1>(v);
-            L.S[k][ky][0] = v[0];
-            L.S[k][ky][1] = v[1];
-            L.S[k][ky][2] = v[4];
-            L.S[k][ky][3] = v[5];
+            Sat(L, k, ky, 0) = v[0];
+            Sat(L, k, ky, 1) = v[1];
+            Sat(L, k, ky, 2) = v[4];
+            Sat(L, k, ky, 3) = v[5];
         }
         __syncthreads();
 
         // ---- inverse y: fold 32 -> 16 bins for the plane's line parity ----
-        if (lane < 20) {
-            const int zs = lane / 5, k = lane - 5 * (lane / 5);
-            const int z = (zs < 2) ? zs : zs + 2;
-            const bool odd = par[z] != 0;
+        {
             c32 v[16];
+            const bool act = lane < 20;
+            const int zs = lane / 5, k = lane - 5 * (lane / 5);
+            if (act) {
+                const int z = (zs < 2) ? zs : zs + 2;
+                const bool odd = par[z] != 0;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const c32 lo = L.S[k][j][zs], hi = L.S[k][j + 16][zs];
-                c32 t = odd ? csub(lo, hi) : cadd(lo, hi);
-                if (odd) t = twiddle<+1>(t, j);
-                v[j] = t;
+                for (int j = 0; j < 16; ++j) {
+                    const c32 lo = Sat(L, k, j, zs), hi = Sat(L, k, j + 16, zs);
+                    c32 t = odd ? csub(lo, hi) : cadd(lo, hi);
+                    if (odd) t = twiddle<+1>(t, j);
+                    v[j] = t;
+                }
             }
-            fft<16, +1>(v);
+            __syncthreads();  // A and S share storage
+            if (act) {
+                fft<16, +1>(v);
 #pragma unroll
-            for (int m = 0; m < 16; ++m) L.A[zs][k][m] = v[m];
+                for (int m = 0; m < 16; ++m) L.u.A[lane][m] = v[m];
+            }
         }
         __syncthreads();
 
         // ---- inverse x (c2r on bins 2..6) and overlap-add ----------------
         {
             c32 C[5];
+            const float sc = 2.0f / 4096.0f;
 #pragma unroll
-            for (int k = 0; k < 5; ++k) C[k] = L.A[zsel][k][om];
-            float out[16];
-#pragma unroll
-            for (int n = 0; n < 16; ++n) {
-                float a = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 5; ++k) a += C[k].x * c_tab.ix_re[k][n] + C[k].y * c_tab.ix_im[k][n];
-                out[n] = a;
+            for (int k = 0; k < 5; ++k) {
+                const c32 t = L.u.A[zsel * 5 + k][om];
+                C[k] = c32{t.x * sc, t.y * sc};
             }
+            float out[16];
+            xidft16(C, out);
             const bool owned = tx >= tx_own0;
             if (owned && orow_ok) {
                 // columns tileX .. tileX+7 are complete (tiles tx-1 and tx)
-                const int c0 = tileX - g.avs + kColOrigin;  // partial column of tileX
+                const int c0 = tileX - g.avs + kColOrigin;
                 float4 *dst = reinterpret_cast<float4 *>(orow + c0);
                 float v8[8];
 #pragma unroll
