@@ -195,12 +195,23 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
         for (int i = 0; i < (fs + 1) * 4; ++i) cfilt[i] = (float)(cf[i] / cdiv);
     }
     std::vector<float> sine(g.fw), cosine(g.fw);
+    // At sampleRate = 4 fSC the carrier samples are exactly 0 and +-1 (the
+    // reference's FP64 sin/cos leave residues near 1e-16 where they vanish),
+    // which the quadrature demodulator exploits.
+    const bool quad = std::fabs(p->fsc_hz / p->sample_rate_hz - 0.25) < 1e-12;
     for (int i = 0; i < g.fw; ++i) {
         const double rad = 2 * M_PI * i * p->fsc_hz / p->sample_rate_hz;
+        double sv = std::sin(rad), cv = std::cos(rad);
+        if (quad) {
+            sv = std::round(sv);
+            cv = std::round(cv);
+        }
         // PAL-M swaps the references (palcolour.cpp:158-174)
-        sine[i] = (float)(g.fh != 263 ? std::sin(rad) : std::cos(rad));
-        cosine[i] = (float)(g.fh != 263 ? std::cos(rad) : std::sin(rad));
+        sine[i] = (float)(g.fh != 263 ? sv : cv);
+        cosine[i] = (float)(g.fh != 263 ? cv : sv);
     }
+    g.quad = (quad && fs <= 8 && g.avs >= fs && g.ave + fs + 1 <= g.fw) ? 1 : 0;
+    g.spar = (g.fw > 1 && sine[1] != 0.0f) ? 1 : 0;
 
     // thresholds (transform_pal.cpp:53-77): squared, z/y/x order
     const int nthr = (p->kind == CHD_DEC_TRANSFORM_3D) ? 768 : 80;

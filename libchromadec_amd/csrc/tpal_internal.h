@@ -45,6 +45,8 @@ struct Geom {
     int32_t n_tx2, n_strips2;
     // demod
     int32_t filter_size;
+    int32_t quad;            // 1: sampleRate == 4 fSC, carrier tables exactly 0/+-1
+    int32_t spar;            // column parity on which the sine table is non-zero
     float burst_floor;       // 130000 / 128
     // output
     int32_t out_w, out_h, pad_left, pad_top;

@@ -135,14 +135,18 @@ __device__ __forceinline__ double wave_sum_d(double v)
 // Keep/discard decision with an FP32 uncertainty test
 // (transform_pal_3d.cpp:313-327: discard when |a|^2 < |b|^2 t^2 or
 // |b|^2 < |a|^2 t^2).  Returns 1 keep, 0 discard, -1 when FP32 error `e` on
-// the bins could change the outcome.  |m_hat - m| <= 2|a|e + e^2 with
-// |a| <= |a.x| + |a.y|, plus the rounding of the products themselves.
+// the bins could change the outcome.  |m_hat - m| <= 2|a|e + e^2, plus the
+// rounding of the products themselves.
 __device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
 {
     const float ma = a.x * a.x + a.y * a.y;
     const float mb = b.x * b.x + b.y * b.y;
-    const float da = e * (2.0f * (fabsf(a.x) + fabsf(a.y)) + e);
-    const float db = e * (2.0f * (fabsf(b.x) + fabsf(b.y)) + e);
+    // |a| <= max(|a.x|,|a.y|) + (sqrt 2 - 1) min(|a.x|,|a.y|)
+    const float ax = fabsf(a.x), ay = fabsf(a.y), bx = fabsf(b.x), by = fabsf(b.y);
+    const float na = fmaf(0.41421357f, fminf(ax, ay), fmaxf(ax, ay));
+    const float nb = fmaf(0.41421357f, fminf(bx, by), fmaxf(bx, by));
+    const float da = e * (2.0f * na + e);
+    const float db = e * (2.0f * nb + e);
     const float g1 = ma - mb * th, g2 = mb - ma * th;
     const float r1 = 4.0f * kU * (ma + mb * th), r2 = 4.0f * kU * (mb + ma * th);
     const float d1 = fmaf(db, th, da) + r1;
@@ -251,23 +255,6 @@ __device__ __forceinline__ float xdft16(const float (&s)[16], c32 (&X)[5])
     return l1;
 }
 
-// The same row DFT in FP64 for the re-check (bins 2..6 of the window).
-__device__ __forceinline__ void xdft16_d(const float (&s)[16], double rw, double (&re)[5], double (&im)[5])
-{
-#pragma unroll
-    for (int j = 0; j < 5; ++j) { re[j] = 0.0; im[j] = 0.0; }
-#pragma unroll
-    for (int n = 0; n < 16; ++n) {
-        const double u = (double)s[n] * (rw * c_tab.wx_d[n]);
-#pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const int ph = (2 * (j + 2) * n) & 31;
-            re[j] = fma(u, c_tab.cos32_d[ph], re[j]);
-            im[j] = fma(-u, c_tab.sin32_d[ph], im[j]);
-        }
-    }
-}
-
 // c2r over bins 2..6 (C[j] = bin j+2, pre-scaled): out[n] = 2 Re sum C e^{+i..}.
 // Even bins repeat every 8 samples, odd bins change sign.
 __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
@@ -335,6 +322,11 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
         rowp[h] = ok ? fp + (int64_t)((tileY + ry[h]) >> 1) * g.pitch + g.lmargin : nullptr;
         roww[h] = c_tab.wz[rz[h]] * c_tab.wy[ry[h]];
     }
+    // vertical-stage roles: forward lanes (z, k) = lane / 5, inverse lanes
+    // (zs, k) = lane / 5 with z = 0,1,4,5
+    const bool fy_odd = par[min(lane / 5, 7)] != 0;
+    const int izs = min(lane / 5, 3);
+    const bool iy_odd = par[(izs < 2) ? izs : izs + 2] != 0;
     // mask-stage role: bins y = lane & 31; lanes 0-31 visit x = 2 (all z) and
     // x = 3 (z < 4), lanes 32-63 visit x = 3 (z >= 4) and x = 4 (all z)
     const int vy = lane & 31, vhalf = lane >> 5;
@@ -388,7 +380,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
             if (act) {
                 const int z = lane / 5, k = lane - 5 * (lane / 5);
                 fft<16, -1>(v);
-                const bool odd = par[z] != 0;
+                const bool odd = fy_odd;
 #pragma unroll
                 for (int ky = 0; ky < 16; ++ky) {
                     c32 x0 = v[ky];
@@ -442,18 +434,6 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
         bool any_flag = false;
         if (lane < 24) any_flag = L.flag[lane] != 0u;
         if (__any(any_flag)) {
-            double dre[2][5], dim[2][5];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                float s[16];
-                if (rowp[h] != nullptr) {
-                    load16(rowp[h] + tileX, g.black, s);
-                } else {
-#pragma unroll
-                    for (int n = 0; n < 16; ++n) s[n] = 0.0f;
-                }
-                xdft16_d(s, c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]], dre[h], dim[h]);
-            }
             for (int w = 0; w < 24; ++w) {
                 uint32_t bits = L.flag[w];
                 while (bits != 0u) {
@@ -463,19 +443,35 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
                     const int y = r2 / 3, x = 2 + (r2 - 3 * y);
                     const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
                     double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-#pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        // phase of bin (kz, ky) at row (z_h, y_h): 4 kz z + ky y (32ths)
+                        if (rowp[h] == nullptr) continue;
+                        // FP64 row DFT at bins x and xr = 8 - x from the samples
+                        // (phases uniform across the wave).  With c, s the
+                        // cos/sin of 2 pi x n / 16, bin x is sum u (c - i s) and
+                        // bin 8-x is sum (-1)^n u (c + i s): split by n parity.
+                        const uint16_t *p = rowp[h] + tileX;
+                        const double rw = c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]];
+                        double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
+#pragma unroll 2
+                        for (int n = 0; n < 16; n += 2) {
+                            const double u0 = ((double)p[n] - (double)g.black) * (rw * c_tab.wx_d[n]);
+                            const double u1 = ((double)p[n + 1] - (double)g.black) * (rw * c_tab.wx_d[n + 1]);
+                            const int j0 = (2 * x * n) & 31, j1 = (2 * x * (n + 1)) & 31;
+                            ce = fma(u0, L.cs[j0], ce);
+                            se = fma(u0, L.sn[j0], se);
+                            co = fma(u1, L.cs[j1], co);
+                            so = fma(u1, L.sn[j1], so);
+                        }
+                        const double ra = ce + co, ia = -(se + so);
+                        const double rb = ce - co, ib = se - so;
+                        // turn by the (ky, kz) phase of the row: 4 kz z + ky y (32ths)
                         const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
                         const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
                         const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
-                        const double xr_ = dre[h][x - 2], xi_ = dim[h][x - 2];
-                        const double yr_ = dre[h][xr - 2], yi_ = dim[h][xr - 2];
-                        // (xr + i xi)(ca - i sa)
-                        a0 += xr_ * ca + xi_ * sa;
-                        a1 += xi_ * ca - xr_ * sa;
-                        b0 += yr_ * cb + yi_ * sb;
-                        b1 += yi_ * cb - yr_ * sb;
+                        a0 += ra * ca + ia * sa;
+                        a1 += ia * ca - ra * sa;
+                        b0 += rb * cb + ib * sb;
+                        b1 += ib * cb - rb * sb;
                     }
                     a0 = wave_sum_d(a0);
                     a1 = wave_sum_d(a1);
@@ -514,8 +510,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
             const bool act = lane < 20;
             const int zs = lane / 5, k = lane - 5 * (lane / 5);
             if (act) {
-                const int z = (zs < 2) ? zs : zs + 2;
-                const bool odd = par[z] != 0;
+                const bool odd = iy_odd;
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     const c32 lo = Sat(L, k, j, zs), hi = Sat(L, k, j + 16, zs);
@@ -992,6 +987,193 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
 }
 
 // ---------------------------------------------------------------------------
+// Demodulation at 4fSC (sampleRate = 4 fSC exactly, the ld-decode lattice):
+// the reference carrier sin/cos(2 pi i / 4) is 0 or +-1, sine non-zero on
+// columns of parity `spar` only and cosine on the others.  Every m[] term of
+// decodeLine's FIR (palcolour.cpp:557-601) then vanishes on one column parity
+// and every n[] term on the other; since taps i-b and i+b share a parity, the
+// FIR regroups exactly into four symmetric convolutions of
+//   t0 = c[n], t1 = -(c[n-2] + c[n+2]), t2 = c[n-1] - c[n+1], t3 = c[n+3] - c[n-3]
+// (times the +-1 carrier), half the multiply-adds of the general form.
+// One 256-thread workgroup = kQLines field lines x 256 output columns.
+// ---------------------------------------------------------------------------
+
+constexpr int kQLines = 16;
+constexpr int kQHalo = 8;                             // >= filter_size
+constexpr int kQCols = kDemodThreads + 2 * kQHalo;    // staged columns
+
+struct LdsQ {
+    float chroma[kQLines + 6][kQCols];
+    float t[4][kQCols];
+    float bp[kQLines], bq[kQLines], vsw[kQLines];
+};
+
+__global__ void __launch_bounds__(kDemodThreads)
+pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
+               const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, int n_segs)
+{
+    __shared__ LdsQ L;
+    const int tid = threadIdx.x;
+    int rem = blockIdx.x;
+    const int seg = rem % n_segs; rem /= n_segs;
+    const int lb = rem % n_lblocks; rem /= n_lblocks;
+    const int fsel = rem & 1;
+    const int job_id = rem >> 1;
+    const FrameJob &job = jobs[job_id];
+    const int off = job.offset[fsel];
+    const uint16_t *comp = job.field[fsel];
+    const int firstLine = (g.first + 1 - off) / 2;
+    const int lastLine = ((g.last - off) / 2) + 1;
+    const int n0 = firstLine + lb * kQLines;
+    if (n0 >= lastLine) return;
+    const int nl = min(kQLines, lastLine - n0);
+    const int fs = g.filter_size;
+    const int i0 = g.avs + seg * kDemodThreads;
+    const int s0 = i0 - kQHalo;
+
+    // ---- stage chroma rows n0-3 .. n0+nl+2 (zero outside the active area) ---
+    const int nrows = nl + 6;
+    for (int idx = tid; idx < nrows * kQCols; idx += kDemodThreads) {
+        const int r = idx / kQCols, cc = idx - r * kQCols;
+        const int line = n0 - 3 + r;
+        const int col = s0 + cc;
+        float v = 0.0f;
+        if (line >= firstLine && line < lastLine && col >= g.avs && col < g.ave) {
+            const int64_t o = (int64_t)line * g.cpitch + (col - g.avs + kColOrigin);
+            const int nc = job.nchroma;
+            v = job.chroma[fsel][0][o];
+            if (nc > 1) v += job.chroma[fsel][1][o];
+            if (nc > 2) v += job.chroma[fsel][2][o];
+            if (nc > 3) v += job.chroma[fsel][3][o];
+        }
+        L.chroma[r][cc] = v;
+    }
+
+    // ---- burst detection per line (palcolour.cpp:344-409) -------------------
+    {
+        const int wave = tid >> 6, lane = tid & 63;
+        const int blen = g.cbe - g.cbs;
+        for (int li = wave; li < nl; li += kDemodThreads / 64) {
+            const int number = n0 + li;
+            float bp = 0, bq = 0, bpo = 0, bqo = 0;
+            for (int i = g.cbs + lane; i < g.cbe; i += 64) {
+                auto px = [&](int ln) -> float {
+                    if (ln < 0 || ln >= g.fh) return 0.0f;  // blackLine (zeros)
+                    if (comp == nullptr) return g.black;
+                    return (float)comp[(int64_t)ln * g.pitch + i + g.lmargin];
+                };
+                const float in0 = px(number), in1 = px(number - 1), in2 = px(number + 1);
+                const float in3 = px(number - 2), in4 = px(number + 2);
+                const float a = (in0 - (in3 + in4) * 0.5f) * 0.5f;
+                const float b = (in2 - in1) * 0.5f;
+                bp += a * sine[i];
+                bq += a * cosine[i];
+                bpo += b * sine[i];
+                bqo += b * cosine[i];
+            }
+            bp = wave_sum(bp) / blen;
+            bq = wave_sum(bq) / blen;
+            bpo = wave_sum(bpo) / blen;
+            bqo = wave_sum(bqo) / blen;
+            if (lane == 0) {
+                const float vs = (((bp - bpo) * (bp - bpo) + (bq - bqo) * (bq - bqo)) < (bp * bp + bq * bq) * 2) ? 1.0f : -1.0f;
+                float lbp = (bp - bqo) * 0.5f, lbq = (bq + bpo) * 0.5f;
+                const float norm = fmaxf(sqrtf(lbp * lbp + lbq * lbq), g.burst_floor);
+                lbp /= norm;
+                lbq /= norm;
+                L.bp[li] = (lbp * g.chroma_gain_cos - lbq * g.chroma_gain_sin);
+                L.bq[li] = (lbp * g.chroma_gain_sin + lbq * g.chroma_gain_cos);
+                L.vsw[li] = vs;
+            }
+        }
+    }
+    __syncthreads();
+
+    const bool is_float = (g.pixel_format == 1 /*YUV444PS*/ || g.pixel_format == 5 /*GRAYS*/);
+    const bool luma_only = (g.pixel_format == 4 /*GRAY16*/ || g.pixel_format == 5 /*GRAYS*/);
+    const int64_t plane = (int64_t)g.out_w * g.out_h;
+    float eyLo = -INFINITY, eyHi = INFINITY, ecLo = -INFINITY, ecHi = INFINITY;
+    float yLo = 0.0f, yHi = 65535.0f, cLo = 0.0f, cHi = 65535.0f;
+    if (g.clamp_mode == 1) {
+        eyLo = 0.0f; eyHi = 1.0f; ecLo = -0.5f; ecHi = 0.5f;
+        yLo = 16.0f * 256.0f; yHi = 235.0f * 256.0f; cLo = 16.0f * 256.0f; cHi = 240.0f * 256.0f;
+    } else if (g.clamp_mode == 3) {
+        eyLo = -15.0f / 219.0f; eyHi = 238.75f / 219.0f; ecLo = -127.0f / 224.0f; ecHi = 126.75f / 224.0f;
+        yLo = 256.0f; yHi = 254.75f * 256.0f; cLo = 256.0f; cHi = 254.75f * 256.0f;
+    }
+    float cf[kQHalo + 1][4];
+#pragma unroll
+    for (int b = 0; b <= kQHalo; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cf[b][j] = (b <= fs) ? cfilt[b * 4 + j] : 0.0f;
+
+    const int i = i0 + tid;
+    const bool col_live = i < g.ave;
+    const int ci = i - s0;
+    for (int li = 0; li < nl; ++li) {
+        const int number = n0 + li;
+        const int r0 = li + 3;
+        for (int cc = tid; cc < kQCols; cc += kDemodThreads) {
+            const int col = s0 + cc;
+            // the carrier: exactly one of sine/cosine is non-zero (+-1)
+            const float w = (col >= 0 && col < g.fw) ? sine[col] + cosine[col] : 0.0f;
+            const float c0 = L.chroma[r0][cc];
+            L.t[0][cc] = c0 * w;
+            L.t[1][cc] = -(L.chroma[r0 - 2][cc] + L.chroma[r0 + 2][cc]) * w;
+            L.t[2][cc] = (L.chroma[r0 - 1][cc] - L.chroma[r0 + 1][cc]) * w;
+            L.t[3][cc] = (L.chroma[r0 + 3][cc] - L.chroma[r0 - 3][cc]) * w;
+        }
+        __syncthreads();
+        if (col_live) {
+            float P01 = 0.0f, Q01 = 0.0f, P23 = 0.0f, Q23 = 0.0f;
+#pragma unroll
+            for (int b = 0; b <= kQHalo; ++b) {
+                const float s0_ = L.t[0][ci + b] + L.t[0][ci - b];
+                const float s1_ = L.t[1][ci + b] + L.t[1][ci - b];
+                const float s2_ = L.t[2][ci + b] + L.t[2][ci - b];
+                const float s3_ = L.t[3][ci + b] + L.t[3][ci - b];
+                const float a01 = fmaf(s1_, cf[b][1], s0_ * cf[b][0]);
+                const float a23 = fmaf(s3_, cf[b][3], s2_ * cf[b][2]);
+                if (((i + b) & 1) == g.spar) {  // sine column: m terms
+                    P01 += a01;
+                    Q23 += a23;
+                } else {                          // cosine column: n terms
+                    Q01 += a01;
+                    P23 += a23;
+                }
+            }
+            const float PU = P01 + P23, PV = P01 - P23, QU = Q01 - Q23, QV = Q01 + Q23;
+            const float bp = L.bp[li], bq = L.bq[li], vsw = L.vsw[li];
+            const float compv = (comp != nullptr) ? (float)comp[(int64_t)number * g.pitch + i + g.lmargin] : g.black;
+            const float Y = compv - L.chroma[r0][ci];
+            const float U = -(PU * bp + QU * bq) * 2.0f;
+            const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
+            const int frame_line = number * 2 + off;
+            const int orow = frame_line - g.first + g.pad_top;
+            const int64_t o = (int64_t)orow * g.out_w + g.pad_left + (i - g.avs);
+            const float ey = (Y - g.black) * g.ey_scale;
+            const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
+            if (is_float) {
+                float *out = reinterpret_cast<float *>(job.out);
+                out[o] = fminf(fmaxf(ey, eyLo), eyHi);
+                if (!luma_only) {
+                    out[plane + o] = fminf(fmaxf(ecb, ecLo), ecHi);
+                    out[2 * plane + o] = fminf(fmaxf(ecr, ecLo), ecHi);
+                }
+            } else {
+                uint16_t *out = reinterpret_cast<uint16_t *>(job.out);
+                out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), yLo), yHi);
+                if (!luma_only) {
+                    out[plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecb + 128.0f * 256.0f), cLo), cHi);
+                    out[2 * plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecr + 128.0f * 256.0f), cLo), cHi);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Packing and border fill.
 // ---------------------------------------------------------------------------
 
@@ -1055,6 +1237,14 @@ void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *
                   const float *cfilt, int n_lblocks, hipStream_t s)
 {
     const int n_segs = (g.aw + kDemodThreads - 1) / kDemodThreads;
+    if (g.quad) {
+        const int nlb = (n_lblocks * kDemodLines + kQLines - 1) / kQLines;
+        const int blocks = n_jobs * 2 * nlb * n_segs;
+        if (blocks > 0)
+            hipLaunchKernelGGL(pal_demod_quad, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt,
+                               nlb, n_segs);
+        return;
+    }
     const int blocks = n_jobs * 2 * n_lblocks * n_segs;
     if (blocks > 0)
         hipLaunchKernelGGL(pal_demod, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt,
