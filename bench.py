@@ -68,6 +68,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=16, help="frames for the CPU baseline (0 = skip)")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain elementwise copy of this many MiB (a known-size kernel that "
+                         "scales hardware FETCH_SIZE/WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the decode shards by frame across GPUs with no "
@@ -88,6 +91,12 @@ def main() -> int:
     out = torch.empty((N_FRAMES, plan.frame_bytes // 4), dtype=torch.float32, device=dev)
     seqs = [(packed.data_ptr(), N_FRAMES, 0, N_FRAMES)]
     stream = torch.cuda.current_stream().cuda_stream
+    if args.copy_mib > 0:
+        n = args.copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.neg(a, out=b)
+        del a, b
     torch.cuda.synchronize()
 
     for _ in range(args.warmup):

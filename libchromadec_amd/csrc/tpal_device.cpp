@@ -217,11 +217,14 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     const int nthr = (p->kind == CHD_DEC_TRANSFORM_3D) ? 768 : 80;
     std::vector<double> thr64(nthr);
     std::vector<float> thr32(nthr);
+    bool uniform = true;
     for (int i = 0; i < nthr; ++i) {
         const double t = p->thresholds ? p->thresholds[i] : p->threshold;
         thr64[i] = t * t;
         thr32[i] = (float)thr64[i];
+        uniform = uniform && thr32[i] == thr32[0];
     }
+    g.thr_uniform = uniform ? thr32[0] : -1.0f;
 
     chd_status_t st = ensure_tables(p->device);
     if (st != CHD_OK) { delete plan; return st; }

@@ -34,6 +34,7 @@ struct Geom {
     int32_t first, last;     // active frame lines, inclusive
     int32_t cbs, cbe;        // colour burst [cbs, cbe)
     float black;             // black16bIre
+    float thr_uniform;       // squared threshold when all bins share it, else -1
     int32_t pitch;           // packed row pitch (uint16 elements)
     int32_t lmargin;         // packed column of sample 0
     int64_t field_elems;     // packed field size (elements)

@@ -181,17 +181,26 @@ struct Lds3 {
                               // inverse: row zs*5+k holds the 16 output lines
         c32 S[5 * kSk];       // spectrum S[k*kSk + ky*kSpad + kz]
     } u;
-    uint32_t keep[40];        // bit (k*32+ky)*8+kz: bin kept by the mask
-    uint32_t flag[24];        // bit v: visit v undecidable in FP32
+    uint32_t keep[40];        // word k*8+kz, bit ky: bin (x = k+2, ky, kz) kept
+    uint32_t bal[24];         // word (x-2)*8+z, bit y: visit (z, y, x) keeps its pair
+    uint32_t flg[24];         // word (x-2)*8+z, bit y: visit undecidable in FP32
     double cs[32], sn[32];    // cos/sin(2 pi j / 32) for the FP64 re-check
 };
 
 __device__ __forceinline__ c32 &Sat(Lds3 &L, int k, int ky, int kz) { return L.u.S[k * kSk + ky * kSpad + kz]; }
-__device__ __forceinline__ void set_keep(Lds3 &L, int k, int ky, int kz)
+
+// Bit y of w moved to bit (8 - y) mod 32: the mirror row yr = (40 - y) % 32
+// of transform_pal_3d.cpp:291, applied to a word of 32 bins along ky.
+__device__ __forceinline__ uint32_t mirror_rows(uint32_t w)
 {
-    const int b = (k * 32 + ky) * 8 + kz;
-    atomicOr(&L.keep[b >> 5], 1u << (b & 31));
+    const uint32_t r = __builtin_bitreverse32(w);  // bit y -> 31 - y
+    return (r << 9) | (r >> 23);                   // + 9 (mod 32)
 }
+
+// Visit plane (x, z) handled in mask step i by lanes 0-31 (half 0) and
+// lanes 32-63 (half 1); bit y of the step's ballot is visit (z, y, x).
+__device__ __forceinline__ constexpr int visit_x(int half, int i) { return half == 0 ? (i < 8 ? 2 : 3) : (i < 4 ? 3 : 4); }
+__device__ __forceinline__ constexpr int visit_z(int half, int i) { return half == 0 ? (i < 8 ? i : i - 8) : (i < 4 ? i + 4 : i - 4); }
 
 // acc + x * c for a coefficient known at compile time after unrolling; the
 // exact 0 / +-1 cases cost nothing.
@@ -285,11 +294,21 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
 {
     __shared__ Lds3 L;
     const int lane = threadIdx.x;
-    const int per_job = g.n_ty3 * g.n_strips3;
-    const int job_id = blockIdx.x / per_job;
-    const int rem = blockIdx.x - job_id * per_job;
-    const int ty = rem / g.n_strips3;
-    const int strip = rem - ty * g.n_strips3;
+    // Workgroups are dispatched round-robin over the 8 XCDs, each with its
+    // own L2.  Give each XCD a contiguous share of the work, ordered slab
+    // fastest, then tile row, then strip: the workgroups resident on one XCD
+    // then cover the same few field columns of neighbouring slabs (which
+    // share 6 of their 8 fields) and neighbouring tile rows (which share 8
+    // of their 16 lines), so each field line is fetched into that L2 about
+    // once instead of once per tile.
+    const int total = gridDim.x;
+    const int xcd = blockIdx.x & 7, q = total >> 3, r = total & 7;
+    const int w = xcd * q + min(xcd, r) + (blockIdx.x >> 3);
+    const int n_jobs = total / (g.n_ty3 * g.n_strips3);
+    const int job_id = w % n_jobs;
+    const int rest = w / n_jobs;
+    const int ty = rest % g.n_ty3;
+    const int strip = rest / g.n_ty3;
     const SlabJob &job = jobs[job_id];
 
     if (lane < 32) {
@@ -364,8 +383,6 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
             for (int j = 0; j < 5; ++j) L.u.A[rz[h] * 5 + j][m] = c32{X[j].x * roww[h], X[j].y * roww[h]};
         }
         const float err = kErrK * kU * wave_sum(l1);
-        if (lane < 40) L.keep[lane] = 0u;
-        if (lane < 24) L.flag[lane] = 0u;
         __syncthreads();
 
         // ---- vertical: 16-point DFT of the data lines + parity twiddle ---
@@ -405,57 +422,89 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
         __syncthreads();
 
         // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
-        // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v]
+        // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v].  Decisions are
+        // gathered by wave ballots into keep words (plane (x-2, z), bit y);
+        // a kept visit also keeps its partner, whose word is the mirror of
+        // the ballot.  Keep is the OR over visits, as in the reference.
+        // rows for a possible FP64 re-decision, fetched while the mask runs
+        uint4 r00 = make_uint4(0u, 0u, 0u, 0u), r01 = r00, r10 = r00, r11 = r00;
+        if (rowp[0] != nullptr) {
+            const uint4 *q = reinterpret_cast<const uint4 *>(rowp[0] + tileX);
+            r00 = q[0];
+            r01 = q[1];
+        }
+        if (rowp[1] != nullptr) {
+            const uint4 *q = reinterpret_cast<const uint4 *>(rowp[1] + tileX);
+            r10 = q[0];
+            r11 = q[1];
+        }
 #pragma unroll 4
         for (int i = 0; i < 12; ++i) {
-            int x, z;
-            if (vhalf == 0) { x = (i < 8) ? 2 : 3; z = (i < 8) ? i : i - 8; }
-            else { x = (i < 4) ? 3 : 4; z = (i < 4) ? i + 4 : i - 4; }
+            const int x = vhalf ? visit_x(1, i) : visit_x(0, i);
+            const int z = vhalf ? visit_z(1, i) : visit_z(0, i);
             const int y = vy;
             const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
-            const int v = (z * 32 + y) * 3 + (x - 2);
-            if (x == xr && y == yr && z == zr) {
-                set_keep(L, x - 2, y, z);
-                continue;
-            }
-            const int d = decide32(Sat(L, x - 2, y, z), Sat(L, xr - 2, yr, zr), err, thr32[v]);
-            if (d > 0) {
-                set_keep(L, x - 2, y, z);
-                set_keep(L, xr - 2, yr, zr);
-            } else if (d < 0) {
-                atomicOr(&L.flag[v >> 5], 1u << (v & 31));
+            const int d = decide32(Sat(L, x - 2, y, z), Sat(L, xr - 2, yr, zr), err,
+                                     g.thr_uniform >= 0.0f ? g.thr_uniform : thr32[(z * 32 + y) * 3 + (x - 2)]);
+            const uint64_t kb = __ballot(d > 0);
+            const uint64_t fb = __ballot(d < 0);
+            if (vy == 0) {
+                const int w = (x - 2) * 8 + z;
+                L.bal[w] = (uint32_t)(kb >> (32 * vhalf));
+                L.flg[w] = (uint32_t)(fb >> (32 * vhalf));
             }
         }
+        __syncthreads();
+        // keep word (k, z) = own visits' ballot | ballot of the visit plane
+        // whose partners lie on (k, z), moved to the mirror rows
+        // (x_v = 8 - x, z_v = (10 - z) % 8); plus the self-mirror bins
+        // (x = 4, y in {4, 20}, z in {1, 5}), which are always kept.
+        constexpr uint32_t kSelf = (1u << 4) | (1u << 20);
+        if (lane < 40) {
+            const int k = lane >> 3, z = lane & 7;
+            uint32_t w = (k <= 2) ? L.bal[lane] : 0u;
+            if (k >= 2) w |= mirror_rows(L.bal[(4 - k) * 8 + ((10 - z) & 7)]);
+            if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
+            L.keep[lane] = w;
+        }
+        bool any_flag = false;
+        if (lane < 24) any_flag = (L.flg[lane] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) != 0u;
         __syncthreads();
 
         // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
         // Each lane sums its two rows' FP64 row DFTs, turned by the (ky, kz)
         // phase of the bin; a wave reduction gives the exact bin.
-        bool any_flag = false;
-        if (lane < 24) any_flag = L.flag[lane] != 0u;
         if (__any(any_flag)) {
-            for (int w = 0; w < 24; ++w) {
-                uint32_t bits = L.flag[w];
+            const double black = (double)g.black;
+#pragma unroll 1
+            for (int wi = 0; wi < 24; ++wi) {
+                uint32_t bits = L.flg[wi];
+                if (wi == 17 || wi == 21) bits &= ~kSelf;
+                const int x = 2 + (wi >> 3), z = wi & 7;
+                const int zr = (10 - z) & 7, xr = 8 - x;
                 while (bits != 0u) {
-                    const int v = w * 32 + __builtin_ctz(bits);
+                    const int y = __builtin_ctz(bits);
                     bits &= bits - 1u;
-                    const int z = v / 96, r2 = v - 96 * z;
-                    const int y = r2 / 3, x = 2 + (r2 - 3 * y);
-                    const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
+                    const int yr = (40 - y) & 31;
+                    const int v = (z * 32 + y) * 3 + (x - 2);
                     double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll 1
                     for (int h = 0; h < 2; ++h) {
-                        if (rowp[h] == nullptr) continue;
                         // FP64 row DFT at bins x and xr = 8 - x from the samples
                         // (phases uniform across the wave).  With c, s the
                         // cos/sin of 2 pi x n / 16, bin x is sum u (c - i s) and
                         // bin 8-x is sum (-1)^n u (c + i s): split by n parity.
-                        const uint16_t *p = rowp[h] + tileX;
+                        // A row outside the tile's lines holds black: u = 0.
+                        if (rowp[h] == nullptr) continue;
+                        const uint4 q0 = h ? r10 : r00, q1 = h ? r11 : r01;
+                        const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
                         const double rw = c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]];
                         double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
-#pragma unroll 2
+#pragma unroll
                         for (int n = 0; n < 16; n += 2) {
-                            const double u0 = ((double)p[n] - (double)g.black) * (rw * c_tab.wx_d[n]);
-                            const double u1 = ((double)p[n + 1] - (double)g.black) * (rw * c_tab.wx_d[n + 1]);
+                            const uint32_t w2 = raw[n >> 1];
+                            const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
+                            const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
                             const int j0 = (2 * x * n) & 31, j1 = (2 * x * (n + 1)) & 31;
                             ce = fma(u0, L.cs[j0], ce);
                             se = fma(u0, L.sn[j0], se);
@@ -478,8 +527,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
                     b0 = wave_sum_d(b0);
                     b1 = wave_sum_d(b1);
                     if (lane == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
-                        set_keep(L, x - 2, y, z);
-                        set_keep(L, xr - 2, yr, zr);
+                        L.keep[(x - 2) * 8 + z] |= 1u << y;
+                        L.keep[(xr - 2) * 8 + zr] |= 1u << yr;
                     }
                 }
             }
@@ -489,12 +538,11 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
         // ---- masked inverse z: outputs z = 0,1,4,5 ------------------------
         for (int c = lane; c < 160; c += 64) {
             const int k = c >> 5, ky = c & 31;
-            const uint32_t kb = (L.keep[c >> 2] >> ((c & 3) * 8)) & 0xFFu;
             c32 v[8];
 #pragma unroll
             for (int z = 0; z < 8; ++z) {
                 const c32 t = Sat(L, k, ky, z);
-                v[z] = ((kb >> z) & 1u) ? t : c32{0.0f, 0.0f};
+                v[z] = ((L.keep[k * 8 + z] >> ky) & 1u) ? t : c32{0.0f, 0.0f};
             }
             fft<8, +1>(v);
             Sat(L, k, ky, 0) = v[0];
@@ -617,12 +665,15 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
     __shared__ Lds2 L;
     const int lane = threadIdx.x;
     const int q = lane >> 4, ln = lane & 15;
-    const int per_job = n_ty_max * g.n_strips2 / 4 + 1;  // unused bound guard
-    (void)per_job;
     const int groups_per_row = (g.n_strips2 + 3) / 4;
     const int per_field = n_ty_max * groups_per_row;
-    const int job_id = blockIdx.x / per_field;
-    const int rem = blockIdx.x - job_id * per_field;
+    // contiguous share of the work per XCD (workgroups go round-robin over
+    // the 8 XCDs), so neighbouring tile rows of a field share one L2
+    const int total = gridDim.x;
+    const int xcd = blockIdx.x & 7, nq = total >> 3, nr = total & 7;
+    const int w = xcd * nq + min(xcd, nr) + (blockIdx.x >> 3);
+    const int job_id = w / per_field;
+    const int rem = w - job_id * per_field;
     const int ty = rem / groups_per_row;
     const int strip = (rem - ty * groups_per_row) * 4 + q;
     const FieldJob2D &job = jobs[job_id];
@@ -701,7 +752,8 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
                 L.keep[q][x - 4][y] = 1;
                 continue;
             }
-            const int d = decide32(L.A[q][x - 4][y], L.A[q][xr - 4][yr], err, thr32[v]);
+            const int d = decide32(L.A[q][x - 4][y], L.A[q][xr - 4][yr], err,
+                                   g.thr_uniform >= 0.0f ? g.thr_uniform : thr32[v]);
             if (d > 0) {
                 L.keep[q][x - 4][y] = 1;
                 L.keep[q][xr - 4][yr] = 1;

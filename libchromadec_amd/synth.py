@@ -91,7 +91,9 @@ def encode_field(field_no: int, y_pic, u_pic, v_pic, opts: EncodeOptions | None 
                  return_truth: bool = False):
     """Encode one field.  y/u/v_pic are [PIC_HEIGHT, PIC_WIDTH] Y'UV planes
     (Y 0..1) for the frame the field belongs to.  Returns uint16 [313, 1135]
-    and, optionally, the band-limited U/V truth [313, 1135] each."""
+    and, optionally, the truth: the band-limited U and V [313, 1135] and the
+    modulated chroma actually added to the signal, in 16-bit sample units
+    without the black level [313, 1135]."""
     opts = opts or EncodeOptions()
     off = field_no % 2
     frame_lines = np.arange(off, 2 * FIELD_HEIGHT, 2)           # 313 lines
@@ -159,7 +161,7 @@ def encode_field(field_no: int, y_pic, u_pic, v_pic, opts: EncodeOptions | None 
         out = out + rng.normal(0.0, sigma, size=out.shape)
     out = np.clip(out, 0x0100, 0xFEFF).astype(np.uint16)
     if return_truth:
-        return out, U, V
+        return out, U, V, np.clip(chroma, -chroma_gate, chroma_gate) * scale
     return out
 
 
@@ -234,19 +236,20 @@ class PatchScene:
 def encode_sequence(frames_rgb, opts: EncodeOptions | None = None, return_truth: bool = False,
                     first_field_no: int = 0):
     """Encode an iterable of R'G'B' pictures; returns uint16 [2N, 313, 1135]
-    (and [2N, 313, 1135] U/V truth if requested)."""
-    fields, tu, tv = [], [], []
+    (and the U, V and modulated-chroma truth, [2N, 313, 1135] float each, if
+    requested)."""
+    fields, tu, tv, tc = [], [], [], []
     for i, rgb in enumerate(frames_rgb):
         y, u, v = rgb_to_yuv(rgb)
         for k in range(2):
             r = encode_field(first_field_no + 2 * i + k, y, u, v, opts, return_truth)
             if return_truth:
-                fields.append(r[0]); tu.append(r[1]); tv.append(r[2])
+                fields.append(r[0]); tu.append(r[1]); tv.append(r[2]); tc.append(r[3])
             else:
                 fields.append(r)
     f = np.stack(fields)
     if return_truth:
-        return f, np.stack(tu), np.stack(tv)
+        return f, np.stack(tu), np.stack(tv), np.stack(tc)
     return f
 
 
