@@ -139,7 +139,24 @@ SIGNATURES = {
     "chd_tpal_set_timing": (C.c_int32, [_P, C.c_int]),
     "chd_tpal_last_timing": (C.c_int32, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    # tpal_mask.h
+    "chd_mask_net_create": (C.c_int32, [C.c_int32, C.c_int64, C.POINTER(_P)]),
+    "chd_mask_net_free": (None, [_P]),
+    "chd_mask_net_set_params": (C.c_int32, [_P, C.POINTER(C.c_float), C.c_int64]),
+    "chd_mask_net_get_params": (C.c_int32, [_P, C.POINTER(C.c_float), C.c_int64]),
+    "chd_mask_net_load": (C.c_int32, [_P, C.c_char_p]),
+    "chd_mask_net_save": (C.c_int32, [_P, C.c_char_p]),
+    "chd_mask_net_forward": (C.c_int32, [_P, _P, C.c_int64, _P, _P]),
+    "chd_mask_net_gradient": (C.c_int32, [_P, _P, _P, C.c_int64, C.c_float, _P, C.POINTER(C.c_double), _P]),
+    "chd_mask_net_adam_step": (C.c_int32, [_P, _P, C.c_float, _P]),
+    "chd_tpal_tiles_per_slab": (C.c_int64, [_P]),
+    "chd_tpal_slab_spectra": (C.c_int32, [_P, _P, C.c_int64, C.c_int64, C.c_int32, _P, _P]),
+    "chd_tpal_set_mask_net": (C.c_int32, [_P, _P]),
 }
+
+MASK_NET_PARAMS = 32033
+TILE_BINS = 1280
+TILE_VISITS = 768
 
 
 def _share_hip_runtime_with_torch():
@@ -328,10 +345,11 @@ def color_scales(color_difference="modern", broadcast_scaling="scientific"):
 
 
 def pal_tpal_params(kind=CHD_DEC_TRANSFORM_3D, pixel_format=CHD_PIXEL_YUV444PS, threshold=0.4,
-                    max_batch_frames=64, first_line=44, last_line=619, avs=185, ave=1107, device=0):
+                    max_batch_frames=64, first_line=44, last_line=619, avs=185, ave=1107, device=0,
+                    black=0x4000, white=0xD300):
     """Device-plan parameters for the line-locked 4fSC PAL geometry."""
     ecb, ecr = color_scales()
-    return TpalParams(kind, 1135, 313, avs, ave, first_line, last_line, 98, 138, 0x4000, 0xD300,
+    return TpalParams(kind, 1135, 313, avs, ave, first_line, last_line, 98, 138, black, white,
                       4433618.75, 17734475.0, threshold, None, 1.0, 0.0, ecb, ecr, pixel_format, 0,
                       ave - avs, last_line - first_line + 1, 0, 0, max_batch_frames, device)
 
@@ -371,9 +389,71 @@ class TpalPlan:
               "chd_tpal_last_timing")
         return a.value, b.value, n1.value, n2.value
 
+    @property
+    def tiles_per_slab(self) -> int:
+        return lib().chd_tpal_tiles_per_slab(self.h)
+
+    def slab_spectra(self, d_fields: int, n_fields: int, first_slab: int, n_slabs: int, d_spec: int,
+                     stream: int = 0):
+        check(lib().chd_tpal_slab_spectra(self.h, d_fields, n_fields, first_slab, n_slabs, d_spec, stream),
+              "chd_tpal_slab_spectra")
+
+    def set_mask_net(self, net):
+        check(lib().chd_tpal_set_mask_net(self.h, net.h if net is not None else None), "chd_tpal_set_mask_net")
+        self._net = net  # keep the network alive while the plan uses it
+
     def close(self):
         if self.h:
             lib().chd_tpal_plan_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MaskNet:
+    """chd_mask_net_* learned carrier-mirror mask (pointers are device addresses)."""
+
+    def __init__(self, max_tiles: int, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().chd_mask_net_create(device, max_tiles, C.byref(self.h)), "chd_mask_net_create")
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        check(lib().chd_mask_net_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size),
+              "chd_mask_net_set_params")
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(MASK_NET_PARAMS, np.float32)
+        check(lib().chd_mask_net_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size),
+              "chd_mask_net_get_params")
+        return p
+
+    def load(self, path: str):
+        check(lib().chd_mask_net_load(self.h, path.encode()), "chd_mask_net_load")
+
+    def save(self, path: str):
+        check(lib().chd_mask_net_save(self.h, path.encode()), "chd_mask_net_save")
+
+    def forward(self, d_spec: int, n_tiles: int, d_mask: int, stream: int = 0):
+        check(lib().chd_mask_net_forward(self.h, d_spec, n_tiles, d_mask, stream), "chd_mask_net_forward")
+
+    def gradient(self, d_spec: int, d_truth: int, n_tiles: int, scale: float, d_grad: int, stream: int = 0,
+                 want_loss: bool = True):
+        loss = C.c_double(0.0)
+        check(lib().chd_mask_net_gradient(self.h, d_spec, d_truth, n_tiles, scale, d_grad,
+                                          C.byref(loss) if want_loss else None, stream), "chd_mask_net_gradient")
+        return loss.value
+
+    def adam_step(self, d_grad: int, lr: float, stream: int = 0):
+        check(lib().chd_mask_net_adam_step(self.h, d_grad, lr, stream), "chd_mask_net_adam_step")
+
+    def close(self):
+        if self.h:
+            lib().chd_mask_net_free(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -13,6 +13,9 @@
 //
 // The chroma decode itself runs on the GPU through the device layer
 // (tpal_device.cpp); this file owns no arithmetic on samples.
+#include <chromadec/decoder.h>
+#include <chromadec/tpal_mask.h>
+#include <dlfcn.h>
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -56,6 +59,31 @@ bool is_ldzeug_kind(chd_decoder_kind_t k)
 
 // registry::optionApplies (registry.cpp:113-173), plus this build's
 // transform_mask / transform_mask_weights for CHD_DEC_TRANSFORM_3D.
+// Packaged learned-mask weights: data/tpal3d_mask.bin beside the library's
+// lib/ directory.
+std::string default_mask_weights()
+{
+    Dl_info info;
+    if (dladdr((void *)&chd_decoder_commit, &info) == 0 || info.dli_fname == nullptr) return std::string();
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
+    return dir + "/../data/tpal3d_mask.bin";
+}
+
+// Header check of a weights file (the format chd_mask_net_load reads).
+bool mask_weights_ok(const std::string &path)
+{
+    if (path.empty()) return false;
+    std::ifstream in(path, std::ios::binary);
+    char magic[8] = {};
+    int32_t count = 0;
+    if (!in.read(magic, 8) || std::memcmp(magic, "CHDMASK1", 8) != 0) return false;
+    if (!in.read(reinterpret_cast<char *>(&count), 4) || count != CHD_MASK_NET_PARAMS) return false;
+    in.seekg(0, std::ios::end);
+    return (int64_t)in.tellg() == 12 + (int64_t)count * 4;
+}
+
 bool option_applies(chd_decoder_kind_t kind, const std::string &n, OptType t)
 {
     if (t == OptType::I32 && (n == "padding_multiple" || n == "thread_count")) return true;
@@ -104,6 +132,7 @@ std::string hip_err(hipError_t e) { return std::string(hipGetErrorName(e)) + " (
 struct Worker {
     std::mutex mu;
     chd_tpal_plan *plan = nullptr;
+    chd_mask_net_t *net = nullptr;      // learned mask (transform_mask "learned")
     hipStream_t stream = nullptr;
     uint16_t *d_slots = nullptr;
     std::vector<int64_t> slot_key;      // field number * 2 + shifted, -1 = empty
@@ -120,6 +149,7 @@ struct Worker {
             if (stream) (void)hipStreamSynchronize(stream);
             chd_tpal_plan_free(plan);
         }
+        chd_mask_net_free(net);
         if (d_slots) (void)hipFree(d_slots);
         if (d_out) (void)hipFree(d_out);
         if (h_stage) (void)hipHostFree(h_stage);
@@ -148,6 +178,7 @@ struct chd_decoder {
     int32_t threadCount = 1;
     chd_tpal_params_t params{};
     std::vector<double> thresholds;
+    std::string maskWeights;            // non-empty: learned mask from this file
     std::mutex workersMu;
     std::vector<std::unique_ptr<Worker>> workers;
 
@@ -188,6 +219,13 @@ chd_status_t worker_init(chd_decoder_t *d, Worker &w, const char *who)
                               (size_t)kCacheSlots * g.field_elems, w.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
     if (e != hipSuccess) return fail(CHD_E_OOM, std::string(who) + ": device allocation failed: " + hip_err(e));
+    if (!d->maskWeights.empty()) {
+        // one slab of tiles per network pass; a frame window uses two
+        st = chd_mask_net_create(p.device, (int64_t)g.n_ty3 * g.n_tx3, &w.net);
+        if (st == CHD_OK) st = chd_mask_net_load(w.net, d->maskWeights.c_str());
+        if (st == CHD_OK) st = chd_tpal_set_mask_net(w.plan, w.net);
+        if (st != CHD_OK) return st;
+    }
     w.slot_key.assign(kCacheSlots, -1);
     w.slot_tick.assign(kCacheSlots, 0);
     return CHD_OK;
@@ -550,8 +588,24 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
     if (auto it = d->str.find("transform_mask"); it != d->str.end()) mask = it->second;
     if (mask != "threshold" && mask != "learned")
         return fail(CHD_E_INVALID_ARG, fn + ": unknown transform_mask \"" + mask + "\"");
-    if (mask == "learned")
-        return fail(CHD_E_UNSUPPORTED, fn + ": transform_mask \"learned\" is not available in this build");
+    d->maskWeights.clear();
+    if (mask == "learned") {
+        // an explicit weights file must load; without one the packaged
+        // weights are used, and if they are absent the threshold stays
+        std::string path;
+        const auto it = d->str.find("transform_mask_weights");
+        const bool explicit_path = it != d->str.end() && !it->second.empty();
+        path = explicit_path ? it->second : default_mask_weights();
+        if (mask_weights_ok(path)) {
+            d->maskWeights = path;
+        } else if (explicit_path) {
+            return fail(CHD_E_NN_MODEL_LOAD, fn + ": transform_mask_weights \"" + path +
+                                                 "\" is not a readable mask weights file");
+        } else {
+            chd::log_message(CHD_LOG_WARN, fn + ": no learned-mask weights at \"" + path +
+                                               "\"; decoding with the threshold mask");
+        }
+    }
 
     // PalColour::Configuration + TransformPal::updateConfiguration
     double threshold = 0.4;

@@ -1,0 +1,570 @@
+// Learned carrier-mirror mask for the 3D Transform-PAL tile on gfx950.
+//
+// The hard keep/discard rule of TransformPal3D::applyFilter
+// (src/decoders/palcolour/transform_pal_3d.cpp:249-333) compares each bin
+// with its mirror about the U/V carrier.  Here a small convolutional network
+// looks at both magnitudes of every visit and its neighbours in (z, y, x)
+// and predicts a soft mask in [0, 1] instead.  Layers are implicit GEMMs on
+// the matrix cores (v_mfma_f32_32x32x16_bf16: BF16 operands, FP32
+// accumulation); activations are stored as BF16 between layers.
+//
+// Operand layouts of v_mfma_f32_32x32x16_bf16 (64 lanes, l = lane):
+//   A (32 x 16): lane l holds A[l % 32][8 (l/32) + e], e = 0..7
+//   B (16 x 32): lane l holds B[8 (l/32) + e][l % 32]
+//   C (32 x 32): lane l, value v holds C[8 (v/4) + 4 (l/32) + v % 4][l % 32]
+//
+// One workgroup of 4 waves handles one tile (768 visits = 24 row blocks of
+// 32, six per wave) in the forward and data-gradient kernels.  The weight
+// gradients accumulate over tiles in registers per workgroup and are
+// summed over workgroups in a fixed order, so every result is
+// deterministic.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mask_net_internal.h"
+
+namespace mnet {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bf16, round to nearest even (finite inputs)
+__device__ __forceinline__ uint32_t f2bf(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return f2bf(lo) | (f2bf(hi) << 16); }
+
+__device__ __forceinline__ f32x16 mfma(uint4 a, uint4 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+// Neighbour of visit p = (z*32 + y)*3 + x at tap t, or -1 where x leaves
+// the grid (z and y wrap).
+__device__ __forceinline__ int nbr(int p, int t)
+{
+    const int zy = p / 3, x = p - 3 * zy;
+    const int y = zy & 31, z = zy >> 5;
+    const int tz = t / 9, r = t - 9 * tz, ty = r / 3, tx = r - 3 * ty;
+    const int xs = x + tx - 1;
+    if (xs < 0 || xs > 2) return -1;
+    return ((((z + tz - 1) & 7) << 5) | ((y + ty - 1) & 31)) * 3 + xs;
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the 256 threads of a workgroup in a fixed order.
+__device__ __forceinline__ float block_sum(float v, float *red)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const float s = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Weight fragments from the FP32 parameters.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(64) k_arrange(const float *__restrict__ th, Frags f)
+{
+    const int j = blockIdx.x, l = threadIdx.x;
+    const int n = l & 31, k0 = 8 * (l >> 5);
+    float v[8];
+    uint4 *dst;
+    if (j < kK1) {  // W1: k = t*4 + ci (k < 108), n = cout
+        for (int e = 0; e < 8; ++e) {
+            const int k = 16 * j + k0 + e;
+            v[e] = k < 108 ? th[kOffW1 + k * kCh + n] : 0.0f;
+        }
+        dst = f.w1 + j * 64;
+    } else if (j < kK1 + kK2) {  // W2: k = t*32 + ci
+        const int jj = j - kK1;
+        for (int e = 0; e < 8; ++e) v[e] = th[kOffW2 + (16 * jj + k0 + e) * kCh + n];
+        dst = f.w2 + jj * 64;
+    } else if (j < kK1 + 2 * kK2) {  // W2 transposed: k = t*32 + cout, n = ci
+        const int jj = j - kK1 - kK2;
+        for (int e = 0; e < 8; ++e) {
+            const int k = 16 * jj + k0 + e, t = k >> 5, co = k & 31;
+            v[e] = th[kOffW2 + (t * kCh + n) * kCh + co];
+        }
+        dst = f.w2t + jj * 64;
+    } else if (j < kK1 + 2 * kK2 + kK3) {  // W3 for the head: k = ci, n = t
+        const int jj = j - kK1 - 2 * kK2;
+        for (int e = 0; e < 8; ++e) v[e] = n < 27 ? th[kOffW3 + n * kCh + 16 * jj + k0 + e] : 0.0f;
+        dst = f.w3t + jj * 64;
+    } else {  // W3 for the data gradient: k = t, n = ci
+        const int jj = j - kK1 - 2 * kK2 - kK3;
+        for (int e = 0; e < 8; ++e) {
+            const int k = 16 * jj + k0 + e;
+            v[e] = k < 27 ? th[kOffW3 + k * kCh + n] : 0.0f;
+        }
+        dst = f.w3 + jj * 64;
+    }
+    dst[l] = make_uint4(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+}
+
+// ---------------------------------------------------------------------------
+// Features and training targets from tile spectra.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void visit_bins(int p, int &ia, int &ib)
+{
+    const int zy = p / 3, x = p - 3 * zy;
+    const int y = zy & 31, z = zy >> 5;
+    const int zr = (10 - z) & 7, yr = (40 - y) & 31;
+    ia = (x * 32 + y) * 8 + z;
+    ib = ((4 - x) * 32 + yr) * 8 + zr;
+}
+
+__global__ void __launch_bounds__(256) k_features(const float2 *__restrict__ spec, uint16_t *__restrict__ feat)
+{
+    __shared__ float red[4];
+    const int64_t t = blockIdx.x;
+    const float2 *S = spec + t * kSpecC;
+    float f0[3], f1[3], s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int p = threadIdx.x + 256 * j;
+        int ia, ib;
+        visit_bins(p, ia, ib);
+        const float2 a = S[ia], b = S[ib];
+        f0[j] = (logf(sqrtf(a.x * a.x + a.y * a.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
+        f1[j] = (logf(sqrtf(b.x * b.x + b.y * b.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
+        s += f0[j];
+    }
+    const float mean = block_sum(s, red) * (1.0f / kP);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int p = threadIdx.x + 256 * j;
+        // self-mirror visits: x = 4 (index 2), y in {4, 20}, z in {1, 5}
+        const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
+        const float sm = (x == 2 && (y == 4 || y == 20) && (z == 1 || z == 5)) ? 1.0f : 0.0f;
+        reinterpret_cast<uint2 *>(feat)[t * kP + p] = make_uint2(pack2(f0[j], f1[j]), pack2(mean, sm));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_targets(const float2 *__restrict__ spec, const float2 *__restrict__ truth,
+                                                 float *__restrict__ w, float *__restrict__ num)
+{
+    const int64_t t = blockIdx.x;
+    const float2 *S = spec + t * kSpecC, *C = truth + t * kSpecC;
+    for (int p = threadIdx.x; p < kP; p += 256) {
+        int ia, ib;
+        visit_bins(p, ia, ib);
+        const float2 a = S[ia], ca = C[ia];
+        float ww = a.x * a.x + a.y * a.y;
+        float nn = ca.x * a.x + ca.y * a.y;
+        if (p % 3 < 2) {  // x = 2, 3: the visit also owns its partner bin
+            const float2 b = S[ib], cb = C[ib];
+            ww += b.x * b.x + b.y * b.y;
+            nn += cb.x * b.x + cb.y * b.y;
+        }
+        w[t * kP + p] = ww;
+        num[t * kP + p] = nn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Convolution layers as implicit GEMMs: out[p][n] = sum_k A[p][k] B[k][n].
+// ---------------------------------------------------------------------------
+
+enum { kFwd1 = 0, kFwd2 = 1, kBwd2 = 2, kBwd3 = 3 };
+
+// A fragment for row (visit) p of k-step j; hi = lane / 32.
+template <int MODE>
+__device__ __forceinline__ uint4 a_frag(const void *__restrict__ in, int64_t tbase, int p, int j, int hi)
+{
+    if constexpr (MODE == kFwd1) {
+        // k = t*4 + ci: this lane's 8 k values are taps 4j+2hi and 4j+2hi+1
+        const uint2 *f = reinterpret_cast<const uint2 *>(in) + tbase;
+        const int t0 = 4 * j + 2 * hi;
+        uint2 lo = make_uint2(0u, 0u), up = make_uint2(0u, 0u);
+        if (t0 < 27) {
+            const int q = nbr(p, t0);
+            if (q >= 0) lo = f[q];
+        }
+        if (t0 + 1 < 27) {
+            const int q = nbr(p, t0 + 1);
+            if (q >= 0) up = f[q];
+        }
+        return make_uint4(lo.x, lo.y, up.x, up.y);
+    } else if constexpr (MODE == kFwd2 || MODE == kBwd2) {
+        // k = t*32 + c; the data gradient reads the mirrored tap 26 - t
+        const uint4 *a = reinterpret_cast<const uint4 *>(in) + tbase * 4;  // 4 uint4 per row
+        const int t = j >> 1, c0 = (j & 1) * 16 + 8 * hi;
+        const int q = nbr(p, MODE == kFwd2 ? t : 26 - t);
+        return q >= 0 ? a[q * 4 + (c0 >> 3)] : make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        // kBwd3: A[q][k = t] = dlogit[nbr(q, -t)], taps 16j + 8hi + e
+        const float *d = reinterpret_cast<const float *>(in) + tbase;
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = 16 * j + 8 * hi + e;
+            const int q = t < 27 ? nbr(p, 26 - t) : -1;
+            v[e] = q >= 0 ? d[q] : 0.0f;
+        }
+        return make_uint4(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+    }
+}
+
+// Forward layers 1 and 2 (bias + ReLU) and the data gradients of layers 3
+// and 2 (times the ReLU gate of the layer below).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv(const void *__restrict__ in, const uint4 *__restrict__ wf,
+                                              const float *__restrict__ bias, const uint16_t *__restrict__ gate,
+                                              uint16_t *__restrict__ out)
+{
+    constexpr int KS = MODE == kFwd1 ? kK1 : (MODE == kBwd3 ? kK3 : kK2);
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 31, hi = lane >> 5;
+    const int64_t tbase = t * kP;  // first visit row of the tile
+    f32x16 acc[6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
+    for (int j = 0; j < KS; ++j) {
+        const uint4 bf = wf[j * 64 + lane];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            const int p = (wave + 4 * b) * 32 + i;
+            acc[b] = mfma(a_frag<MODE>(in, tbase, p, j, hi), bf, acc[b]);
+        }
+    }
+    const float bn = (MODE == kFwd1 || MODE == kFwd2) ? bias[i] : 0.0f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int p = (wave + 4 * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+            const int64_t o = (t * kP + p) * kCh + i;
+            float x = acc[b][v];
+            if constexpr (MODE == kFwd1 || MODE == kFwd2) {
+                x = fmaxf(x + bn, 0.0f);
+            } else {
+                x = bf2f(gate[o]) > 0.0f ? x : 0.0f;
+            }
+            out[o] = (uint16_t)f2bf(x);
+        }
+    }
+}
+
+// Layer 3 (32 -> 1) and the sigmoid.  G[q][t] = sum_c act2[q][c] W3[t][c]
+// on the matrix cores, then logit[p] = b3 + sum_t G[nbr(p, t)][t] in a
+// fixed tap order.
+__global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ act2, const uint4 *__restrict__ w3t,
+                                              const float *__restrict__ b3, float *__restrict__ logit,
+                                              float *__restrict__ mask)
+{
+    __shared__ float G[kP * 27];
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 31, hi = lane >> 5;
+    const uint4 *a = reinterpret_cast<const uint4 *>(act2) + t * kP * 4;
+    const uint4 bf0 = w3t[lane], bf1 = w3t[64 + lane];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const int p = (wave + 4 * b) * 32 + i;
+        f32x16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+        acc = mfma(a[p * 4 + hi], bf0, acc);
+        acc = mfma(a[p * 4 + 2 + hi], bf1, acc);
+        if (i < 27) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int q = (wave + 4 * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                G[q * 27 + i] = acc[v];
+            }
+        }
+    }
+    __syncthreads();
+    const float bias = b3[0];
+    for (int p = threadIdx.x; p < kP; p += 256) {
+        float s = bias;
+        for (int tap = 0; tap < 27; ++tap) {
+            const int q = nbr(p, tap);
+            if (q >= 0) s += G[q * 27 + tap];
+        }
+        logit[t * kP + p] = s;
+        mask[t * kP + p] = 1.0f / (1.0f + expf(-s));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Loss and its gradient at the logits.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_loss(const float *__restrict__ mask, const float *__restrict__ w,
+                                              const float *__restrict__ num, float scale,
+                                              float *__restrict__ dlogit, float *__restrict__ loss_tile)
+{
+    __shared__ float red[4];
+    const int64_t t = blockIdx.x;
+    float l = 0.0f;
+    for (int p = threadIdx.x; p < kP; p += 256) {
+        const int64_t o = t * kP + p;
+        const float m = mask[o], ww = w[o], nn = num[o];
+        dlogit[o] = 2.0f * scale * (ww * m - nn) * m * (1.0f - m);
+        l += scale * (ww * m * m - 2.0f * nn * m);
+    }
+    l = block_sum(l, red);
+    if (threadIdx.x == 0) loss_tile[t] = l;
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradients: per workgroup over tiles blockIdx.x, +gridDim.x, ...,
+// written to partial[slot][param]; k_reduce sums the slots in order.
+// ---------------------------------------------------------------------------
+
+// dW2[t][ci][co] = sum_q act1[q][ci] dpre2[nbr(q, -t)][co]; wave w owns taps
+// w, w+4, ... (C rows = ci, columns = co, K = q).
+__global__ void __launch_bounds__(256) k_wgrad2(const uint16_t *__restrict__ act1,
+                                                const uint16_t *__restrict__ dpre2, int64_t n_tiles,
+                                                float *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 31, hi = lane >> 5;
+    f32x16 acc[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[k][v] = 0.0f;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint16_t *A = act1 + t * kP * kCh;
+        const uint16_t *D = dpre2 + t * kP * kCh;
+        for (int kb = 0; kb < kP / 16; ++kb) {
+            const int q0 = kb * 16 + 8 * hi;
+            uint32_t av[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                av[e] = (uint32_t)A[(q0 + 2 * e) * kCh + i] | ((uint32_t)A[(q0 + 2 * e + 1) * kCh + i] << 16);
+            const uint4 af = make_uint4(av[0], av[1], av[2], av[3]);
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const int tap = wave + 4 * k;
+                if (tap < 27) {
+                    uint32_t bv[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int q = nbr(q0 + e, 26 - tap);
+                        bv[e] = q >= 0 ? (uint32_t)D[q * kCh + i] : 0u;
+                    }
+                    const uint4 bfr = make_uint4(bv[0] | (bv[1] << 16), bv[2] | (bv[3] << 16),
+                                                 bv[4] | (bv[5] << 16), bv[6] | (bv[7] << 16));
+                    acc[k] = mfma(af, bfr, acc[k]);
+                }
+            }
+        }
+    }
+    float *dst = partial + (int64_t)blockIdx.x * kNParams + kOffW2;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const int tap = wave + 4 * k;
+        if (tap < 27) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int ci = 8 * (v >> 2) + 4 * hi + (v & 3);
+                dst[(tap * kCh + ci) * kCh + i] = acc[k][v];
+            }
+        }
+    }
+}
+
+// dW1[t][ci][co] = sum_p feat[nbr(p, t)][ci] dpre1[p][co]; C rows m = t*4+ci
+// (108, four blocks of 32, one per wave), columns co, K = p.
+__global__ void __launch_bounds__(256) k_wgrad1(const uint16_t *__restrict__ feat,
+                                                const uint16_t *__restrict__ dpre1, int64_t n_tiles,
+                                                float *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 31, hi = lane >> 5;
+    const int m = wave * 32 + i, tap = m >> 2, ci = m & 3;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint16_t *F = feat + t * kP * 4;
+        const uint16_t *D = dpre1 + t * kP * kCh;
+        for (int kb = 0; kb < kP / 16; ++kb) {
+            const int p0 = kb * 16 + 8 * hi;
+            uint32_t av[8], bv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int q = m < 108 ? nbr(p0 + e, tap) : -1;
+                av[e] = q >= 0 ? (uint32_t)F[q * 4 + ci] : 0u;
+                bv[e] = (uint32_t)D[(p0 + e) * kCh + i];
+            }
+            const uint4 af = make_uint4(av[0] | (av[1] << 16), av[2] | (av[3] << 16), av[4] | (av[5] << 16),
+                                        av[6] | (av[7] << 16));
+            const uint4 bfr = make_uint4(bv[0] | (bv[1] << 16), bv[2] | (bv[3] << 16), bv[4] | (bv[5] << 16),
+                                         bv[6] | (bv[7] << 16));
+            acc = mfma(af, bfr, acc);
+        }
+    }
+    float *dst = partial + (int64_t)blockIdx.x * kNParams + kOffW1;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = wave * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+        if (row < 108) dst[row * kCh + i] = acc[v];
+    }
+}
+
+// dW3[t][ci] = sum_q act2[q][ci] dlogit[nbr(q, -t)] (C rows ci, columns t,
+// K = q split over the 4 waves), and the bias gradients
+// db1 = sum dpre1, db2 = sum dpre2, db3 = sum dlogit.
+__global__ void __launch_bounds__(256) k_wgrad3_bias(const uint16_t *__restrict__ act2,
+                                                     const float *__restrict__ dlogit,
+                                                     const uint16_t *__restrict__ dpre1,
+                                                     const uint16_t *__restrict__ dpre2, int64_t n_tiles,
+                                                     float *__restrict__ partial)
+{
+    __shared__ float red[4][32 * 32 + 65];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 31, hi = lane >> 5;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    float bsum = 0.0f;  // thread c < 32: db1[c]; 32..63: db2[c-32]; 64: db3 (per wave share)
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint16_t *A = act2 + t * kP * kCh;
+        const float *D = dlogit + t * kP;
+        for (int kb = wave; kb < kP / 16; kb += 4) {
+            const int q0 = kb * 16 + 8 * hi;
+            uint32_t av[8], bv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                av[e] = (uint32_t)A[(q0 + e) * kCh + i];
+                const int q = i < 27 ? nbr(q0 + e, 26 - i) : -1;
+                bv[e] = q >= 0 ? f2bf(D[q]) : 0u;
+            }
+            const uint4 af = make_uint4(av[0] | (av[1] << 16), av[2] | (av[3] << 16), av[4] | (av[5] << 16),
+                                        av[6] | (av[7] << 16));
+            const uint4 bfr = make_uint4(bv[0] | (bv[1] << 16), bv[2] | (bv[3] << 16), bv[4] | (bv[5] << 16),
+                                         bv[6] | (bv[7] << 16));
+            acc = mfma(af, bfr, acc);
+        }
+        // bias sums: lane-owned channel over this wave's quarter of the visits
+        const int c = lane;
+        const uint16_t *src = c < 32 ? dpre1 + t * kP * kCh : dpre2 + t * kP * kCh;
+        for (int p = wave; p < kP; p += 4) bsum += bf2f(src[p * kCh + (c & 31)]);
+    }
+    float db3 = 0.0f;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x)
+        for (int p = threadIdx.x; p < kP; p += 256) db3 += dlogit[t * kP + p];
+    db3 = wave_sum(db3);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) red[wave][(8 * (v >> 2) + 4 * hi + (v & 3)) * 32 + i] = acc[v];
+    red[wave][1024 + lane] = bsum;
+    if (lane == 0) red[wave][1024 + 64] = db3;
+    __syncthreads();
+    float *dst = partial + (int64_t)blockIdx.x * kNParams;
+    for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+        const int ci = e >> 5, tp = e & 31;
+        const float s = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+        if (tp < 27) dst[kOffW3 + tp * kCh + ci] = s;
+    }
+    if (threadIdx.x < 64) {
+        const int c = threadIdx.x;
+        const float s = (red[0][1024 + c] + red[1][1024 + c]) + (red[2][1024 + c] + red[3][1024 + c]);
+        dst[(c < 32 ? kOffB1 : kOffB2) + (c & 31)] = s;
+    }
+    if (threadIdx.x == 0)
+        dst[kOffB3] = (red[0][1088] + red[1][1088]) + (red[2][1088] + red[3][1088]);
+}
+
+__global__ void k_reduce(const float *__restrict__ partial, int slots, float *__restrict__ grad)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kNParams) return;
+    float s = 0.0f;
+    for (int j = 0; j < slots; ++j) s += partial[(int64_t)j * kNParams + k];
+    grad[k] = s;
+}
+
+__global__ void k_adam(float *__restrict__ th, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kNParams) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.0f - b1) * gk;
+    const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    th[k] -= lr * (mk * c1) / (sqrtf(vk * c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+
+void launch_arrange(const float *theta, const Frags &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_arrange, dim3(kK1 + 2 * kK2 + 2 * kK3), dim3(64), 0, s, theta, f);
+}
+
+void launch_features(const float2 *spec, int64_t n_tiles, uint16_t *feat, hipStream_t s)
+{
+    if (n_tiles > 0) hipLaunchKernelGGL(k_features, dim3((uint32_t)n_tiles), dim3(256), 0, s, spec, feat);
+}
+
+void launch_targets(const float2 *spec, const float2 *truth, int64_t n_tiles, float *w, float *num, hipStream_t s)
+{
+    if (n_tiles > 0) hipLaunchKernelGGL(k_targets, dim3((uint32_t)n_tiles), dim3(256), 0, s, spec, truth, w, num);
+}
+
+void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n_tiles, float *mask, hipStream_t s)
+{
+    if (n_tiles <= 0) return;
+    const dim3 grid((uint32_t)n_tiles), blk(256);
+    hipLaunchKernelGGL(k_conv<kFwd1>, grid, blk, 0, s, (const void *)a.feat, f.w1, theta + kOffB1,
+                       (const uint16_t *)nullptr, a.act1);
+    hipLaunchKernelGGL(k_conv<kFwd2>, grid, blk, 0, s, (const void *)a.act1, f.w2, theta + kOffB2,
+                       (const uint16_t *)nullptr, a.act2);
+    hipLaunchKernelGGL(k_head, grid, blk, 0, s, a.act2, f.w3t, theta + kOffB3, a.logit, mask);
+}
+
+void launch_backward(const float *theta, const Frags &f, const Acts &a, const float *mask, int64_t n_tiles,
+                     float scale, float *partial, float *grad, hipStream_t s)
+{
+    (void)theta;
+    if (n_tiles <= 0) return;
+    const dim3 grid((uint32_t)n_tiles), blk(256);
+    hipLaunchKernelGGL(k_loss, grid, blk, 0, s, mask, a.w, a.num, scale, a.dlogit, a.loss_tile);
+    hipLaunchKernelGGL(k_conv<kBwd3>, grid, blk, 0, s, (const void *)a.dlogit, f.w3, (const float *)nullptr,
+                       (const uint16_t *)a.act2, a.dpre2);
+    hipLaunchKernelGGL(k_conv<kBwd2>, grid, blk, 0, s, (const void *)a.dpre2, f.w2t, (const float *)nullptr,
+                       (const uint16_t *)a.act1, a.dpre1);
+    const int slots = (int)(n_tiles < kWgradSlots ? n_tiles : kWgradSlots);
+    hipLaunchKernelGGL(k_wgrad1, dim3(slots), blk, 0, s, a.feat, a.dpre1, n_tiles, partial);
+    hipLaunchKernelGGL(k_wgrad2, dim3(slots), blk, 0, s, a.act1, a.dpre2, n_tiles, partial);
+    hipLaunchKernelGGL(k_wgrad3_bias, dim3(slots), blk, 0, s, a.act2, a.dlogit, a.dpre1, a.dpre2, n_tiles,
+                       partial);
+    hipLaunchKernelGGL(k_reduce, dim3((kNParams + 255) / 256), dim3(256), 0, s, partial, slots, grad);
+}
+
+void launch_adam(float *theta, const float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,
+                 int step, hipStream_t s)
+{
+    const float c1 = 1.0f / (1.0f - powf(beta1, (float)step));
+    const float c2 = 1.0f / (1.0f - powf(beta2, (float)step));
+    hipLaunchKernelGGL(k_adam, dim3((kNParams + 255) / 256), dim3(256), 0, s, theta, grad, m, v, lr, beta1, beta2,
+                       eps, c1, c2);
+}
+
+}  // namespace mnet

@@ -8,6 +8,7 @@
 // run of F frames needs F+2 slabs instead of the 2F the reference computes.
 #include <chromadec/decoder.h>
 #include <chromadec/tpal_device.h>
+#include <chromadec/tpal_mask.h>
 
 #include <hip/hip_runtime_api.h>
 
@@ -19,6 +20,7 @@
 #include <vector>
 
 #include "common.h"
+#include "mask_net_state.h"
 #include "tpal_device_internal.h"
 #include "tpal_internal.h"
 
@@ -297,7 +299,21 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hi
         for (int i = 0; i < n_frames; ++i) tpal::launch_fill_border(g, plan->h_frame_jobs[i].out, 1, s);
     }
     if (plan->timing) hipEventRecord(plan->ev[0], s);
-    if (plan->kind == CHD_DEC_TRANSFORM_3D)
+    if (plan->kind == CHD_DEC_TRANSFORM_3D && plan->net != nullptr) {
+        // learned mask: tile spectra -> network -> masked inverse
+        chd_mask_net *net = plan->net;
+        const int64_t tiles = (int64_t)n_units * g.n_ty3 * g.n_tx3;
+        tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra,
+                            plan->d_spec, nullptr);
+        for (int64_t t0 = 0; t0 < tiles; t0 += net->max_tiles) {
+            const int64_t cnt = std::min<int64_t>(net->max_tiles, tiles - t0);
+            const float2 *sp = reinterpret_cast<const float2 *>(plan->d_spec) + t0 * tpal::kTileBins;
+            mnet::launch_features(sp, cnt, net->a.feat, s);
+            mnet::launch_forward(net->theta, net->f, net->a, cnt, plan->d_mask + t0 * mnet::kP, s);
+        }
+        tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply,
+                            plan->d_spec, plan->d_mask);
+    } else if (plan->kind == CHD_DEC_TRANSFORM_3D)
         tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s);
     else
         tpal::launch_tpal2d(g, plan->d_field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
@@ -419,6 +435,8 @@ void chd_tpal_plan_free(chd_tpal_plan_t *plan)
     hipFree(plan->d_slab_jobs);
     hipFree(plan->d_field_jobs);
     hipFree(plan->d_frame_jobs);
+    if (plan->d_spec) hipFree(plan->d_spec);
+    if (plan->d_mask) hipFree(plan->d_mask);
     if (plan->h_slab_jobs) hipHostFree(plan->h_slab_jobs);
     if (plan->h_field_jobs) hipHostFree(plan->h_field_jobs);
     if (plan->h_frame_jobs) hipHostFree(plan->h_frame_jobs);
@@ -550,6 +568,83 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
         }
     }
     return flush();
+}
+
+int64_t chd_tpal_tiles_per_slab(const chd_tpal_plan_t *plan)
+{
+    return (plan && plan->kind == CHD_DEC_TRANSFORM_3D) ? (int64_t)plan->g.n_ty3 * plan->g.n_tx3 : 0;
+}
+
+// Spectra and masks of a whole chunk (max_units slabs) for the learned mask.
+static chd_status_t ensure_spectra(chd_tpal_plan *plan, const char *who)
+{
+    if (plan->d_spec != nullptr) return CHD_OK;
+    const size_t tiles = (size_t)plan->max_units * plan->g.n_ty3 * plan->g.n_tx3;
+    hipError_t e = hipMalloc(&plan->d_spec, tiles * tpal::kTileBins * 2 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&plan->d_mask, tiles * mnet::kP * sizeof(float));
+    if (e != hipSuccess) {
+        hipFree(plan->d_spec);
+        plan->d_spec = nullptr;
+        plan->d_mask = nullptr;
+        return chd::fail(CHD_E_OOM, hip_msg(who, e));
+    }
+    return CHD_OK;
+}
+
+chd_status_t chd_tpal_set_mask_net(chd_tpal_plan_t *plan, chd_mask_net_t *net)
+{
+    if (plan == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_set_mask_net: null plan");
+    if (net != nullptr && plan->kind != CHD_DEC_TRANSFORM_3D)
+        return chd::fail(CHD_E_UNSUPPORTED, "chd_tpal_set_mask_net: the learned mask is for CHD_DEC_TRANSFORM_3D");
+    if (net != nullptr && net->device != plan->device)
+        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_set_mask_net: network and plan are on different devices");
+    hipSetDevice(plan->device);
+    wait_staging(plan);
+    if (net != nullptr) {
+        chd_status_t st = ensure_spectra(plan, "chd_tpal_set_mask_net");
+        if (st != CHD_OK) return st;
+    }
+    plan->net = net;
+    return CHD_OK;
+}
+
+chd_status_t chd_tpal_slab_spectra(chd_tpal_plan_t *plan, const uint16_t *d_fields, int64_t n_fields,
+                                   int64_t first_slab, int32_t n_slabs, void *d_spectra, void *stream)
+{
+    if (plan == nullptr || n_fields < 0 || n_slabs < 0 || (n_slabs > 0 && d_spectra == nullptr) ||
+        (n_fields > 0 && d_fields == nullptr))
+        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_slab_spectra: null argument");
+    if (plan->kind != CHD_DEC_TRANSFORM_3D)
+        return chd::fail(CHD_E_UNSUPPORTED, "chd_tpal_slab_spectra: 3D plans only");
+    hipSetDevice(plan->device);
+    const hipStream_t s = (hipStream_t)stream;
+    const Geom &g = plan->g;
+    const int64_t tps = (int64_t)g.n_ty3 * g.n_tx3;
+    for (int base = 0; base < n_slabs; base += plan->max_units) {
+        const int cnt = std::min(plan->max_units, n_slabs - base);
+        wait_staging(plan);
+        for (int k = 0; k < cnt; ++k) {
+            tpal::SlabJob &sj = plan->h_slab_jobs[k];
+            const int64_t f0 = 2 * (first_slab + base + k);
+            for (int z = 0; z < 8; ++z) {
+                const int64_t fi = f0 + z;
+                sj.field[z] = (fi >= 0 && fi < n_fields) ? d_fields + fi * g.field_elems : nullptr;
+                sj.offset[z] = z & 1;
+            }
+            sj.partial = nullptr;
+        }
+        hipError_t e = hipMemcpyAsync(plan->d_slab_jobs, plan->h_slab_jobs, cnt * sizeof(tpal::SlabJob),
+                                      hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
+        if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_slab_spectra: job upload", e));
+        plan->staging_pending = true;
+        void *dst = (char *)d_spectra + (size_t)base * tps * tpal::kTileBins * 2 * sizeof(float);
+        tpal::launch_tpal3d(g, plan->d_slab_jobs, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra, dst,
+                            nullptr);
+        e = hipGetLastError();
+        if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_slab_spectra: launch", e));
+    }
+    return CHD_OK;
 }
 
 chd_status_t chd_tpal_set_timing(chd_tpal_plan_t *plan, int enable)

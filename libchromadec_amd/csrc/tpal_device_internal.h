@@ -33,6 +33,10 @@ struct chd_tpal_plan {
     bool timing = false;
     double last_tile_ms = 0, last_demod_ms = 0;
     int n_tile_launches = 0, n_demod_launches = 0;
+    // learned mask (3D): the network, and tile spectra / masks of one chunk
+    struct chd_mask_net *net = nullptr;
+    void *d_spec = nullptr;
+    float *d_mask = nullptr;
 };
 
 // One chd_decode_frame window: the packed fields the reference's

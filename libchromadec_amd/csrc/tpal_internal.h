@@ -96,8 +96,16 @@ struct Tables {
 
 // Launchers (tpal_kernels.hip).
 hipError_t upload_tables(const Tables &t);
+// 3D tile kernel modes: the hard carrier-mirror mask; tile spectra out
+// (bins x = 2..6, kTileBins complex per tile); a soft per-visit mask applied
+// to stored spectra.
+enum { kTilePlain = 0, kTileSpectra = 1, kTileApply = 2 };
+constexpr int kTileBins = 5 * 32 * 8;
+// strip: tiles per work item (4, 8 or 16); max_groups > 0 caps the grid
+// (persistent workgroups), 0 launches one workgroup per item.
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
-                   hipStream_t s);
+                   hipStream_t s, int mode = kTilePlain, void *spec = nullptr, const float *mask = nullptr,
+                   int strip = kStrip3, int max_groups = 0);
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
                    const double *thr64, hipStream_t s);
 void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,

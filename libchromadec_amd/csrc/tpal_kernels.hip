@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "tpal_internal.h"
 
 namespace tpal {
@@ -187,7 +189,23 @@ struct Lds3 {
     double cs[32], sn[32];    // cos/sin(2 pi j / 32) for the FP64 re-check
 };
 
-__device__ __forceinline__ c32 &Sat(Lds3 &L, int k, int ky, int kz) { return L.u.S[k * kSk + ky * kSpad + kz]; }
+// Apply mode holds the tile's soft mask (one value per visit) as well.
+struct Lds3M : Lds3 {
+    float m[768];
+};
+template <int MODE> struct LdsFor { using T = Lds3; };
+template <> struct LdsFor<kTileApply> { using T = Lds3M; };
+
+template <class LT>
+__device__ __forceinline__ c32 &Sat(LT &L, int k, int ky, int kz) { return L.u.S[k * kSk + ky * kSpad + kz]; }
+
+// Visit whose mask value applies to bin (x = k+2, ky, kz): its own visit for
+// x = 2..4, the visit it is the carrier-mirror partner of for x = 5, 6.
+__device__ __forceinline__ int mask_visit(int k, int ky, int kz)
+{
+    if (k <= 2) return (kz * 32 + ky) * 3 + k;
+    return (((10 - kz) & 7) * 32 + ((40 - ky) & 31)) * 3 + (4 - k);
+}
 
 // Bit y of w moved to bit (8 - y) mod 32: the mirror row yr = (40 - y) % 32
 // of transform_pal_3d.cpp:291, applied to a word of 32 bins along ky.
@@ -288,33 +306,44 @@ __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
     }
 }
 
+// MODE kTilePlain: the reference's hard mask.  kTileSpectra: stop after the
+// forward transform and write each tile's bins x = 2..6 to `spec`
+// ([tile][k][ky][kz], tile = (job * n_ty3 + ty) * n_tx3 + tx).
+// kTileApply: read those bins back, weight each by the soft mask of its
+// visit (`mask`, [tile][768]) and run the inverse and overlap-add.
+// Work item = one strip of STRIP tiles of one tile row of one slab.  The
+// grid is either one workgroup per item (PERSIST false) or persistent (a
+// multiple of 8 workgroups, each looping over items).
+template <int MODE, int STRIP, bool PERSIST>
 __global__ void __launch_bounds__(64)
-tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__restrict__ thr32,
-             const double *__restrict__ thr64)
+tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const float *__restrict__ thr32,
+             const double *__restrict__ thr64, float2 *__restrict__ spec, const float *__restrict__ mask)
 {
-    __shared__ Lds3 L;
+    __shared__ typename LdsFor<MODE>::T L;
     const int lane = threadIdx.x;
+    if (lane < 32) {
+        L.cs[lane] = c_tab.cos32_d[lane];
+        L.sn[lane] = c_tab.sin32_d[lane];
+    }
     // Workgroups are dispatched round-robin over the 8 XCDs, each with its
-    // own L2.  Give each XCD a contiguous share of the work, ordered slab
+    // own L2.  Give each XCD a contiguous share of the items, ordered slab
     // fastest, then tile row, then strip: the workgroups resident on one XCD
     // then cover the same few field columns of neighbouring slabs (which
     // share 6 of their 8 fields) and neighbouring tile rows (which share 8
     // of their 16 lines), so each field line is fetched into that L2 about
     // once instead of once per tile.
-    const int total = gridDim.x;
+    const int n_strips = (g.n_tx3 + STRIP - 1) / STRIP;
+    const int total = n_jobs * g.n_ty3 * n_strips;
+    const int G = gridDim.x;
     const int xcd = blockIdx.x & 7, q = total >> 3, r = total & 7;
-    const int w = xcd * q + min(xcd, r) + (blockIdx.x >> 3);
-    const int n_jobs = total / (g.n_ty3 * g.n_strips3);
+    const int lo = xcd * q + min(xcd, r), hi = lo + q + (xcd < r ? 1 : 0);
+    const int per_xcd = (G - xcd + 7) >> 3;  // workgroups on this XCD
+    for (int w = lo + (blockIdx.x >> 3); w < hi; w += per_xcd) {
     const int job_id = w % n_jobs;
     const int rest = w / n_jobs;
     const int ty = rest % g.n_ty3;
     const int strip = rest / g.n_ty3;
     const SlabJob &job = jobs[job_id];
-
-    if (lane < 32) {
-        L.cs[lane] = c_tab.cos32_d[lane];
-        L.sn[lane] = c_tab.sin32_d[lane];
-    }
 
     const int tileY = g.first - kYT / 2 + (kYT / 2) * ty;
     const int startY = max(g.first - tileY, 0);
@@ -323,9 +352,10 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
 #pragma unroll
     for (int z = 0; z < 8; ++z) par[z] = (job.offset[z] - tileY) & 1;
 
-    const int tx_own0 = strip * kStrip3;
-    const int tx_end = min(tx_own0 + kStrip3, g.n_tx3);
-    const int tx_begin = max(tx_own0 - 1, 0);
+    const int tx_own0 = strip * STRIP;
+    const int tx_end = min(tx_own0 + STRIP, g.n_tx3);
+    // the leading tile completes the first owned columns; spectra need none
+    const int tx_begin = MODE == kTileSpectra ? tx_own0 : max(tx_own0 - 1, 0);
 
     // forward-stage role: rows (z, m) = lane and lane + 64
     const uint16_t *rowp[2];
@@ -356,8 +386,9 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
     const int opar = (job.offset[oz] - tileY) & 1;
     const int oy = 2 * om + opar;
     const bool orow_ok = (oy >= startY) && (oy < endY);
-    float *orow = job.partial + (int64_t)(zsel * 2 + (ty & 1)) * g.part_plane +
-                  (int64_t)((tileY + oy) >> 1) * g.cpitch;
+    float *orow = MODE == kTileSpectra ? nullptr
+                                       : job.partial + (int64_t)(zsel * 2 + (ty & 1)) * g.part_plane +
+                                             (int64_t)((tileY + oy) >> 1) * g.cpitch;
     float carry[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) carry[i] = 0.0f;
@@ -365,173 +396,193 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
     for (int tx = tx_begin; tx < tx_end; ++tx) {
         const int tileX = g.avs - kXT / 2 + (kXT / 2) * tx;
 
-        // ---- forward x: two rows per lane -> x bins 2..6 ----------------
-        float l1 = 0.0f;
+        const int64_t tile = ((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx;
+        if constexpr (MODE != kTileApply) {
+            // ---- forward x: two rows per lane -> x bins 2..6 ----------------
+            float l1 = 0.0f;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            float s[16];
-            if (rowp[h] != nullptr) {
-                load16(rowp[h] + tileX, g.black, s);
-            } else {
+            for (int h = 0; h < 2; ++h) {
+                float s[16];
+                if (rowp[h] != nullptr) {
+                    load16(rowp[h] + tileX, g.black, s);
+                } else {
 #pragma unroll
-                for (int n = 0; n < 16; ++n) s[n] = 0.0f;
+                    for (int n = 0; n < 16; ++n) s[n] = 0.0f;
+                }
+                c32 X[5];
+                l1 += xdft16(s, X) * roww[h];
+                const int m = (lane + 64 * h) & 15;
+#pragma unroll
+                for (int j = 0; j < 5; ++j) L.u.A[rz[h] * 5 + j][m] = c32{X[j].x * roww[h], X[j].y * roww[h]};
             }
-            c32 X[5];
-            l1 += xdft16(s, X) * roww[h];
-            const int m = (lane + 64 * h) & 15;
-#pragma unroll
-            for (int j = 0; j < 5; ++j) L.u.A[rz[h] * 5 + j][m] = c32{X[j].x * roww[h], X[j].y * roww[h]};
-        }
-        const float err = kErrK * kU * wave_sum(l1);
-        __syncthreads();
+            const float err = kErrK * kU * wave_sum(l1);
+            __syncthreads();
 
-        // ---- vertical: 16-point DFT of the data lines + parity twiddle ---
-        {
-            c32 v[16];
-            const bool act = lane < 40;
-            if (act) {
+            // ---- vertical: 16-point DFT of the data lines + parity twiddle ---
+            {
+                c32 v[16];
+                const bool act = lane < 40;
+                if (act) {
 #pragma unroll
-                for (int m = 0; m < 16; ++m) v[m] = L.u.A[lane][m];
-            }
-            __syncthreads();  // A and S share storage
-            if (act) {
-                const int z = lane / 5, k = lane - 5 * (lane / 5);
-                fft<16, -1>(v);
-                const bool odd = fy_odd;
+                    for (int m = 0; m < 16; ++m) v[m] = L.u.A[lane][m];
+                }
+                __syncthreads();  // A and S share storage
+                if (act) {
+                    const int z = lane / 5, k = lane - 5 * (lane / 5);
+                    fft<16, -1>(v);
+                    const bool odd = fy_odd;
 #pragma unroll
-                for (int ky = 0; ky < 16; ++ky) {
-                    c32 x0 = v[ky];
-                    if (odd) x0 = twiddle<-1>(x0, ky);
-                    Sat(L, k, ky, z) = x0;
-                    Sat(L, k, ky + 16, z) = odd ? c32{-x0.x, -x0.y} : x0;
+                    for (int ky = 0; ky < 16; ++ky) {
+                        c32 x0 = v[ky];
+                        if (odd) x0 = twiddle<-1>(x0, ky);
+                        Sat(L, k, ky, z) = x0;
+                        Sat(L, k, ky + 16, z) = odd ? c32{-x0.x, -x0.y} : x0;
+                    }
                 }
             }
-        }
-        __syncthreads();
+            __syncthreads();
 
-        // ---- temporal: 8-point DFT per (k, ky) ----------------------------
-        for (int c = lane; c < 160; c += 64) {
-            const int k = c >> 5, ky = c & 31;
-            c32 v[8];
+            // ---- temporal: 8-point DFT per (k, ky) ----------------------------
+            for (int c = lane; c < 160; c += 64) {
+                const int k = c >> 5, ky = c & 31;
+                c32 v[8];
 #pragma unroll
-            for (int z = 0; z < 8; ++z) v[z] = Sat(L, k, ky, z);
-            fft<8, -1>(v);
+                for (int z = 0; z < 8; ++z) v[z] = Sat(L, k, ky, z);
+                fft<8, -1>(v);
 #pragma unroll
-            for (int z = 0; z < 8; ++z) Sat(L, k, ky, z) = v[z];
-        }
-        __syncthreads();
+                for (int z = 0; z < 8; ++z) Sat(L, k, ky, z) = v[z];
+            }
+            __syncthreads();
 
-        // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
-        // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v].  Decisions are
-        // gathered by wave ballots into keep words (plane (x-2, z), bit y);
-        // a kept visit also keeps its partner, whose word is the mirror of
-        // the ballot.  Keep is the OR over visits, as in the reference.
-        // rows for a possible FP64 re-decision, fetched while the mask runs
-        uint4 r00 = make_uint4(0u, 0u, 0u, 0u), r01 = r00, r10 = r00, r11 = r00;
-        if (rowp[0] != nullptr) {
-            const uint4 *q = reinterpret_cast<const uint4 *>(rowp[0] + tileX);
-            r00 = q[0];
-            r01 = q[1];
-        }
-        if (rowp[1] != nullptr) {
-            const uint4 *q = reinterpret_cast<const uint4 *>(rowp[1] + tileX);
-            r10 = q[0];
-            r11 = q[1];
-        }
+            if constexpr (MODE == kTileSpectra) {
+                for (int i = lane; i < 1280; i += 64) {
+                    const c32 v = Sat(L, i >> 8, (i >> 3) & 31, i & 7);
+                    spec[tile * 1280 + i] = make_float2(v.x, v.y);
+                }
+                __syncthreads();
+                continue;
+            }
+
+            // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
+            // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v].  Decisions are
+            // gathered by wave ballots into keep words (plane (x-2, z), bit y);
+            // a kept visit also keeps its partner, whose word is the mirror of
+            // the ballot.  Keep is the OR over visits, as in the reference.
+            // rows for a possible FP64 re-decision, fetched while the mask runs
+            uint4 r00 = make_uint4(0u, 0u, 0u, 0u), r01 = r00, r10 = r00, r11 = r00;
+            if (rowp[0] != nullptr) {
+                const uint4 *q = reinterpret_cast<const uint4 *>(rowp[0] + tileX);
+                r00 = q[0];
+                r01 = q[1];
+            }
+            if (rowp[1] != nullptr) {
+                const uint4 *q = reinterpret_cast<const uint4 *>(rowp[1] + tileX);
+                r10 = q[0];
+                r11 = q[1];
+            }
 #pragma unroll 4
-        for (int i = 0; i < 12; ++i) {
-            const int x = vhalf ? visit_x(1, i) : visit_x(0, i);
-            const int z = vhalf ? visit_z(1, i) : visit_z(0, i);
-            const int y = vy;
-            const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
-            const int d = decide32(Sat(L, x - 2, y, z), Sat(L, xr - 2, yr, zr), err,
-                                     g.thr_uniform >= 0.0f ? g.thr_uniform : thr32[(z * 32 + y) * 3 + (x - 2)]);
-            const uint64_t kb = __ballot(d > 0);
-            const uint64_t fb = __ballot(d < 0);
-            if (vy == 0) {
-                const int w = (x - 2) * 8 + z;
-                L.bal[w] = (uint32_t)(kb >> (32 * vhalf));
-                L.flg[w] = (uint32_t)(fb >> (32 * vhalf));
-            }
-        }
-        __syncthreads();
-        // keep word (k, z) = own visits' ballot | ballot of the visit plane
-        // whose partners lie on (k, z), moved to the mirror rows
-        // (x_v = 8 - x, z_v = (10 - z) % 8); plus the self-mirror bins
-        // (x = 4, y in {4, 20}, z in {1, 5}), which are always kept.
-        constexpr uint32_t kSelf = (1u << 4) | (1u << 20);
-        if (lane < 40) {
-            const int k = lane >> 3, z = lane & 7;
-            uint32_t w = (k <= 2) ? L.bal[lane] : 0u;
-            if (k >= 2) w |= mirror_rows(L.bal[(4 - k) * 8 + ((10 - z) & 7)]);
-            if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
-            L.keep[lane] = w;
-        }
-        bool any_flag = false;
-        if (lane < 24) any_flag = (L.flg[lane] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) != 0u;
-        __syncthreads();
-
-        // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
-        // Each lane sums its two rows' FP64 row DFTs, turned by the (ky, kz)
-        // phase of the bin; a wave reduction gives the exact bin.
-        if (__any(any_flag)) {
-            const double black = (double)g.black;
-#pragma unroll 1
-            for (int wi = 0; wi < 24; ++wi) {
-                uint32_t bits = L.flg[wi];
-                if (wi == 17 || wi == 21) bits &= ~kSelf;
-                const int x = 2 + (wi >> 3), z = wi & 7;
-                const int zr = (10 - z) & 7, xr = 8 - x;
-                while (bits != 0u) {
-                    const int y = __builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    const int yr = (40 - y) & 31;
-                    const int v = (z * 32 + y) * 3 + (x - 2);
-                    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-#pragma unroll 1
-                    for (int h = 0; h < 2; ++h) {
-                        // FP64 row DFT at bins x and xr = 8 - x from the samples
-                        // (phases uniform across the wave).  With c, s the
-                        // cos/sin of 2 pi x n / 16, bin x is sum u (c - i s) and
-                        // bin 8-x is sum (-1)^n u (c + i s): split by n parity.
-                        // A row outside the tile's lines holds black: u = 0.
-                        if (rowp[h] == nullptr) continue;
-                        const uint4 q0 = h ? r10 : r00, q1 = h ? r11 : r01;
-                        const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-                        const double rw = c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]];
-                        double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
-#pragma unroll
-                        for (int n = 0; n < 16; n += 2) {
-                            const uint32_t w2 = raw[n >> 1];
-                            const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
-                            const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
-                            const int j0 = (2 * x * n) & 31, j1 = (2 * x * (n + 1)) & 31;
-                            ce = fma(u0, L.cs[j0], ce);
-                            se = fma(u0, L.sn[j0], se);
-                            co = fma(u1, L.cs[j1], co);
-                            so = fma(u1, L.sn[j1], so);
-                        }
-                        const double ra = ce + co, ia = -(se + so);
-                        const double rb = ce - co, ib = se - so;
-                        // turn by the (ky, kz) phase of the row: 4 kz z + ky y (32ths)
-                        const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
-                        const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
-                        const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
-                        a0 += ra * ca + ia * sa;
-                        a1 += ia * ca - ra * sa;
-                        b0 += rb * cb + ib * sb;
-                        b1 += ib * cb - rb * sb;
-                    }
-                    a0 = wave_sum_d(a0);
-                    a1 = wave_sum_d(a1);
-                    b0 = wave_sum_d(b0);
-                    b1 = wave_sum_d(b1);
-                    if (lane == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
-                        L.keep[(x - 2) * 8 + z] |= 1u << y;
-                        L.keep[(xr - 2) * 8 + zr] |= 1u << yr;
-                    }
+            for (int i = 0; i < 12; ++i) {
+                const int x = vhalf ? visit_x(1, i) : visit_x(0, i);
+                const int z = vhalf ? visit_z(1, i) : visit_z(0, i);
+                const int y = vy;
+                const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
+                const int d = decide32(Sat(L, x - 2, y, z), Sat(L, xr - 2, yr, zr), err,
+                                         g.thr_uniform >= 0.0f ? g.thr_uniform : thr32[(z * 32 + y) * 3 + (x - 2)]);
+                const uint64_t kb = __ballot(d > 0);
+                const uint64_t fb = __ballot(d < 0);
+                if (vy == 0) {
+                    const int w = (x - 2) * 8 + z;
+                    L.bal[w] = (uint32_t)(kb >> (32 * vhalf));
+                    L.flg[w] = (uint32_t)(fb >> (32 * vhalf));
                 }
             }
+            __syncthreads();
+            // keep word (k, z) = own visits' ballot | ballot of the visit plane
+            // whose partners lie on (k, z), moved to the mirror rows
+            // (x_v = 8 - x, z_v = (10 - z) % 8); plus the self-mirror bins
+            // (x = 4, y in {4, 20}, z in {1, 5}), which are always kept.
+            constexpr uint32_t kSelf = (1u << 4) | (1u << 20);
+            if (lane < 40) {
+                const int k = lane >> 3, z = lane & 7;
+                uint32_t w = (k <= 2) ? L.bal[lane] : 0u;
+                if (k >= 2) w |= mirror_rows(L.bal[(4 - k) * 8 + ((10 - z) & 7)]);
+                if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
+                L.keep[lane] = w;
+            }
+            bool any_flag = false;
+            if (lane < 24) any_flag = (L.flg[lane] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) != 0u;
+            __syncthreads();
+
+            // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
+            // Each lane sums its two rows' FP64 row DFTs, turned by the (ky, kz)
+            // phase of the bin; a wave reduction gives the exact bin.
+            if (__any(any_flag)) {
+                const double black = (double)g.black;
+#pragma unroll 1
+                for (int wi = 0; wi < 24; ++wi) {
+                    uint32_t bits = L.flg[wi];
+                    if (wi == 17 || wi == 21) bits &= ~kSelf;
+                    const int x = 2 + (wi >> 3), z = wi & 7;
+                    const int zr = (10 - z) & 7, xr = 8 - x;
+                    while (bits != 0u) {
+                        const int y = __builtin_ctz(bits);
+                        bits &= bits - 1u;
+                        const int yr = (40 - y) & 31;
+                        const int v = (z * 32 + y) * 3 + (x - 2);
+                        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll 1
+                        for (int h = 0; h < 2; ++h) {
+                            // FP64 row DFT at bins x and xr = 8 - x from the samples
+                            // (phases uniform across the wave).  With c, s the
+                            // cos/sin of 2 pi x n / 16, bin x is sum u (c - i s) and
+                            // bin 8-x is sum (-1)^n u (c + i s): split by n parity.
+                            // A row outside the tile's lines holds black: u = 0.
+                            if (rowp[h] == nullptr) continue;
+                            const uint4 q0 = h ? r10 : r00, q1 = h ? r11 : r01;
+                            const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+                            const double rw = c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]];
+                            double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
+#pragma unroll
+                            for (int n = 0; n < 16; n += 2) {
+                                const uint32_t w2 = raw[n >> 1];
+                                const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
+                                const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
+                                const int j0 = (2 * x * n) & 31, j1 = (2 * x * (n + 1)) & 31;
+                                ce = fma(u0, L.cs[j0], ce);
+                                se = fma(u0, L.sn[j0], se);
+                                co = fma(u1, L.cs[j1], co);
+                                so = fma(u1, L.sn[j1], so);
+                            }
+                            const double ra = ce + co, ia = -(se + so);
+                            const double rb = ce - co, ib = se - so;
+                            // turn by the (ky, kz) phase of the row: 4 kz z + ky y (32ths)
+                            const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
+                            const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
+                            const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
+                            a0 += ra * ca + ia * sa;
+                            a1 += ia * ca - ra * sa;
+                            b0 += rb * cb + ib * sb;
+                            b1 += ib * cb - rb * sb;
+                        }
+                        a0 = wave_sum_d(a0);
+                        a1 = wave_sum_d(a1);
+                        b0 = wave_sum_d(b0);
+                        b1 = wave_sum_d(b1);
+                        if (lane == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
+                            L.keep[(x - 2) * 8 + z] |= 1u << y;
+                            L.keep[(xr - 2) * 8 + zr] |= 1u << yr;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+
+        } else {
+            for (int i = lane; i < 1280; i += 64) {
+                const float2 v = spec[tile * 1280 + i];
+                Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{v.x, v.y};
+            }
+            for (int i = lane; i < 768; i += 64) L.m[i] = mask[tile * 768 + i];
             __syncthreads();
         }
 
@@ -542,7 +593,12 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
 #pragma unroll
             for (int z = 0; z < 8; ++z) {
                 const c32 t = Sat(L, k, ky, z);
-                v[z] = ((L.keep[k * 8 + z] >> ky) & 1u) ? t : c32{0.0f, 0.0f};
+                if constexpr (MODE == kTileApply) {
+                    const float mv = L.m[mask_visit(k, ky, z)];
+                    v[z] = c32{t.x * mv, t.y * mv};
+                } else {
+                    v[z] = ((L.keep[k * 8 + z] >> ky) & 1u) ? t : c32{0.0f, 0.0f};
+                }
             }
             fft<8, +1>(v);
             Sat(L, k, ky, 0) = v[0];
@@ -615,6 +671,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, const float *__rest
         }
         __syncthreads();
     }
+    if constexpr (!PERSIST) break;
+    }  // work items
 }
 
 // ---------------------------------------------------------------------------
@@ -1269,11 +1327,34 @@ namespace tpal {
 
 hipError_t upload_tables(const Tables &t) { return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof(Tables)); }
 
-void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
-                   hipStream_t s)
+template <int STRIP>
+static void launch3d_strip(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
+                           hipStream_t s, int mode, float2 *sp, const float *mask, int max_groups)
 {
-    const int blocks = n_jobs * g.n_ty3 * g.n_strips3;
-    if (blocks > 0) hipLaunchKernelGGL(tpal3d_tiles, dim3(blocks), dim3(64), 0, s, g, jobs, thr32, thr64);
+    const int items = n_jobs * g.n_ty3 * ((g.n_tx3 + STRIP - 1) / STRIP);
+    if (items <= 0) return;
+    // persistent grids are whole multiples of 8 (one share per XCD)
+    const bool persist = max_groups > 0 && max_groups < items;
+    const dim3 grid(persist ? std::max(8, max_groups & ~7) : items), blk(64);
+#define CHD_LAUNCH3D(M, P)                                                                                   \
+    hipLaunchKernelGGL((tpal3d_tiles<M, STRIP, P>), grid, blk, 0, s, g, jobs, n_jobs, thr32, thr64, sp, mask)
+    if (mode == kTileSpectra) {
+        if (persist) CHD_LAUNCH3D(kTileSpectra, true); else CHD_LAUNCH3D(kTileSpectra, false);
+    } else if (mode == kTileApply) {
+        if (persist) CHD_LAUNCH3D(kTileApply, true); else CHD_LAUNCH3D(kTileApply, false);
+    } else {
+        if (persist) CHD_LAUNCH3D(kTilePlain, true); else CHD_LAUNCH3D(kTilePlain, false);
+    }
+#undef CHD_LAUNCH3D
+}
+
+void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
+                   hipStream_t s, int mode, void *spec, const float *mask, int strip, int max_groups)
+{
+    float2 *sp = reinterpret_cast<float2 *>(spec);
+    if (strip == 4) launch3d_strip<4>(g, jobs, n_jobs, thr32, thr64, s, mode, sp, mask, max_groups);
+    else if (strip == 16) launch3d_strip<16>(g, jobs, n_jobs, thr32, thr64, s, mode, sp, mask, max_groups);
+    else launch3d_strip<kStrip3>(g, jobs, n_jobs, thr32, thr64, s, mode, sp, mask, max_groups);
 }
 
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,

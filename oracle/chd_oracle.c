@@ -270,11 +270,34 @@ static inline double sample_at(const uint16_t *row, int x, int fw, int black)
 
 enum { ZT3 = 8, YT3 = 32, XT3 = 16, XC3 = XT3 / 2 + 1 };
 
-void orc_tpal3d_filter_fields(const orc_params *p, const uint16_t *const *fields, const int32_t *offsets,
-                              int32_t nfields, int32_t start, int32_t end, const double *thr_sq,
-                              double *const *out)
+/* Soft mask in place of applyFilter (the learned mask of this library; no
+ * reference counterpart): bins x = 2..4 take their visit's mask value m[v],
+ * v = (z*32 + y)*3 + (x-2); bins x = 5, 6 take the value of the visit whose
+ * carrier-mirror partner they are, (z_ref, y_ref, 8-x); other bins are 0. */
+void orc_apply_soft_mask3d(const double *in_re, const double *in_im, double *out_re, double *out_im,
+                              const float *m)
 {
-    (void)nfields;
+    memset(out_re, 0, sizeof(double) * ZT3 * YT3 * XC3);
+    memset(out_im, 0, sizeof(double) * ZT3 * YT3 * XC3);
+    for (int z = 0; z < ZT3; z++) {
+        const int z_ref = (2 + ZT3 - z) % ZT3;
+        for (int y = 0; y < YT3; y++) {
+            const int y_ref = (8 + YT3 - y) % YT3;
+            const int bi = (z * YT3 + y) * XC3;
+            for (int x = 2; x <= 6; x++) {
+                const double mv = x <= 4 ? m[(z * YT3 + y) * 3 + (x - 2)] : m[(z_ref * YT3 + y_ref) * 3 + (6 - x)];
+                out_re[bi + x] = in_re[bi + x] * mv;
+                out_im[bi + x] = in_im[bi + x] * mv;
+            }
+        }
+    }
+}
+
+/* masks == NULL: the reference's threshold rule.  Otherwise masks holds 768
+ * values per tile, tiles in (tileZ, tileY, tileX) loop order. */
+static void tpal3d_filter(const orc_params *p, const uint16_t *const *fields, const int32_t *offsets,
+                          int32_t start, int32_t end, const double *thr_sq, const float *masks, double *const *out)
+{
     const int fw = p->field_width, fh = p->field_height;
     for (int i = 0; i < end - start; i++) memset(out[i], 0, sizeof(double) * fw * fh);
 
@@ -320,7 +343,12 @@ void orc_tpal3d_filter_fields(const orc_params *p, const uint16_t *const *fields
                 orc_r2c3d(real, ZT3, YT3, XT3, hin_re, hin_im, sre, sim);
 
                 /* applyFilter (:249-333) */
-                apply_filter(hin_re, hin_im, hout_re, hout_im, ZT3, YT3, XT3, ZT3 / 4, YT3 / 4, thr_sq);
+                if (masks != NULL) {
+                    orc_apply_soft_mask3d(hin_re, hin_im, hout_re, hout_im, masks);
+                    masks += 768;
+                } else {
+                    apply_filter(hin_re, hin_im, hout_re, hout_im, ZT3, YT3, XT3, ZT3 / 4, YT3 / 4, thr_sq);
+                }
 
                 /* inverseFFTTile (:205-240) */
                 const int startX = imax(avs - tileX, 0);
@@ -346,6 +374,14 @@ void orc_tpal3d_filter_fields(const orc_params *p, const uint16_t *const *fields
     free(win);
     free(real); free(sre); free(sim);
     free(hin_re); free(hin_im); free(hout_re); free(hout_im);
+}
+
+void orc_tpal3d_filter_fields(const orc_params *p, const uint16_t *const *fields, const int32_t *offsets,
+                              int32_t nfields, int32_t start, int32_t end, const double *thr_sq,
+                              double *const *out)
+{
+    (void)nfields;
+    tpal3d_filter(p, fields, offsets, start, end, thr_sq, NULL, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -682,9 +718,9 @@ void orc_output_u16(const orc_params *p, const double *Y, const double *U, const
 /* frame (3 x field_width x (2*field_height-1) doubles).                      */
 /* ------------------------------------------------------------------------ */
 
-int orc_decode_window(const orc_params *p, int32_t kind, const double *thr_sq, const uint16_t *const *fields,
-                      const int32_t *offsets, int32_t nfields, int32_t start, int32_t end, int32_t out_fmt,
-                      void *out, double *chroma_out, double *yuv_out)
+static int decode_window(const orc_params *p, int32_t kind, const double *thr_sq, const float *masks,
+                         const uint16_t *const *fields, const int32_t *offsets, int32_t nfields, int32_t start,
+                         int32_t end, int32_t out_fmt, void *out, double *chroma_out, double *yuv_out)
 {
     if (end - start != 2) return -1;
     const size_t fa = (size_t)p->field_width * p->field_height;
@@ -693,7 +729,7 @@ int orc_decode_window(const orc_params *p, int32_t kind, const double *thr_sq, c
     double *outs[2] = {chroma, chroma + fa};
     if (kind == 8) {
         if (start < 4 || nfields - end < 4) { free(chroma); return -2; }
-        orc_tpal3d_filter_fields(p, fields, offsets, nfields, start, end, thr_sq, outs);
+        tpal3d_filter(p, fields, offsets, start, end, thr_sq, masks, outs);
     } else if (kind == 7) {
         orc_tpal2d_filter_fields(p, fields, offsets, nfields, start, end, thr_sq, outs);
     } else {
@@ -712,6 +748,23 @@ int orc_decode_window(const orc_params *p, int32_t kind, const double *thr_sq, c
     free(yuv);
     free(chroma);
     return 0;
+}
+
+int orc_decode_window(const orc_params *p, int32_t kind, const double *thr_sq, const uint16_t *const *fields,
+                      const int32_t *offsets, int32_t nfields, int32_t start, int32_t end, int32_t out_fmt,
+                      void *out, double *chroma_out, double *yuv_out)
+{
+    return decode_window(p, kind, thr_sq, NULL, fields, offsets, nfields, start, end, out_fmt, out, chroma_out,
+                         yuv_out);
+}
+
+/* 3D decode of one frame window with a soft mask per tile (masks: 768
+ * floats per tile of the window's two Z tiles, in tile loop order). */
+int orc_decode_window_masked(const orc_params *p, const float *masks, const uint16_t *const *fields,
+                             const int32_t *offsets, int32_t nfields, int32_t start, int32_t end, int32_t out_fmt,
+                             void *out, double *chroma_out)
+{
+    return decode_window(p, 8, NULL, masks, fields, offsets, nfields, start, end, out_fmt, out, chroma_out, NULL);
 }
 
 /* Decode frames of a regularly paired sequence (first field first, field 2f

@@ -3,7 +3,7 @@
 Nothing here is part of the decode path.  Only the tests, the smoke check
 and bench.py's CPU comparison may import this module; the library computes
 the same quantities with its own HIP kernels (libchromadec_amd/csrc/
-mask_net.hip) and these functions are the yardstick they are checked
+mask_net_kernels.hip) and these functions are the yardstick they are checked
 against.
 
 The learned mask replaces the hard keep/discard rule of

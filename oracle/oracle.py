@@ -64,6 +64,9 @@ def lib():
         _lib.orc_decode_window.argtypes = [P(OrcParams), i32, P(d), P(P(C.c_uint16)), P(i32), i32, i32, i32,
                                            i32, C.c_void_p, P(d), P(d)]
         _lib.orc_decode_window.restype = C.c_int
+        _lib.orc_decode_window_masked.argtypes = [P(OrcParams), P(C.c_float), P(P(C.c_uint16)), P(i32), i32, i32,
+                                                  i32, i32, C.c_void_p, P(d)]
+        _lib.orc_decode_window_masked.restype = C.c_int
         _lib.orc_decode_sequence.argtypes = [P(OrcParams), i32, P(d), P(C.c_uint16), i32, P(i32), i32, i32,
                                              C.c_void_p, i32]
         _lib.orc_decode_sequence.restype = C.c_int
@@ -176,6 +179,24 @@ def frame_window(seq_fields: np.ndarray, frame: int, kind: int, black: int = 0x4
 def decode_frame(p: OrcParams, kind: int, seq_fields, frame, thr_sq, out_fmt=0, **kw):
     fields, offs, s, e = frame_window(seq_fields, frame, kind, p.black16b)
     return decode_window(p, kind, fields, offs, s, e, thr_sq, out_fmt, **kw)
+
+
+def decode_frame_masked(p: OrcParams, seq_fields, frame, masks, out_fmt=0):
+    """3D decode of `frame` with a soft mask per tile instead of the
+    threshold: masks [2, n_tile_rows, n_tile_cols, 768] for the window's two
+    Z tiles (slabs frame-2 and frame), visits v = (z*32 + y)*3 + (x-2)."""
+    L = lib()
+    fields, offs, s, e = frame_window(seq_fields, frame, 8, p.black16b)
+    arr, keep = _field_ptrs(fields)
+    o = np.ascontiguousarray(offs, dtype=np.int32)
+    m = np.ascontiguousarray(masks, dtype=np.float32)
+    out = np.zeros((3, p.out_height, p.out_width), dtype=np.float32 if out_fmt == 0 else np.uint16)
+    rc = L.orc_decode_window_masked(C.byref(p), m.ctypes.data_as(C.POINTER(C.c_float)), arr,
+                                    o.ctypes.data_as(C.POINTER(C.c_int32)), len(fields), s, e, out_fmt,
+                                    out.ctypes.data_as(C.c_void_p), None)
+    if rc != 0:
+        raise RuntimeError(f"orc_decode_window_masked failed: {rc}")
+    return out
 
 
 def decode_sequence(p: OrcParams, kind: int, seq_fields, frames, thr_sq, out_fmt=0, threads=0):

@@ -47,6 +47,7 @@ def test_library_exports_every_declared_symbol():
 def test_headers_compile_as_c11_and_cxx(tmp_path):
     src = tmp_path / "t.c"
     src.write_text("#include <chromadec/chromadec.h>\n#include <chromadec/tpal_device.h>\n"
+                   "#include <chromadec/tpal_mask.h>\n"
                    "int main(void){return (int)sizeof(chd_video_info_t) > 0 ? 0 : 1;}\n")
     for cmd in (["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", str(src),
                  "-o", str(tmp_path / "a.o")],
