@@ -10,6 +10,16 @@ already resident in HBM, output planes written to HBM.
 Prints one JSON line.  `roofline` is for the dominant kernel (the 3D tile
 transform), timed with HIP events on its stream; `cpu_baseline` times the
 FP64 oracle (a port of the reference algorithm) on the host.
+
+Beside the headline (plain threshold) the line carries:
+  * `learned`: the same 64 frames decoded with the packaged learned mask
+    (frames/s), and the chroma PSNR of both decodes against the ideal
+    separation (the same demodulator fed the true chroma the encoder
+    modulated) and against the encoder's U/V;
+  * `batched`: 16 sequences of seeded lengths 8-40 frames (about 384 frames;
+    bars, moving patches and luma zone plates up to 5 MHz) decoded as one
+    fixed batch, one persistent tile launch replayed from a HIP graph, with
+    the strip length and grid picked by the library's autotuner.
 """
 from __future__ import annotations
 
@@ -32,19 +42,57 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
-def make_workload(n_frames: int, seed: int = 2) -> np.ndarray:
+def make_workload(n_frames: int, seed: int = 2):
+    """Composite fields and the true modulated chroma (uint16 around 32768)."""
     from libchromadec_amd import synth
-    cache = os.path.join("/tmp", f"chd_bench_c2_{n_frames}_{seed}.npy")
+    cache = os.path.join("/tmp", f"chd_bench_c2t_{n_frames}_{seed}.npz")
     if os.path.exists(cache):
-        return np.load(cache)
+        z = np.load(cache)
+        return z["fields"], z["truth"], z["u"], z["v"]
     scene = synth.PatchScene(seed=seed)
-    fields = synth.encode_sequence([scene.frame(f) for f in range(n_frames)],
-                                   synth.EncodeOptions(snr_db=40, rng=np.random.default_rng(seed)))
+    fields, u, v, chroma = synth.encode_sequence([scene.frame(f) for f in range(n_frames)],
+                                                 synth.EncodeOptions(snr_db=40, rng=np.random.default_rng(seed)),
+                                                 return_truth=True)
+    truth = np.clip(np.rint(chroma + 32768.0), 0, 65535).astype(np.uint16)
+    u, v = u.astype(np.float32), v.astype(np.float32)
     try:
-        np.save(cache, fields)
+        np.savez(cache, fields=fields, truth=truth, u=u, v=v)
     except OSError:
         pass
-    return fields
+    return fields, truth, u, v
+
+
+def _c3_job(args):
+    from libchromadec_amd import synth
+    i, n, seed = args
+    kind = i % 3
+    if kind == 0:
+        pics = [synth.bars_gradient(f) for f in range(n)]
+    else:
+        scene = synth.PatchScene(seed=seed * 1000 + i, zone_plate=(kind == 2))
+        pics = [scene.frame(f) for f in range(n)]
+    return synth.encode_sequence(pics, synth.EncodeOptions(snr_db=40, rng=np.random.default_rng(seed * 1000 + i)))
+
+
+def make_batch_workload(seed: int = 3):
+    """16 sequences, lengths U{8..40} redrawn until the total is 384 +- 8."""
+    from concurrent.futures import ProcessPoolExecutor
+    cache = os.path.join("/tmp", f"chd_bench_c3_{seed}.npz")
+    if os.path.exists(cache):
+        z = np.load(cache)
+        return [z[f"s{i}"] for i in range(16)]
+    rng = np.random.default_rng(seed)
+    while True:
+        lens = rng.integers(8, 41, 16)
+        if abs(int(lens.sum()) - 384) <= 8:
+            break
+    with ProcessPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        seqs = list(ex.map(_c3_job, [(i, int(n), seed) for i, n in enumerate(lens)]))
+    try:
+        np.savez(cache, **{f"s{i}": q for i, q in enumerate(seqs)})
+    except OSError:
+        pass
+    return seqs
 
 
 def cpu_baseline(fields: np.ndarray, n_sample: int):
@@ -68,6 +116,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=16, help="frames for the CPU baseline (0 = skip)")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the learned-mask and batched legs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain elementwise copy of this many MiB (a known-size kernel that "
                          "scales hardware FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -80,7 +129,7 @@ def main() -> int:
     import torch
     from libchromadec_amd import abi
 
-    fields = make_workload(N_FRAMES)
+    fields, truth, tu, tv = make_workload(N_FRAMES)
     dev = torch.device("cuda:0")
     plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, pixel_format=abi.CHD_PIXEL_YUV444PS,
                                             max_batch_frames=N_FRAMES))
@@ -173,10 +222,77 @@ def main() -> int:
                               "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)}},
         "parity_max_abs_vs_oracle_frame1": parity,
     }
+    if not args.no_extras:
+        res["learned"] = learned_leg(plan, packed, fields, truth, tu, tv, out, seqs, stream, max(3, args.steps // 4))
+        res["batched"] = batched_leg(args.steps)
     if args.cpu_sample > 0:
         res["cpu_baseline"] = cpu_baseline(fields, args.cpu_sample)
     print(json.dumps(res))
     return 0
+
+
+def learned_leg(plain_plan, packed, fields, truth, tu, tv, out, seqs, stream, steps):
+    import torch
+    from libchromadec_amd import abi
+    from libchromadec_amd import mask_training as mt
+    weights = os.path.join(ROOT, "libchromadec_amd", "data", "tpal3d_mask.bin")
+    net = abi.MaskNet(max_tiles=16384)
+    net.load(weights)
+    plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, pixel_format=abi.CHD_PIXEL_YUV444PS,
+                                            max_batch_frames=N_FRAMES))
+    plan.set_mask_net(net)
+    plan.decode(seqs, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.decode(seqs, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    learned = out.cpu().numpy().reshape(N_FRAMES, 3, 576, 922)
+    plain_plan.decode(seqs, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    plain = out.cpu().numpy().reshape(N_FRAMES, 3, 576, 922)
+    ideal = mt.ideal_frames(plain_plan, fields, truth)
+    ecb, ecr = abi.color_scales()
+    return {"value": round(N_FRAMES * steps / dt, 1), "unit": "frames/s", "ms_per_step": round(1e3 * dt / steps, 3),
+            "weights": "libchromadec_amd/data/tpal3d_mask.bin",
+            "psnr_vs_ideal_separation_db": {"plain": round(mt.separation_psnr(plain, ideal), 3),
+                                            "learned": round(mt.separation_psnr(learned, ideal), 3)},
+            "psnr_vs_encoder_uv_db": {"plain": round(mt.chroma_psnr(plain, tu, tv, range(N_FRAMES), ecb, ecr), 3),
+                                      "learned": round(mt.chroma_psnr(learned, tu, tv, range(N_FRAMES), ecb, ecr), 3)}}
+
+
+def batched_leg(steps):
+    import torch
+    from libchromadec_amd import abi
+    seqs = make_batch_workload()
+    plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, pixel_format=abi.CHD_PIXEL_YUV444PS,
+                                            max_batch_frames=64))
+    dev = torch.device("cuda:0")
+    keep, runs = [], []
+    for f in seqs:
+        src = torch.from_numpy(f.view(np.int16)).to(dev)
+        pk = torch.empty((f.shape[0], plan.packed_field_elems), dtype=torch.int16, device=dev)
+        plan.pack(src.data_ptr(), f.shape[0], pk.data_ptr())
+        keep.append(pk)
+        runs.append((pk.data_ptr(), f.shape[0] // 2, 0, f.shape[0] // 2))
+    total = sum(r[3] for r in runs)
+    out = torch.empty((total, plan.frame_bytes // 4), dtype=torch.float32, device=dev)
+    batch = abi.TpalBatch(plan, runs, out.data_ptr(), autotune=True)
+    strip, groups, tile_ms = batch.config()
+    stream = torch.cuda.current_stream().cuda_stream
+    batch.run(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"value": round(total * steps / dt, 1), "unit": "frames/s", "frames_per_launch": total,
+            "sequences": len(seqs), "lengths": [r[3] for r in runs], "ms_per_graph": round(1e3 * dt / steps, 3),
+            "autotuned": {"strip_tiles": strip, "workgroups": groups,
+                          "grid": "one per work item" if groups == 0 else "persistent",
+                          "tile_launch_ms": round(tile_ms, 3)}}
 
 
 if __name__ == "__main__":

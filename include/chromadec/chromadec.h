@@ -9,6 +9,7 @@
 #include <chromadec/video.h>
 #include <chromadec/decoder.h>
 #include <chromadec/frame.h>
+#include <chromadec/dropout.h>
 #include <chromadec/pipeline.h>
 
 #endif

@@ -81,6 +81,42 @@ typedef struct chd_tpal_sequence {
 chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs,
                              void *d_out, void *stream);
 
+/* Demodulate frames whose chroma was separated elsewhere: the second half
+ * of the reference's decode, PalColour::decodeFrames with prefiltered
+ * chroma (palcolour.cpp:279-335) and OutputWriter, without the transform.
+ * d_chroma holds for fields 2k, 2k+1 of every frame field_height rows of
+ * field_width floats: chroma in 16-bit sample units, as
+ * TransformPal::filterFields outputs it (transform_pal.h:60-66); only the
+ * active samples are read.  With the true chroma of a synthetic capture this
+ * gives the ideal-separation picture a separation is measured against.
+ * Frames are fields 2k, 2k+1 of d_fields (packed); output as chd_tpal_decode. */
+chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_fields, const float *d_chroma,
+                                       int64_t n_frames, void *d_out, void *stream);
+
+/* A fixed batch: every listed frame run of every sequence decoded by one
+ * tile-transform launch (persistent workgroups over all slabs) and one
+ * demodulation launch, captured once in a HIP graph and replayed by
+ * chd_tpal_batch_run.  Job lists and the overlap-add workspace for the whole
+ * batch live in device memory; the sequences' fields and d_out must stay at
+ * the same addresses for the life of the batch (their contents may change).
+ * With autotune != 0 (3D plans) the strip length (tiles per work item: 4, 8,
+ * 16) and the grid (one workgroup per item, or 2-8 persistent workgroups
+ * per CU) are chosen by timing each on this batch.  A plan with a learned
+ * mask network (chd_tpal_set_mask_net) is captured with it.  The plan must
+ * outlive the batch; use one batch per stream at a time. */
+typedef struct chd_tpal_batch chd_tpal_batch_t;
+chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs,
+                                   void *d_out, int32_t autotune, chd_tpal_batch_t **out);
+chd_status_t chd_tpal_batch_run(chd_tpal_batch_t *batch, void *stream);
+/* Chosen launch shape: tiles per work item, workgroups of the tile launch
+ * (0 = one per item), and the tile-launch time it measured (ms, 0 without
+ * autotuning). */
+chd_status_t chd_tpal_batch_config(const chd_tpal_batch_t *batch, int32_t *strip, int32_t *groups,
+                                   double *tile_ms);
+/* Re-capture with a given shape (e.g. one an earlier autotune chose). */
+chd_status_t chd_tpal_batch_set_shape(chd_tpal_batch_t *batch, int32_t strip, int32_t groups);
+void         chd_tpal_batch_free(chd_tpal_batch_t *batch);
+
 /* Kernel-level timing of the last chd_tpal_decode on this plan when timing
  * was enabled: milliseconds in the tile-transform and demodulation kernels
  * (HIP events on the decode stream).  enable != 0 turns recording on. */

@@ -82,6 +82,15 @@ class TpalSequence(C.Structure):
                 ("first_frame", C.c_int64), ("count", C.c_int64)]
 
 
+class DropoutOpts(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("overcorrect", C.c_int), ("intra_field_only", C.c_int),
+                ("reserved", C.c_void_p * 4)]
+
+
+class DropoutStats(C.Structure):
+    _fields_ = [("corrected", C.c_int32), ("failed", C.c_int32), ("total_distance", C.c_int64)]
+
+
 FRAME_DONE_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int64, C.c_void_p)
 
 _lib = None
@@ -137,8 +146,20 @@ SIGNATURES = {
     "chd_tpal_pack_fields": (C.c_int32, [_P, _P, C.c_int64, _P, _P]),
     "chd_tpal_decode": (C.c_int32, [_P, C.POINTER(TpalSequence), C.c_int32, _P, _P]),
     "chd_tpal_set_timing": (C.c_int32, [_P, C.c_int]),
+    "chd_tpal_decode_separated": (C.c_int32, [_P, _P, _P, C.c_int64, _P, _P]),
+    "chd_tpal_batch_create": (C.c_int32, [_P, C.POINTER(TpalSequence), C.c_int32, _P, C.c_int32, C.POINTER(_P)]),
+    "chd_tpal_batch_run": (C.c_int32, [_P, _P]),
+    "chd_tpal_batch_config": (C.c_int32, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_double)]),
+    "chd_tpal_batch_set_shape": (C.c_int32, [_P, C.c_int32, C.c_int32]),
+    "chd_tpal_batch_free": (None, [_P]),
     "chd_tpal_last_timing": (C.c_int32, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    # dropout.h
+    "chd_decoder_set_dropout": (C.c_int32, [_P, C.POINTER(DropoutOpts)]),
+    "chd_decoder_get_last_dropout_stats": (C.c_int32, [_P, C.POINTER(DropoutStats)]),
+    "chd_video_correct_dropouts": (C.c_int32, [_P, C.c_int64, C.POINTER(DropoutOpts), _P, _P,
+                                               C.POINTER(DropoutStats)]),
     # tpal_mask.h
     "chd_mask_net_create": (C.c_int32, [C.c_int32, C.c_int64, C.POINTER(_P)]),
     "chd_mask_net_free": (None, [_P]),
@@ -217,6 +238,17 @@ class Video:
         check(lib().chd_video_get_info(self.h, C.byref(i)), "chd_video_get_info")
         return i
 
+    def correct_dropouts(self, frame_index: int, first: np.ndarray, second: np.ndarray, overcorrect=False,
+                         intra_field_only=False):
+        """Conceal the frame's dropouts in place in host copies of its fields."""
+        assert first.dtype == np.uint16 and second.dtype == np.uint16
+        assert first.flags.c_contiguous and second.flags.c_contiguous
+        o = DropoutOpts(1, int(overcorrect), int(intra_field_only))
+        st = DropoutStats()
+        check(lib().chd_video_correct_dropouts(self.h, frame_index, C.byref(o), first.ctypes.data, second.ctypes.data,
+                                               C.byref(st)), "chd_video_correct_dropouts")
+        return st.corrected, st.failed, st.total_distance
+
     def close(self):
         if self.h:
             lib().chd_video_free(self.h)
@@ -286,6 +318,16 @@ class Decoder:
         else:
             st = L.chd_decoder_set_option_str(self.h, name.encode(), str(value).encode())
         check(st, f"chd_decoder_set_option({name})")
+
+    def set_dropout(self, enabled=True, overcorrect=False, intra_field_only=False):
+        o = DropoutOpts(int(enabled), int(overcorrect), int(intra_field_only))
+        check(lib().chd_decoder_set_dropout(self.h, C.byref(o)), "chd_decoder_set_dropout")
+        return self
+
+    def last_dropout_stats(self):
+        st = DropoutStats()
+        check(lib().chd_decoder_get_last_dropout_stats(self.h, C.byref(st)), "chd_decoder_get_last_dropout_stats")
+        return st.corrected, st.failed, st.total_distance
 
     def commit(self):
         check(lib().chd_decoder_commit(self.h), "chd_decoder_commit")
@@ -379,6 +421,10 @@ class TpalPlan:
         arr = (TpalSequence * len(seqs))(*[TpalSequence(*s) for s in seqs])
         check(lib().chd_tpal_decode(self.h, arr, len(seqs), d_out, stream), "chd_tpal_decode")
 
+    def decode_separated(self, d_fields: int, d_chroma: int, n_frames: int, d_out: int, stream: int = 0):
+        check(lib().chd_tpal_decode_separated(self.h, d_fields, d_chroma, n_frames, d_out, stream),
+              "chd_tpal_decode_separated")
+
     def set_timing(self, on: bool):
         check(lib().chd_tpal_set_timing(self.h, int(on)), "chd_tpal_set_timing")
 
@@ -454,6 +500,39 @@ class MaskNet:
     def close(self):
         if self.h:
             lib().chd_mask_net_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TpalBatch:
+    """chd_tpal_batch_*: a fixed multi-sequence decode captured in a HIP graph."""
+
+    def __init__(self, plan: TpalPlan, seqs, d_out: int, autotune: bool = True):
+        arr = (TpalSequence * len(seqs))(*[TpalSequence(*q) for q in seqs])
+        self.h = C.c_void_p()
+        self._plan = plan
+        check(lib().chd_tpal_batch_create(plan.h, arr, len(seqs), d_out, int(autotune), C.byref(self.h)),
+              "chd_tpal_batch_create")
+
+    def run(self, stream: int = 0):
+        check(lib().chd_tpal_batch_run(self.h, stream), "chd_tpal_batch_run")
+
+    def config(self):
+        a, b, t = C.c_int32(), C.c_int32(), C.c_double()
+        check(lib().chd_tpal_batch_config(self.h, C.byref(a), C.byref(b), C.byref(t)), "chd_tpal_batch_config")
+        return a.value, b.value, t.value
+
+    def set_shape(self, strip: int, groups: int):
+        check(lib().chd_tpal_batch_set_shape(self.h, strip, groups), "chd_tpal_batch_set_shape")
+
+    def close(self):
+        if self.h:
+            lib().chd_tpal_batch_free(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

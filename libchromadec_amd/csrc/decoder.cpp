@@ -21,6 +21,9 @@
 #include <algorithm>
 #include <cmath>
 #include <condition_variable>
+#include <deque>
+#include <map>
+#include <tuple>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -31,6 +34,7 @@
 #include <vector>
 
 #include "common.h"
+#include "dropout_internal.h"
 #include "handles.h"
 #include "tpal_device_internal.h"
 
@@ -141,6 +145,7 @@ struct Worker {
     uint16_t *h_stage = nullptr;        // pinned: 14 fields
     void *d_out = nullptr;
     int device = 0;
+    uint64_t seen_revision = 0;         // dropout options the cached fields were corrected with
 
     ~Worker()
     {
@@ -179,6 +184,22 @@ struct chd_decoder {
     chd_tpal_params_t params{};
     std::vector<double> thresholds;
     std::string maskWeights;            // non-empty: learned mask from this file
+
+    // dropout concealment (dropout.h)
+    std::mutex dropMu;
+    chd_dropout_opts_t dropoutOpts{};
+    bool dropoutSet = false;
+    uint64_t dropoutRevision = 1;
+    chd_dropout_stats_t lastStats{};
+    chd::dropout::Params dropParams;
+    struct Corrected {
+        std::vector<uint16_t> first, second;
+        chd::dropout::Stats stats;
+    };
+    // corrected frames by (first field, second field, options revision), so
+    // the frames of overlapping windows are corrected once
+    std::map<std::tuple<int32_t, int32_t, uint64_t>, std::shared_ptr<const Corrected>> corrected;
+    std::deque<std::tuple<int32_t, int32_t, uint64_t>> correctedOrder;
     std::mutex workersMu;
     std::vector<std::unique_ptr<Worker>> workers;
 
@@ -240,11 +261,13 @@ Worker *get_worker(chd_decoder_t *d, size_t idx)
 
 // Make `number` (1-based) resident in the worker's field cache and return its
 // packed device pointer.  `pinned` marks slots the current window uses.
+// Device copy of field `number` (1-based), read from the file, or, when
+// `corrected` is given, that host copy (already shifted and concealed).
 const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool shifted, int stage_idx,
-                            std::vector<int> &pinned, chd_status_t &st)
+                            std::vector<int> &pinned, chd_status_t &st, const uint16_t *corrected = nullptr)
 {
     const tpal::Geom &g = w.plan->g;
-    const int64_t key = (int64_t)number * 2 + (shifted ? 1 : 0);
+    const int64_t key = ((int64_t)number * 2 + (shifted ? 1 : 0)) * 2 + (corrected ? 1 : 0);
     w.tick++;
     for (int s = 0; s < kCacheSlots; ++s) {
         if (w.slot_key[s] == key) {
@@ -259,13 +282,15 @@ const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool sh
         if (victim < 0 || w.slot_tick[s] < w.slot_tick[victim]) victim = s;
     }
     uint16_t *stage = w.h_stage + (size_t)stage_idx * g.fw * g.fh;
-    if (!d->video->tbc.read_field(number, stage)) {
+    if (corrected != nullptr) {
+        std::memcpy(stage, corrected, (size_t)g.fw * g.fh * sizeof(uint16_t));
+    } else if (!d->video->tbc.read_field(number, stage)) {
         st = fail(CHD_E_IO, "chd_decode_frame: failed to read field " + std::to_string(number));
         return nullptr;
     }
     uint16_t *dst = w.d_slots + (size_t)victim * g.field_elems + g.lmargin;
     hipError_t e;
-    if (!shifted) {
+    if (!shifted || corrected != nullptr) {
         e = hipMemcpy2DAsync(dst, g.pitch * 2, stage, g.fw * 2, g.fw * 2, g.fh, hipMemcpyHostToDevice, w.stream);
     } else {
         // SourceField::shiftSecondField (source_field.cpp:51-66): drop the
@@ -287,6 +312,70 @@ const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool sh
     w.slot_tick[victim] = w.tick;
     pinned.push_back(victim);
     return w.d_slots + (size_t)victim * g.field_elems;
+}
+
+chd::dropout::Params dropout_params(const chd::VideoParams &vp)
+{
+    chd::dropout::Params p;
+    p.field_width = vp.fieldWidth;
+    p.field_height = vp.fieldHeight;
+    p.colour_burst_end = vp.colourBurstEnd;
+    p.active_video_end = vp.activeVideoEnd;
+    p.first_active_frame_line = vp.firstActiveFrameLine;
+    p.last_active_frame_line = vp.lastActiveFrameLine;
+    p.system = (int32_t)vp.system;
+    return p;
+}
+
+std::vector<chd::dropout::Run> runs_of(const chd::DropOuts &d)
+{
+    std::vector<chd::dropout::Run> r(d.startx.size());
+    for (size_t i = 0; i < r.size(); ++i) r[i] = {d.startx[i], d.endx[i], d.fieldLine[i]};
+    return r;
+}
+
+// A frame's fields with their dropouts concealed (c_api_decoder.cpp:
+// 238-310): read, shift the second field of a subcarrier-locked PAL source
+// with its dropouts (source_field.cpp:51-66), correct.
+std::shared_ptr<const chd_decoder::Corrected> corrected_frame(chd_decoder_t *d, int32_t f1, int32_t f2, bool shift,
+                                                              const chd_dropout_opts_t &opts, uint64_t rev,
+                                                              const chd::dropout::Params &params, chd_status_t &st)
+{
+    const auto key = std::make_tuple(f1, f2, rev);
+    {
+        std::lock_guard<std::mutex> lk(d->dropMu);
+        auto it = d->corrected.find(key);
+        if (it != d->corrected.end()) return it->second;
+    }
+    const chd::VideoParams &vp = d->video->meta.vp;
+    const size_t n = (size_t)vp.fieldWidth * vp.fieldHeight;
+    auto c = std::make_shared<chd_decoder::Corrected>();
+    c->first.resize(n);
+    c->second.resize(n);
+    if (!d->video->tbc.read_field(f1, c->first.data()) || !d->video->tbc.read_field(f2, c->second.data())) {
+        st = fail(CHD_E_IO, "chd_decode_frame: failed to read fields " + std::to_string(f1) + "/" + std::to_string(f2));
+        return nullptr;
+    }
+    std::vector<chd::dropout::Run> r1 = runs_of(d->video->meta.field(f1).dropOuts);
+    std::vector<chd::dropout::Run> r2 = runs_of(d->video->meta.field(f2).dropOuts);
+    if (shift) {
+        std::memmove(c->second.data(), c->second.data() + 2, (n - 2) * sizeof(uint16_t));
+        c->second[n - 2] = c->second[n - 1] = (uint16_t)vp.black16bIre;
+        std::vector<chd::dropout::Run> s;
+        for (const auto &r : r2)
+            if (r.endx - 2 > 0) s.push_back({std::max(r.startx - 2, 0), r.endx - 2, r.field_line});
+        r2.swap(s);
+    }
+    chd::dropout::correct_frame(params, c->first.data(), c->second.data(), r1, r2, opts.overcorrect != 0,
+                                opts.intra_field_only != 0, &c->stats);
+    std::lock_guard<std::mutex> lk(d->dropMu);
+    d->corrected[key] = c;
+    d->correctedOrder.push_back(key);
+    while (d->correctedOrder.size() > 48) {
+        d->corrected.erase(d->correctedOrder.front());
+        d->correctedOrder.pop_front();
+    }
+    return c;
 }
 
 // decodeFrameLocked (c_api_decoder.cpp:183-332, 461-515) on worker `w`.
@@ -312,6 +401,22 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
     TpalWindow win;
     std::memset(&win, 0, sizeof(win));
     std::vector<int> pinned;
+    chd_dropout_opts_t dopts{};
+    uint64_t drev = 0;
+    {
+        std::lock_guard<std::mutex> lk(d->dropMu);
+        if (d->dropoutSet && d->dropoutOpts.enabled) {
+            dopts = d->dropoutOpts;
+            drev = d->dropoutRevision;
+        }
+    }
+    if (drev != w.seen_revision) {
+        // cached device copies of corrected fields belong to other options
+        for (int s = 0; s < kCacheSlots; ++s)
+            if (w.slot_key[s] >= 0 && (w.slot_key[s] & 1)) w.slot_key[s] = -1;
+        w.seen_revision = drev;
+    }
+    chd::dropout::Stats frameStats;
     int32_t frameNumber = firstFrameNumber - behind;
     for (int32_t i = 0; i < nwin; i += 2, ++frameNumber) {
         bool blank = frameNumber < 1 || frameNumber > numFrames;
@@ -337,9 +442,19 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
             win.offset[i + 1] = 1;
             blank = true;
         }
+        const bool hasDrops = !metaBad && (!meta.field(f1).dropOuts.empty() || !meta.field(f2).dropOuts.empty());
         if (blank) {
             win.field[i] = nullptr;
             win.field[i + 1] = nullptr;
+        } else if (drev != 0 && hasDrops) {
+            // correct every frame of the window (c_api_decoder.cpp:297-315)
+            auto c = corrected_frame(d, f1, f2, shiftSecond, dopts, drev, d->dropParams, st);
+            if (st != CHD_OK) return st;
+            if (i == 2 * behind) frameStats = c->stats;
+            win.field[i] = cache_field(d, w, f1, false, i, pinned, st, c->first.data());
+            if (st != CHD_OK) return st;
+            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, i + 1, pinned, st, c->second.data());
+            if (st != CHD_OK) return st;
         } else {
             win.field[i] = cache_field(d, w, f1, false, i, pinned, st);
             if (st != CHD_OK) return st;
@@ -352,6 +467,12 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
     win.out = w.d_out;
     st = tpal_decode_windows(w.plan, &win, 1, w.stream);
     if (st != CHD_OK) return st;
+    {
+        std::lock_guard<std::mutex> lk(d->dropMu);
+        d->lastStats.corrected = frameStats.corrected;
+        d->lastStats.failed = frameStats.failed;
+        d->lastStats.total_distance = frameStats.total_distance;
+    }
 
     auto frame = std::make_unique<chd_frame>();
     frame->format = d->format;
@@ -549,6 +670,7 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
 
     d->reverseFieldOrder = reverse;
     d->vp = vp;
+    d->dropParams = dropout_params(vp);
     d->format = fmt;
     d->outW = ow;
     d->outH = oh;
@@ -815,5 +937,50 @@ chd_status_t chd_frame_get_plane_float(const chd_frame_t *f, chd_plane_t p, cons
 }
 
 void chd_frame_free(chd_frame_t *f) { delete f; }
+
+chd_status_t chd_decoder_set_dropout(chd_decoder_t *d, const chd_dropout_opts_t *opts)
+{
+    if (d == nullptr || opts == nullptr) return fail(CHD_E_INVALID_ARG, "chd_decoder_set_dropout: null argument");
+    std::lock_guard<std::mutex> lk(d->dropMu);
+    d->dropoutOpts = *opts;
+    d->dropoutSet = true;
+    d->dropoutRevision += 1;
+    d->corrected.clear();
+    d->correctedOrder.clear();
+    return CHD_OK;
+}
+
+chd_status_t chd_decoder_get_last_dropout_stats(const chd_decoder_t *d, chd_dropout_stats_t *out)
+{
+    if (d == nullptr || out == nullptr)
+        return fail(CHD_E_INVALID_ARG, "chd_decoder_get_last_dropout_stats: null argument");
+    std::lock_guard<std::mutex> lk(const_cast<chd_decoder_t *>(d)->dropMu);
+    *out = d->lastStats;
+    return CHD_OK;
+}
+
+chd_status_t chd_video_correct_dropouts(const chd_video_t *video, int64_t frame_index, const chd_dropout_opts_t *opts,
+                                        uint16_t *first_field, uint16_t *second_field, chd_dropout_stats_t *stats)
+{
+    if (video == nullptr || opts == nullptr || first_field == nullptr || second_field == nullptr)
+        return fail(CHD_E_INVALID_ARG, "chd_video_correct_dropouts: null argument");
+    const chd::Metadata &meta = video->meta;
+    const bool fff = meta.isFirstFieldFirst;
+    if (frame_index < 0 || frame_index >= meta.numberOfFrames(fff))
+        return fail(CHD_E_OUT_OF_RANGE, "chd_video_correct_dropouts: frame_index out of range");
+    const int32_t f1 = meta.fieldNumber((int32_t)frame_index + 1, false, fff);
+    const int32_t f2 = meta.fieldNumber((int32_t)frame_index + 1, true, fff);
+    chd::dropout::Stats st;
+    if (opts->enabled)
+        chd::dropout::correct_frame(dropout_params(meta.vp), first_field, second_field, runs_of(meta.field(f1).dropOuts),
+                                    runs_of(meta.field(f2).dropOuts), opts->overcorrect != 0,
+                                    opts->intra_field_only != 0, &st);
+    if (stats) {
+        stats->corrected = st.corrected;
+        stats->failed = st.failed;
+        stats->total_distance = st.total_distance;
+    }
+    return CHD_OK;
+}
 
 }  // extern "C"

@@ -647,6 +647,353 @@ chd_status_t chd_tpal_slab_spectra(chd_tpal_plan_t *plan, const uint16_t *d_fiel
     return CHD_OK;
 }
 
+chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_fields, const float *d_chroma,
+                                       int64_t n_frames, void *d_out, void *stream)
+{
+    if (plan == nullptr || n_frames < 0 || (n_frames > 0 && (d_fields == nullptr || d_chroma == nullptr ||
+                                                            d_out == nullptr)))
+        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_decode_separated: null argument");
+    hipSetDevice(plan->device);
+    const hipStream_t s = (hipStream_t)stream;
+    const Geom &g = plan->g;
+    const int64_t fb = frame_bytes(plan);
+    const int per_chunk = std::max(1, std::min(plan->max_frames, plan->max_units / 2));
+    const bool border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
+    for (int64_t base = 0; base < n_frames; base += per_chunk) {
+        const int cnt = (int)std::min<int64_t>(per_chunk, n_frames - base);
+        wait_staging(plan);
+        tpal::launch_chroma_to_planes(g, d_chroma + base * 2 * (int64_t)g.fw * g.fh, plan->d_partials,
+                                      (int64_t)plan->unit_floats, 2 * cnt, s);
+        for (int i = 0; i < cnt; ++i) {
+            tpal::FrameJob &fj = plan->h_frame_jobs[i];
+            std::memset(&fj, 0, sizeof(fj));
+            fj.out = (char *)d_out + (base + i) * fb;
+            for (int fsel = 0; fsel < 2; ++fsel) {
+                fj.field[fsel] = d_fields + (2 * (base + i) + fsel) * g.field_elems;
+                fj.offset[fsel] = fsel;
+                fj.chroma[fsel][0] = plan->d_partials + (size_t)(2 * i + fsel) * plan->unit_floats;
+            }
+            fj.nchroma = 1;
+            if (border) tpal::launch_fill_border(g, fj.out, 1, s);
+        }
+        hipError_t e = hipMemcpyAsync(plan->d_frame_jobs, plan->h_frame_jobs, cnt * sizeof(tpal::FrameJob),
+                                      hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
+        if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: job upload", e));
+        plan->staging_pending = true;
+        tpal::launch_demod(g, plan->d_frame_jobs, cnt, plan->d_sine, plan->d_cosine, plan->d_cfilt, plan->n_lblocks,
+                           s);
+        e = hipGetLastError();
+        if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: launch", e));
+    }
+    return CHD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed batches: all jobs resident, one tile launch, one demod launch, graph.
+
+}  // extern "C"
+
+struct chd_tpal_batch {
+    chd_tpal_plan *plan = nullptr;
+    int n_units = 0, n_frames = 0;
+    tpal::SlabJob *d_slab = nullptr;
+    tpal::FieldJob2D *d_field = nullptr;
+    tpal::FrameJob *d_frame = nullptr;
+    float *d_partials = nullptr;
+    void *d_out = nullptr;
+    bool border = false;
+    int strip = tpal::kStrip3, groups = 0;
+    double tile_ms = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *d_spec = nullptr;        // learned mask: spectra and masks of chunk_units slabs
+    float *d_mask = nullptr;
+    int chunk_units = 0;
+};
+
+namespace {
+
+constexpr int kBatchLearnedChunk = 16;   // slabs per spectra/network/apply round
+
+void batch_release(chd_tpal_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->cap) hipStreamDestroy(b->cap);
+    void *ptrs[] = {b->d_slab, b->d_field, b->d_frame, b->d_partials, b->d_spec, b->d_mask};
+    for (void *q : ptrs)
+        if (q) hipFree(q);
+}
+
+void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
+{
+    chd_tpal_plan *plan = b->plan;
+    const Geom &g = plan->g;
+    if (plan->kind != CHD_DEC_TRANSFORM_3D) {
+        tpal::launch_tpal2d(g, b->d_field, b->n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
+        return;
+    }
+    if (plan->net == nullptr) {
+        tpal::launch_tpal3d(g, b->d_slab, b->n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTilePlain, nullptr,
+                            nullptr, strip, groups);
+        return;
+    }
+    chd_mask_net *net = plan->net;
+    const int64_t tps = (int64_t)g.n_ty3 * g.n_tx3;
+    for (int u0 = 0; u0 < b->n_units; u0 += b->chunk_units) {
+        const int cnt = std::min(b->chunk_units, b->n_units - u0);
+        tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra, b->d_spec,
+                            nullptr, strip, groups);
+        const int64_t tiles = cnt * tps;
+        for (int64_t t0 = 0; t0 < tiles; t0 += net->max_tiles) {
+            const int64_t n = std::min<int64_t>(net->max_tiles, tiles - t0);
+            const float2 *sp = reinterpret_cast<const float2 *>(b->d_spec) + t0 * tpal::kTileBins;
+            mnet::launch_features(sp, n, net->a.feat, s);
+            mnet::launch_forward(net->theta, net->f, net->a, n, b->d_mask + t0 * mnet::kP, s);
+        }
+        tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply, b->d_spec,
+                            b->d_mask, strip, groups);
+    }
+}
+
+void batch_all(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
+{
+    chd_tpal_plan *plan = b->plan;
+    if (b->border) tpal::launch_fill_border(plan->g, b->d_out, b->n_frames, s);
+    batch_tiles(b, s, strip, groups);
+    tpal::launch_demod(plan->g, b->d_frame, b->n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
+                       plan->n_lblocks, s);
+}
+
+hipError_t batch_capture(chd_tpal_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        batch_all(b, b->cap, b->strip, b->groups);
+        const hipError_t le = hipGetLastError();
+        e = hipStreamEndCapture(b->cap, &b->graph);
+        if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs,
+                                   void *d_out, int32_t autotune, chd_tpal_batch_t **out)
+{
+    static const char *W = "chd_tpal_batch_create";
+    if (plan == nullptr || out == nullptr || n_seqs <= 0 || seqs == nullptr || d_out == nullptr)
+        return chd::fail(CHD_E_INVALID_ARG, std::string(W) + ": null argument or no sequences");
+    for (int i = 0; i < n_seqs; ++i) {
+        const chd_tpal_sequence_t &q = seqs[i];
+        if (q.count < 0 || q.first_frame < 0 || q.first_frame + q.count > q.n_frames ||
+            (q.count > 0 && q.d_fields == nullptr))
+            return chd::fail(CHD_E_OUT_OF_RANGE, std::string(W) + ": frame run outside its sequence");
+    }
+    hipSetDevice(plan->device);
+    wait_staging(plan);
+    const Geom &g = plan->g;
+    const bool is3d = plan->kind == CHD_DEC_TRANSFORM_3D;
+    const int64_t fb = frame_bytes(plan);
+    std::vector<tpal::SlabJob> slabs;
+    std::vector<tpal::FieldJob2D> fjobs;
+    std::vector<tpal::FrameJob> frames;
+    // partial offsets are assigned first; device pointers are added below
+    for (int si = 0; si < n_seqs; ++si) {
+        const chd_tpal_sequence_t &q = seqs[si];
+        if (q.count == 0) continue;
+        auto field_ptr = [&](int64_t fi) -> const uint16_t * {
+            return (fi >= 0 && fi < 2 * q.n_frames) ? q.d_fields + fi * g.field_elems : nullptr;
+        };
+        const size_t unit0 = is3d ? slabs.size() : fjobs.size();
+        if (is3d) {
+            for (int64_t k = 0; k < q.count + 2; ++k) {
+                tpal::SlabJob sj{};
+                const int64_t slab = q.first_frame - 2 + k;
+                for (int z = 0; z < 8; ++z) {
+                    sj.field[z] = field_ptr(2 * slab + z);
+                    sj.offset[z] = z & 1;
+                }
+                sj.partial = reinterpret_cast<float *>((unit0 + k) * plan->unit_floats);  // offset for now
+                slabs.push_back(sj);
+            }
+        } else {
+            for (int64_t k = 0; k < 2 * q.count; ++k) {
+                tpal::FieldJob2D jj{};
+                jj.field = field_ptr(2 * q.first_frame + k);
+                jj.offset = (int32_t)(k & 1);
+                jj.partial = reinterpret_cast<float *>((unit0 + k) * plan->unit_floats);
+                fjobs.push_back(jj);
+            }
+        }
+        for (int64_t i = 0; i < q.count; ++i) {
+            tpal::FrameJob fj{};
+            fj.out = reinterpret_cast<void *>((int64_t)frames.size() * fb);  // offset for now
+            for (int fsel = 0; fsel < 2; ++fsel) {
+                fj.field[fsel] = field_ptr(2 * (q.first_frame + i) + fsel);
+                fj.offset[fsel] = fsel;
+                if (is3d) {
+                    const size_t a = (unit0 + i) * plan->unit_floats + (2 + fsel) * 2 * g.part_plane;
+                    const size_t bb = (unit0 + i + 2) * plan->unit_floats + fsel * 2 * g.part_plane;
+                    fj.chroma[fsel][0] = reinterpret_cast<float *>(a);
+                    fj.chroma[fsel][1] = reinterpret_cast<float *>(a + g.part_plane);
+                    fj.chroma[fsel][2] = reinterpret_cast<float *>(bb);
+                    fj.chroma[fsel][3] = reinterpret_cast<float *>(bb + g.part_plane);
+                } else {
+                    const size_t a = (unit0 + 2 * i + fsel) * plan->unit_floats;
+                    fj.chroma[fsel][0] = reinterpret_cast<float *>(a);
+                    fj.chroma[fsel][1] = reinterpret_cast<float *>(a + g.part_plane);
+                }
+            }
+            fj.nchroma = is3d ? 4 : 2;
+            frames.push_back(fj);
+        }
+    }
+    if (frames.empty()) return chd::fail(CHD_E_INVALID_ARG, std::string(W) + ": no frames to decode");
+    auto *b = new chd_tpal_batch();
+    b->plan = plan;
+    b->n_units = (int)(is3d ? slabs.size() : fjobs.size());
+    b->n_frames = (int)frames.size();
+    b->d_out = d_out;
+    b->border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
+    hipError_t e = hipMalloc((void **)&b->d_partials, (size_t)b->n_units * plan->unit_floats * sizeof(float));
+    auto rebase = [&](const float *off) { return b->d_partials + reinterpret_cast<size_t>(off); };
+    if (e == hipSuccess) {
+        for (auto &sj : slabs) sj.partial = rebase(sj.partial);
+        for (auto &jj : fjobs) jj.partial = rebase(jj.partial);
+        for (auto &fj : frames) {
+            fj.out = (char *)d_out + reinterpret_cast<int64_t>(fj.out);
+            for (int fsel = 0; fsel < 2; ++fsel)
+                for (int k = 0; k < fj.nchroma; ++k) fj.chroma[fsel][k] = rebase(fj.chroma[fsel][k]);
+        }
+    }
+    if (e == hipSuccess && is3d) e = hipMalloc((void **)&b->d_slab, slabs.size() * sizeof(tpal::SlabJob));
+    if (e == hipSuccess && !is3d) e = hipMalloc((void **)&b->d_field, fjobs.size() * sizeof(tpal::FieldJob2D));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_frame, frames.size() * sizeof(tpal::FrameJob));
+    if (e == hipSuccess && is3d)
+        e = hipMemcpy(b->d_slab, slabs.data(), slabs.size() * sizeof(tpal::SlabJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !is3d)
+        e = hipMemcpy(b->d_field, fjobs.data(), fjobs.size() * sizeof(tpal::FieldJob2D), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_frame, frames.data(), frames.size() * sizeof(tpal::FrameJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess && is3d && plan->net != nullptr) {
+        b->chunk_units = std::min(kBatchLearnedChunk, b->n_units);
+        const size_t tiles = (size_t)b->chunk_units * g.n_ty3 * g.n_tx3;
+        e = hipMalloc(&b->d_spec, tiles * tpal::kTileBins * 2 * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_mask, tiles * mnet::kP * sizeof(float));
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        batch_release(b);
+        delete b;
+        return chd::fail(CHD_E_OOM, hip_msg(W, e));
+    }
+
+    // autotune: strip length x grid (one workgroup per item, or persistent)
+    if (autotune && is3d) {
+        int cus = 0;
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device);
+        if (cus <= 0) cus = 256;
+        const int strips[3] = {4, 8, 16};
+        const int per_cu[6] = {0, 2, 3, 4, 6, 8};
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        e = hipEventCreate(&t0);
+        if (e == hipSuccess) e = hipEventCreate(&t1);
+        double best = 1e30;
+        for (int si = 0; si < 3 && e == hipSuccess; ++si) {
+            for (int pi = 0; pi < 6 && e == hipSuccess; ++pi) {
+                const int groups = per_cu[pi] * cus;
+                batch_tiles(b, b->cap, strips[si], groups);   // warm-up
+                double t = 1e30;
+                for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+                    hipEventRecord(t0, b->cap);
+                    batch_tiles(b, b->cap, strips[si], groups);
+                    hipEventRecord(t1, b->cap);
+                    e = hipEventSynchronize(t1);
+                    float ms = 0.0f;
+                    hipEventElapsedTime(&ms, t0, t1);
+                    t = std::min(t, (double)ms);
+                }
+                if (e == hipSuccess && t < best) {
+                    best = t;
+                    b->strip = strips[si];
+                    b->groups = groups;
+                }
+            }
+        }
+        if (t0) hipEventDestroy(t0);
+        if (t1) hipEventDestroy(t1);
+        b->tile_ms = best < 1e30 ? best : 0.0;
+        if (e != hipSuccess) {
+            batch_release(b);
+            delete b;
+            return chd::fail(CHD_E_INTERNAL, hip_msg(W, e));
+        }
+    }
+
+    // capture the whole batch once
+    e = batch_capture(b);
+    if (e != hipSuccess) {
+        batch_release(b);
+        delete b;
+        return chd::fail(CHD_E_INTERNAL, hip_msg(W, e));
+    }
+    *out = b;
+    return CHD_OK;
+}
+
+chd_status_t chd_tpal_batch_run(chd_tpal_batch_t *batch, void *stream)
+{
+    if (batch == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_batch_run: null batch");
+    hipSetDevice(batch->plan->device);
+    const hipError_t e = hipGraphLaunch(batch->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_batch_run", e));
+    return CHD_OK;
+}
+
+chd_status_t chd_tpal_batch_config(const chd_tpal_batch_t *batch, int32_t *strip, int32_t *groups, double *tile_ms)
+{
+    if (batch == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_batch_config: null batch");
+    if (strip) *strip = batch->strip;
+    if (groups) *groups = batch->groups;
+    if (tile_ms) *tile_ms = batch->tile_ms;
+    return CHD_OK;
+}
+
+chd_status_t chd_tpal_batch_set_shape(chd_tpal_batch_t *batch, int32_t strip, int32_t groups)
+{
+    if (batch == nullptr || (strip != 4 && strip != 8 && strip != 16) || groups < 0)
+        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_batch_set_shape: strip must be 4, 8 or 16 and groups >= 0");
+    hipSetDevice(batch->plan->device);
+    hipStreamSynchronize(batch->cap);
+    batch->strip = strip;
+    batch->groups = groups;
+    batch->tile_ms = 0.0;
+    const hipError_t e = batch_capture(batch);
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_batch_set_shape", e));
+    return CHD_OK;
+}
+
+void chd_tpal_batch_free(chd_tpal_batch_t *batch)
+{
+    if (batch == nullptr) return;
+    hipSetDevice(batch->plan->device);
+    if (batch->cap) hipStreamSynchronize(batch->cap);
+    hipDeviceSynchronize();
+    batch_release(batch);
+    delete batch;
+}
+
 chd_status_t chd_tpal_set_timing(chd_tpal_plan_t *plan, int enable)
 {
     if (plan == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_set_timing: null argument");

@@ -1302,6 +1302,23 @@ __global__ void pack_fields(const Geom g, const uint16_t *__restrict__ src, int6
     }
 }
 
+// Chroma fields (field_height x field_width floats each) into the partial
+// plane layout the demodulator reads (columns of the active samples only).
+__global__ void chroma_to_planes(const Geom g, const float *__restrict__ src, float *__restrict__ dst,
+                                 int64_t plane_stride, int n_fields)
+{
+    const int64_t per = (int64_t)g.fh * g.aw;
+    const int64_t total = (int64_t)n_fields * per;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = i / per;
+        const int64_t r = i - f * per;
+        const int line = (int)(r / g.aw);
+        const int c = (int)(r - (int64_t)line * g.aw);
+        dst[f * plane_stride + (int64_t)line * g.cpitch + c + kColOrigin] =
+            src[(f * g.fh + line) * (int64_t)g.fw + g.avs + c];
+    }
+}
+
 // Fill whole output frames with the black border value (only needed when
 // padding adds a border around the active picture).
 __global__ void fill_border(const Geom g, void *out, int64_t n_frames)
@@ -1388,6 +1405,12 @@ void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, ui
                  hipStream_t s)
 {
     if (n_fields > 0) hipLaunchKernelGGL(pack_fields, dim3(1024), dim3(256), 0, s, g, src, src_field_elems, dst, n_fields);
+}
+
+void launch_chroma_to_planes(const Geom &g, const float *src, float *dst, int64_t plane_stride, int n_fields,
+                             hipStream_t s)
+{
+    if (n_fields > 0) hipLaunchKernelGGL(chroma_to_planes, dim3(1024), dim3(256), 0, s, g, src, dst, plane_stride, n_fields);
 }
 
 void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s)

@@ -11,10 +11,16 @@ gathers only.
     python -m libchromadec_amd.mask_training --frames 2000 --steps 4000 \\
         --out libchromadec_amd/data/tpal3d_mask.bin
 
-Evaluation (`chroma_psnr`) decodes held-out frames with the plain
-threshold and with the learned mask and reports PSNR of E'Cb/E'Cr against
-the band-limited U/V the encoder modulated, over the active picture less an
-8-sample / 4-line margin.
+Evaluation decodes held-out frames with the plain threshold and with the
+learned mask and reports two chroma PSNRs over the active picture less an
+8-sample / 4-line margin:
+  * separation PSNR: E'Cb/E'Cr against the ideal separation, i.e. the same
+    PalColour demodulation applied to the true modulated chroma
+    (chd_tpal_decode_separated).  The mask only changes the Y/C separation,
+    and this isolates it;
+  * PSNR against the band-limited U/V the encoder modulated, which also
+    carries the demodulator's own chroma bandwidth limit (the same for every
+    mask).
 """
 from __future__ import annotations
 
@@ -47,9 +53,10 @@ def sequence_spec(seed: int):
                 jitter=float(rng.choice([0.0, 0.0, 1.0])), bars=bool(rng.uniform() < 0.15))
 
 
-def encode_sequence(spec: dict, n_frames: int):
+def encode_sequence(spec: dict, n_frames: int, with_uv: bool = True):
     """Returns composite fields [2n, 313, 1135] uint16, truth chroma as
-    uint16 around TRUTH_BLACK, and the band-limited U, V (float32)."""
+    uint16 around TRUTH_BLACK, and the band-limited U, V (float32; None
+    unless with_uv)."""
     if spec.get("bars"):
         pics = [synth.bars_gradient(f, speed=min(spec["max_speed"], 4.0)) for f in range(n_frames)]
     else:
@@ -59,15 +66,17 @@ def encode_sequence(spec: dict, n_frames: int):
                                rng=np.random.default_rng(spec["seed"] + 7919))
     fields, u, v, chroma = synth.encode_sequence(pics, opts, return_truth=True)
     truth = np.clip(np.rint(chroma + TRUTH_BLACK), 0, 65535).astype(np.uint16)
+    if not with_uv:
+        return fields, truth, None, None
     return fields, truth, u.astype(np.float32), v.astype(np.float32)
 
 
 def _encode_job(args):
-    spec, n = args
-    return encode_sequence(spec, n)
+    spec, n, with_uv = args
+    return encode_sequence(spec, n, with_uv)
 
 
-def make_dataset(n_frames: int, seed: int, seq_len: int = 20, workers: int = 0, log=print):
+def make_dataset(n_frames: int, seed: int, seq_len: int = 20, workers: int = 0, log=print, with_uv: bool = True):
     """Sequences of `seq_len` frames totalling n_frames, encoded in parallel."""
     n_seq = max(1, (n_frames + seq_len - 1) // seq_len)
     specs = [sequence_spec(seed * 100003 + i) for i in range(n_seq)]
@@ -76,7 +85,7 @@ def make_dataset(n_frames: int, seed: int, seq_len: int = 20, workers: int = 0, 
     out = []
     t0 = time.time()
     with ProcessPoolExecutor(workers) as ex:
-        for i, r in enumerate(ex.map(_encode_job, zip(specs, lens))):
+        for i, r in enumerate(ex.map(_encode_job, [(sp, n, with_uv) for sp, n in zip(specs, lens)])):
             out.append(r)
             if (i + 1) % max(1, n_seq // 10) == 0 or i + 1 == n_seq:
                 log(f"encoded {i + 1}/{n_seq} sequences ({sum(lens[:i + 1])} frames) in {time.time() - t0:.0f} s")
@@ -167,6 +176,14 @@ def train(data, steps: int = 4000, batch: int = 256, lr: float = 2e-3, slabs_per
 # Evaluation
 # ---------------------------------------------------------------------------
 
+def separation_psnr(frames_out: np.ndarray, ideal: np.ndarray, margin_x: int = 8, margin_y: int = 4) -> float:
+    """PSNR (dB) of E'Cb/E'Cr against the ideal-separation decode."""
+    a = frames_out[:, 1:, margin_y:frames_out.shape[2] - margin_y, margin_x:frames_out.shape[3] - margin_x]
+    b = ideal[:, 1:, margin_y:ideal.shape[2] - margin_y, margin_x:ideal.shape[3] - margin_x]
+    d = a.astype(np.float64) - b
+    return 10.0 * math.log10(1.0 / float(np.mean(d * d)))
+
+
 def chroma_psnr(frames_out: np.ndarray, u: np.ndarray, v: np.ndarray, frame_ids, ecb: float, ecr: float,
                 margin_x: int = 8, margin_y: int = 4) -> float:
     """PSNR (dB) of decoded E'Cb/E'Cr (yuv444ps frames, full active picture
@@ -202,24 +219,46 @@ def decode_frames(plan, fields: np.ndarray, device: int = 0) -> np.ndarray:
     return out.cpu().numpy().reshape(n, 3, 576, 922)
 
 
+def ideal_frames(plan, fields: np.ndarray, truth: np.ndarray, device: int = 0) -> np.ndarray:
+    """Frames decoded from the true chroma (uint16 around TRUTH_BLACK)."""
+    import torch
+    dev = torch.device(f"cuda:{device}")
+    n = fields.shape[0] // 2
+    src = torch.from_numpy(fields.view(np.int16)).to(dev)
+    pk = torch.empty((fields.shape[0], plan.packed_field_elems), dtype=torch.int16, device=dev)
+    plan.pack(src.data_ptr(), fields.shape[0], pk.data_ptr())
+    ch = torch.from_numpy(truth.astype(np.float32) - float(TRUTH_BLACK)).to(dev)
+    out = torch.empty((n, plan.frame_bytes // 4), dtype=torch.float32, device=dev)
+    plan.decode_separated(pk.data_ptr(), ch.data_ptr(), n, out.data_ptr())
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(n, 3, 576, 922)
+
+
 def evaluate(net, data, device: int = 0, threshold: float = 0.4):
-    """(plain PSNR, learned PSNR) over every frame of every sequence."""
+    """Chroma PSNRs over every frame of every sequence: dict with
+    sep_plain / sep_learned (against the ideal separation) and uv_plain /
+    uv_learned (against the encoder's U/V)."""
     from . import abi
     ecb, ecr = abi.color_scales()
     plain = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, threshold=threshold, device=device,
                                              max_batch_frames=16))
     learned = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, device=device, max_batch_frames=16))
     learned.set_mask_net(net)
-    se = {"plain": [0.0, 0], "learned": [0.0, 0]}
-    for fields, _, u, v in data:
+    acc = {k: [0.0, 0] for k in ("sep_plain", "sep_learned", "uv_plain", "uv_learned")}
+
+    def add(key, psnr, n):
+        acc[key][0] += n * 10.0 ** (-psnr / 10.0)
+        acc[key][1] += n
+
+    for fields, truth, u, v in data:
         n = fields.shape[0] // 2
+        ideal = ideal_frames(plain, fields, truth, device)
         for name, plan in (("plain", plain), ("learned", learned)):
             out = decode_frames(plan, fields, device)
-            p = chroma_psnr(out, u, v, range(n), ecb, ecr)
-            se[name][0] += n * 10.0 ** (-p / 10.0)
-            se[name][1] += n
+            add("sep_" + name, separation_psnr(out, ideal), n)
+            add("uv_" + name, chroma_psnr(out, u, v, range(n), ecb, ecr), n)
     learned.set_mask_net(None)
-    return tuple(10.0 * math.log10(se[k][1] / se[k][0]) for k in ("plain", "learned"))
+    return {k: 10.0 * math.log10(a[1] / a[0]) for k, a in acc.items()}
 
 
 def main(argv=None) -> int:
@@ -239,7 +278,7 @@ def main(argv=None) -> int:
     def log(msg):
         print(msg, flush=True)
 
-    _, data = make_dataset(args.frames, args.seed, args.seq_len, log=log)
+    _, data = make_dataset(args.frames, args.seed, args.seq_len, log=log, with_uv=False)
     net, hist = train(data, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
     del data
     net.save(args.out)
@@ -248,11 +287,11 @@ def main(argv=None) -> int:
                    loss_history=hist)
     if args.eval_frames > 0:
         _, evdata = make_dataset(args.eval_frames, args.eval_seed, args.seq_len, log=log)
-        p_plain, p_learned = evaluate(net, evdata)
-        log(f"held-out chroma PSNR: plain {p_plain:.2f} dB, learned {p_learned:.2f} dB "
-            f"({p_learned - p_plain:+.2f} dB)")
-        summary.update(eval_frames=args.eval_frames, eval_seed=args.eval_seed, psnr_plain_db=p_plain,
-                       psnr_learned_db=p_learned)
+        r = evaluate(net, evdata)
+        log(f"held-out separation PSNR: plain {r['sep_plain']:.2f} dB, learned {r['sep_learned']:.2f} dB "
+            f"({r['sep_learned'] - r['sep_plain']:+.2f} dB); against the encoder's U/V: plain "
+            f"{r['uv_plain']:.2f} dB, learned {r['uv_learned']:.2f} dB")
+        summary.update(eval_frames=args.eval_frames, eval_seed=args.eval_seed, **{"psnr_" + k: v for k, v in r.items()})
     if args.summary:
         with open(args.summary, "w") as fh:
             json.dump(summary, fh, indent=1)

@@ -199,6 +199,29 @@ def decode_frame_masked(p: OrcParams, seq_fields, frame, masks, out_fmt=0):
     return out
 
 
+def decode_frame_from_chroma(p: OrcParams, field0, field1, chroma0, chroma1, out_fmt=0):
+    """PalColour::decodeFrames with given chroma fields (palcolour.cpp:279-335)
+    and OutputWriter: the demodulation half of a decode."""
+    L = lib()
+    fw, fh = p.field_width, p.field_height
+    ca = fw * (2 * fh - 1)
+    f0 = np.ascontiguousarray(field0, dtype=np.uint16)
+    f1 = np.ascontiguousarray(field1, dtype=np.uint16)
+    c0 = np.ascontiguousarray(chroma0, dtype=np.float64)
+    c1 = np.ascontiguousarray(chroma1, dtype=np.float64)
+    yuv = np.zeros(3 * ca)
+    L.orc_palcolour_decode_frame(C.byref(p), _u16p(f0), 0, _u16p(f1), 1, _dp(c0), _dp(c1), _dp(yuv),
+                                 _dp(yuv[ca:]), _dp(yuv[2 * ca:]))
+    out = np.zeros((3, p.out_height, p.out_width), dtype=np.float32 if out_fmt == 0 else np.uint16)
+    if out_fmt == 0:
+        L.orc_output_float(C.byref(p), _dp(yuv), _dp(yuv[ca:]), _dp(yuv[2 * ca:]),
+                           out.ctypes.data_as(C.POINTER(C.c_float)))
+    else:
+        L.orc_output_u16(C.byref(p), _dp(yuv), _dp(yuv[ca:]), _dp(yuv[2 * ca:]),
+                         out.ctypes.data_as(C.POINTER(C.c_uint16)))
+    return out
+
+
 def decode_sequence(p: OrcParams, kind: int, seq_fields, frames, thr_sq, out_fmt=0, threads=0):
     L = lib()
     seq = np.ascontiguousarray(seq_fields, dtype=np.uint16)

@@ -184,3 +184,14 @@ def test_masked_decode_all_ones_is_threshold_zero_and_linear(orc, patch_capture)
     rhs = orc.decode_frame_masked(p, fields, 2, m1) + orc.decode_frame_masked(p, fields, 2, m2) - zero
     assert np.abs(lhs[1:] - rhs[1:]).max() < 1e-5   # U/V linear in the mask
     assert np.all(zero[1:] == 0.0)
+
+
+def test_demod_from_chroma_reproduces_the_full_decode(orc, patch_capture):
+    """decode_frame_from_chroma is the second half of decode_window: fed the
+    transform's own chroma it gives the same frame."""
+    _, fields = patch_capture
+    p = orc.pal_params(first_line=200, last_line=263, avs=400, ave=528)
+    fw, offs, s, e = orc.frame_window(fields, 2, 8)
+    full, chroma = orc.decode_window(p, 8, fw, offs, s, e, orc.uniform_thresholds(8), want_chroma=True)
+    got = orc.decode_frame_from_chroma(p, fields[4], fields[5], chroma[0], chroma[1])
+    assert np.array_equal(got, full)
