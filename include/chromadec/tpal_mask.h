@@ -37,9 +37,10 @@ extern "C" {
 
 typedef struct chd_mask_net chd_mask_net_t;
 
-/* Network state on `device` with working memory for max_tiles tiles per
- * forward or gradient call (about 0.2 MB per tile).  Parameters start at 0
- * (mask 0.5 everywhere) until set or loaded. */
+/* Network state on `device` with training working memory for max_tiles
+ * tiles per gradient call (about 0.2 MB per tile; 1 for a network used only
+ * to decode).  Parameters start at 0 (mask 0.5 everywhere) until set or
+ * loaded. */
 chd_status_t chd_mask_net_create(int32_t device, int64_t max_tiles, chd_mask_net_t **out);
 void         chd_mask_net_free(chd_mask_net_t *net);
 
@@ -52,8 +53,9 @@ chd_status_t chd_mask_net_get_params(chd_mask_net_t *net, float *params, int64_t
 chd_status_t chd_mask_net_load(chd_mask_net_t *net, const char *path);
 chd_status_t chd_mask_net_save(chd_mask_net_t *net, const char *path);
 
-/* Soft mask of n_tiles tiles: d_mask[tile][768].  Any n_tiles; processed in
- * groups of max_tiles.  Enqueued on `stream`. */
+/* Soft mask of n_tiles tiles: d_mask[tile][768].  Any n_tiles (inference
+ * needs no working memory: one fused kernel, activations on chip).
+ * Enqueued on `stream`. */
 chd_status_t chd_mask_net_forward(chd_mask_net_t *net, const void *d_spectra, int64_t n_tiles, float *d_mask,
                                   void *stream);
 

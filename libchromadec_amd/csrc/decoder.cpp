@@ -241,8 +241,8 @@ chd_status_t worker_init(chd_decoder_t *d, Worker &w, const char *who)
     if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
     if (e != hipSuccess) return fail(CHD_E_OOM, std::string(who) + ": device allocation failed: " + hip_err(e));
     if (!d->maskWeights.empty()) {
-        // one slab of tiles per network pass; a frame window uses two
-        st = chd_mask_net_create(p.device, (int64_t)g.n_ty3 * g.n_tx3, &w.net);
+        // inference only: no training working memory
+        st = chd_mask_net_create(p.device, 1, &w.net);
         if (st == CHD_OK) st = chd_mask_net_load(w.net, d->maskWeights.c_str());
         if (st == CHD_OK) st = chd_tpal_set_mask_net(w.plan, w.net);
         if (st != CHD_OK) return st;

@@ -50,6 +50,9 @@ chd_status_t chd_mask_net_create(int32_t device, int64_t max_tiles, chd_mask_net
     auto *n = new chd_mask_net();
     n->device = device;
     n->max_tiles = max_tiles;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        n->cus = cus;
     const size_t T = (size_t)max_tiles, P = mnet::kP;
     bool ok = true;
     auto dm = [&](void **p, size_t bytes) {
@@ -172,12 +175,8 @@ chd_status_t chd_mask_net_forward(chd_mask_net_t *net, const void *d_spectra, in
         return chd::fail(CHD_E_INVALID_ARG, "chd_mask_net_forward: null argument");
     hipSetDevice(net->device);
     const hipStream_t s = (hipStream_t)stream;
-    for (int64_t t0 = 0; t0 < n_tiles; t0 += net->max_tiles) {
-        const int64_t cnt = std::min<int64_t>(net->max_tiles, n_tiles - t0);
-        const float2 *spec = reinterpret_cast<const float2 *>(d_spectra) + t0 * mnet::kSpecC;
-        mnet::launch_features(spec, cnt, net->a.feat, s);
-        mnet::launch_forward(net->theta, net->f, net->a, cnt, d_mask + t0 * mnet::kP, s);
-    }
+    // fused kernel: no working memory, any number of tiles in one launch
+    mnet::launch_infer(reinterpret_cast<const float2 *>(d_spectra), n_tiles, net->theta, net->f, d_mask, net->cus, s);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_mask_net_forward", e));
     return CHD_OK;

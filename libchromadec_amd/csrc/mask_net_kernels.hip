@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mask_net_internal.h"
 
 namespace mnet {
@@ -306,6 +308,217 @@ __global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ act2,
 }
 
 // ---------------------------------------------------------------------------
+// Fused inference: features, the three layers and the sigmoid of a tile in
+// one workgroup, activations in LDS.  Persistent: each workgroup keeps the
+// layer-2 weight fragments in LDS and loops over tiles.  Same arithmetic as
+// k_features + k_conv<kFwd1/kFwd2> + k_head (BF16 operands, FP32
+// accumulation, BF16 activations), so masks agree with the layered forward
+// up to FP32 summation order.
+// ---------------------------------------------------------------------------
+
+constexpr int kActStride = 40;  // bf16 per activation row in LDS (32 + 8: spreads gathered rows over banks)
+constexpr int kZeroRow = kP;    // an all-zero row: the neighbour of a visit across the x edge
+
+struct LdsInfer {
+    uint4 w2[kK2 * 64];                        // layer-2 B fragments
+    union {
+        uint16_t act[(kP + 1) * kActStride];   // act1, then act2 (+ zero row)
+        float g[(kP + 1) * 27];                // head partial sums G[q][t] (+ zero row)
+    } u;
+    uint2 feat[kP + 1];                        // 4 bf16 features per visit (+ zero row)
+    float red[4];
+};
+
+// Row of the neighbour of a visit with coordinates (z, y, x) at tap offsets
+// (dz, dy, dx), or the zero row when x + dx leaves the grid.
+__device__ __forceinline__ int nbr_row(int z, int y, int x, int dz, int dy, int dx)
+{
+    const int xs = x + dx;
+    return (unsigned)xs < 3u ? ((((z + dz) & 7) << 5) | ((y + dy) & 31)) * 3 + xs : kZeroRow;
+}
+
+constexpr int kInferWaves = 8;           // 512 threads: two waves per SIMD share the tile
+constexpr int kInferBlocks = 24 / kInferWaves;   // row blocks of 32 visits per wave
+
+__global__ void __launch_bounds__(64 * kInferWaves, 1)
+k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restrict__ theta, Frags f,
+        float *__restrict__ mask)
+{
+    constexpr int NT = 64 * kInferWaves, NB = kInferBlocks;
+    __shared__ LdsInfer L;
+    __shared__ float red[kInferWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 31, hi = lane >> 5;
+    for (int k = tid; k < kK2 * 64; k += NT) L.w2[k] = f.w2[k];
+    if (tid < 27) L.u.g[kZeroRow * 27 + tid] = 0.0f;   // beyond the act region: set once
+    if (tid == 0) L.feat[kZeroRow] = make_uint2(0u, 0u);
+    const float b1 = theta[kOffB1 + i], b2 = theta[kOffB2 + i], b3 = theta[kOffB3];
+    const uint4 w3a = f.w3t[lane], w3b = f.w3t[64 + lane];
+    // coordinates of this lane's A rows (one per block)
+    int pz[NB], py[NB], px[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int p = (wave + kInferWaves * b) * 32 + i;
+        const int zy = p / 3;
+        px[b] = p - 3 * zy;
+        py[b] = zy & 31;
+        pz[b] = zy >> 5;
+    }
+    __syncthreads();
+
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        // ---- features (as k_features) ----------------------------------
+        const float2 *S = spec + t * kSpecC;
+        float f0[2] = {0.0f, 0.0f}, f1[2] = {0.0f, 0.0f}, s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int p = tid + NT * j;
+            if (p < kP) {
+                int ia, ib;
+                visit_bins(p, ia, ib);
+                const float2 a = S[ia], b = S[ib];
+                f0[j] = (logf(sqrtf(a.x * a.x + a.y * a.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
+                f1[j] = (logf(sqrtf(b.x * b.x + b.y * b.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
+                s += f0[j];
+            }
+        }
+        s = wave_sum(s);
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        float tot = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kInferWaves; ++w) tot += red[w];
+        const float mean = tot * (1.0f / kP);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int p = tid + NT * j;
+            if (p < kP) {
+                const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
+                const float sm = (x == 2 && (y == 4 || y == 20) && (z == 1 || z == 5)) ? 1.0f : 0.0f;
+                L.feat[p] = make_uint2(pack2(f0[j], f1[j]), pack2(mean, sm));
+            }
+        }
+        if (tid < kActStride / 2) reinterpret_cast<uint32_t *>(&L.u.act[kZeroRow * kActStride])[tid] = 0u;
+        __syncthreads();
+
+        f32x16 acc[NB];
+        // ---- layer 1: K = t*4 + ci (112) --------------------------------
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kK1; ++j) {
+            const uint4 bf = f.w1[j * 64 + lane];
+            // this lane's taps: 4j + 2hi and 4j + 2hi + 1 (>= 27: padding)
+            const int ta = 4 * j + 2 * hi, tb = ta + 1;
+            const int daz = ta / 9 - 1, day = (ta / 3) % 3 - 1, dax = ta % 3 - 1;
+            const int dbz = tb / 9 - 1, dby = (tb / 3) % 3 - 1, dbx = tb % 3 - 1;
+            uint4 af[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int qa = ta < 27 ? nbr_row(pz[b], py[b], px[b], daz, day, dax) : kZeroRow;
+                const int qb = tb < 27 ? nbr_row(pz[b], py[b], px[b], dbz, dby, dbx) : kZeroRow;
+                const uint2 lo = L.feat[qa], up = L.feat[qb];
+                af[b] = make_uint4(lo.x, lo.y, up.x, up.y);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) acc[b] = mfma(af[b], bf, acc[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int p = (wave + kInferWaves * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                L.u.act[p * kActStride + i] = (uint16_t)f2bf(fmaxf(acc[b][v] + b1, 0.0f));
+            }
+        __syncthreads();
+
+        // ---- layer 2: K = t*32 + ci (864), loads one half-tap ahead ------
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
+        {
+            int row[NB];
+            uint4 cur[NB], bcur;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                row[b] = nbr_row(pz[b], py[b], px[b], -1, -1, -1) * kActStride + 8 * hi;
+                cur[b] = *reinterpret_cast<const uint4 *>(&L.u.act[row[b]]);
+            }
+            bcur = L.w2[lane];
+            for (int k = 0; k < 2 * 27; ++k) {
+                // operands of half-tap k + 1 (tap = k' / 2, channel half = k' % 2)
+                const int kn = k + 1 < 2 * 27 ? k + 1 : k;
+                const int tn = kn >> 1, hn = kn & 1;
+                int rn[NB];
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    rn[b] = nbr_row(pz[b], py[b], px[b], tn / 9 - 1, (tn / 3) % 3 - 1, tn % 3 - 1) * kActStride +
+                            8 * hi + 16 * hn;
+                uint4 nxt[NB];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) nxt[b] = *reinterpret_cast<const uint4 *>(&L.u.act[rn[b]]);
+                const uint4 bnxt = L.w2[kn * 64 + lane];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) acc[b] = mfma(cur[b], bcur, acc[b]);
+#pragma unroll
+                for (int b = 0; b < NB; ++b) cur[b] = nxt[b];
+                bcur = bnxt;
+            }
+        }
+        __syncthreads();  // every wave has read act1
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int p = (wave + kInferWaves * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                L.u.act[p * kActStride + i] = (uint16_t)f2bf(fmaxf(acc[b][v] + b2, 0.0f));
+            }
+        __syncthreads();
+
+        // ---- head: G[q][t] = sum_c act2[q][c] W3[t][c] ------------------
+        {
+            uint4 a0[NB], a1[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int p = (wave + kInferWaves * b) * 32 + i;
+                a0[b] = *reinterpret_cast<const uint4 *>(&L.u.act[p * kActStride + 8 * hi]);
+                a1[b] = *reinterpret_cast<const uint4 *>(&L.u.act[p * kActStride + 16 + 8 * hi]);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
+                acc[b] = mfma(a0[b], w3a, acc[b]);
+                acc[b] = mfma(a1[b], w3b, acc[b]);
+            }
+        }
+        __syncthreads();  // act2 is read; G overwrites it
+        if (i < 27) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int q = (wave + kInferWaves * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                    L.u.g[q * 27 + i] = acc[b][v];
+                }
+        }
+        __syncthreads();
+        for (int p = tid; p < kP; p += NT) {
+            const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
+            float sum = b3;
+#pragma unroll
+            for (int tap = 0; tap < 27; ++tap)
+                sum += L.u.g[nbr_row(z, y, x, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1) * 27 + tap];
+            mask[t * kP + p] = 1.0f / (1.0f + expf(-sum));
+        }
+        __syncthreads();  // G is read before the next tile's features
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Loss and its gradient at the logits.
 // ---------------------------------------------------------------------------
 
@@ -537,6 +750,14 @@ void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n
     hipLaunchKernelGGL(k_conv<kFwd2>, grid, blk, 0, s, (const void *)a.act1, f.w2, theta + kOffB2,
                        (const uint16_t *)nullptr, a.act2);
     hipLaunchKernelGGL(k_head, grid, blk, 0, s, a.act2, f.w3t, theta + kOffB3, a.logit, mask);
+}
+
+void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
+                  int n_groups, hipStream_t s)
+{
+    if (n_tiles <= 0) return;
+    const int64_t g = std::min<int64_t>(n_tiles, std::max(1, n_groups));
+    hipLaunchKernelGGL(k_infer, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, spec, n_tiles, theta, f, mask);
 }
 
 void launch_backward(const float *theta, const Frags &f, const Acts &a, const float *mask, int64_t n_tiles,

@@ -14,6 +14,7 @@ struct chd_mask_net {
     float *partial = nullptr;                // weight-gradient partials per workgroup
     float *mask = nullptr;                   // [max_tiles][768] for the gradient pass
     int step = 0;                            // Adam steps taken
+    int cus = 256;                           // compute units: persistent inference grid
     mnet::Frags f{};
     mnet::Acts a{};
 };

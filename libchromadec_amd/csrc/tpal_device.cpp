@@ -305,12 +305,8 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hi
         const int64_t tiles = (int64_t)n_units * g.n_ty3 * g.n_tx3;
         tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra,
                             plan->d_spec, nullptr);
-        for (int64_t t0 = 0; t0 < tiles; t0 += net->max_tiles) {
-            const int64_t cnt = std::min<int64_t>(net->max_tiles, tiles - t0);
-            const float2 *sp = reinterpret_cast<const float2 *>(plan->d_spec) + t0 * tpal::kTileBins;
-            mnet::launch_features(sp, cnt, net->a.feat, s);
-            mnet::launch_forward(net->theta, net->f, net->a, cnt, plan->d_mask + t0 * mnet::kP, s);
-        }
+        mnet::launch_infer(reinterpret_cast<const float2 *>(plan->d_spec), tiles, net->theta, net->f, plan->d_mask,
+                           net->cus, s);
         tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply,
                             plan->d_spec, plan->d_mask);
     } else if (plan->kind == CHD_DEC_TRANSFORM_3D)
@@ -746,13 +742,8 @@ void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
         const int cnt = std::min(b->chunk_units, b->n_units - u0);
         tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra, b->d_spec,
                             nullptr, strip, groups);
-        const int64_t tiles = cnt * tps;
-        for (int64_t t0 = 0; t0 < tiles; t0 += net->max_tiles) {
-            const int64_t n = std::min<int64_t>(net->max_tiles, tiles - t0);
-            const float2 *sp = reinterpret_cast<const float2 *>(b->d_spec) + t0 * tpal::kTileBins;
-            mnet::launch_features(sp, n, net->a.feat, s);
-            mnet::launch_forward(net->theta, net->f, net->a, n, b->d_mask + t0 * mnet::kP, s);
-        }
+        mnet::launch_infer(reinterpret_cast<const float2 *>(b->d_spec), cnt * tps, net->theta, net->f, b->d_mask,
+                           net->cus, s);
         tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply, b->d_spec,
                             b->d_mask, strip, groups);
     }

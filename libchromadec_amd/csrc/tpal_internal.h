@@ -26,7 +26,10 @@ constexpr int kStrip3 = 8;                   // owned 3D tiles per workgroup str
 constexpr int kStrip2 = 4;                   // owned 2D tiles per lane-group strip
 constexpr int kDemodLines = 8;               // field lines per demod workgroup
 constexpr int kMaxFilter = 15;               // max PalColour filterSize supported
-constexpr int kColOrigin = 16;               // partial column of sample active_video_start
+// Partial column of sample active_video_start.  3D strips own 64 columns
+// from tile column 8 tx - 8: with origin 40 every strip starts on a 128-byte
+// line, so no two workgroups (on different XCDs) write one cache line.
+constexpr int kColOrigin = 40;
 
 struct Geom {
     int32_t fw, fh;          // field width / height

@@ -112,7 +112,7 @@ def run(n_frames: int = 1000, weights: str | None = None, log=print):
     net = None
     for name in ("plain", "learned"):
         if name == "learned":
-            net = abi.MaskNet(max_tiles=16384)
+            net = abi.MaskNet(max_tiles=1)
             net.load(weights or os.path.join(os.path.dirname(__file__), "data", "tpal3d_mask.bin"))
             plan.set_mask_net(net)
         plan.decode(seqs, out.data_ptr())
