@@ -30,14 +30,25 @@ namespace mnet {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// float -> bf16, round to nearest even (finite inputs)
-__device__ __forceinline__ uint32_t f2bf(float f)
-{
-    const uint32_t u = __float_as_uint(f);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// float -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
-__device__ __forceinline__ uint32_t pack2(float lo, float hi) { return f2bf(lo) | (f2bf(hi) << 16); }
+__device__ __forceinline__ uint32_t pack2(float lo, float hi)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+
+// Magnitude feature of a bin: (ln(|a| + 1) - 10) / 3, from the hardware
+// square root and base-2 logarithm (the argument is >= 1: no denormals).
+// Its error, a few 1e-7, is far below the bf16 step the feature is stored in.
+__device__ __forceinline__ float mag_feature(float2 a)
+{
+    const float m = __builtin_amdgcn_sqrtf(a.x * a.x + a.y * a.y);
+    return fmaf(__builtin_amdgcn_logf(m + 1.0f), 0.23104906f /* ln 2 / 3 */, -10.0f / 3.0f);
+}
 
 __device__ __forceinline__ f32x16 mfma(uint4 a, uint4 b, f32x16 c)
 {
@@ -142,8 +153,8 @@ __global__ void __launch_bounds__(256) k_features(const float2 *__restrict__ spe
         int ia, ib;
         visit_bins(p, ia, ib);
         const float2 a = S[ia], b = S[ib];
-        f0[j] = (logf(sqrtf(a.x * a.x + a.y * a.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
-        f1[j] = (logf(sqrtf(b.x * b.x + b.y * b.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
+        f0[j] = mag_feature(a);
+        f1[j] = mag_feature(b);
         s += f0[j];
     }
     const float mean = block_sum(s, red) * (1.0f / kP);
@@ -317,6 +328,7 @@ __global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ act2,
 // ---------------------------------------------------------------------------
 
 constexpr int kActStride = 40;  // bf16 per activation row in LDS (32 + 8: spreads gathered rows over banks)
+constexpr int kActRowB = 2 * kActStride;
 constexpr int kZeroRow = kP;    // an all-zero row: the neighbour of a visit across the x edge
 
 struct LdsInfer {
@@ -326,25 +338,119 @@ struct LdsInfer {
         float g[(kP + 1) * 27];                // head partial sums G[q][t] (+ zero row)
     } u;
     uint2 feat[kP + 1];                        // 4 bf16 features per visit (+ zero row)
-    float red[4];
 };
 
-// Row of the neighbour of a visit with coordinates (z, y, x) at tap offsets
-// (dz, dy, dx), or the zero row when x + dx leaves the grid.
-__device__ __forceinline__ int nbr_row(int z, int y, int x, int dz, int dy, int dx)
+// Tile-invariant neighbourhood of one visit: the rows of (z+dz, y+dy, x) for
+// the 9 (dz, dy) pairs, and whether x sits on either edge.  Tap t's row is
+// then a compile-time pick plus at most one add and one select.
+struct Nbhd {
+    int zy[9];
+    bool lo, hi;
+};
+
+__device__ __forceinline__ Nbhd make_nbhd(int p)
 {
-    const int xs = x + dx;
-    return (unsigned)xs < 3u ? ((((z + dz) & 7) << 5) | ((y + dy) & 31)) * 3 + xs : kZeroRow;
+    Nbhd n;
+    const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) n.zy[j] = ((((z + j / 3 - 1) & 7) << 5) | ((y + j % 3 - 1) & 31)) * 3 + x;
+    n.lo = x == 0;
+    n.hi = x == 2;
+    return n;
+}
+
+template <int T>
+__device__ __forceinline__ int tap_row(const Nbhd &n)
+{
+    constexpr int dx = T % 3 - 1;
+    const int r = n.zy[T / 3];
+    if constexpr (dx < 0) return n.lo ? kZeroRow : r - 1;
+    else if constexpr (dx > 0) return n.hi ? kZeroRow : r + 1;
+    else return r;
+}
+
+template <int T>
+__device__ __forceinline__ int tap_row_or_zero(const Nbhd &n)
+{
+    if constexpr (T >= 27) return kZeroRow;
+    else return tap_row<T>(n);
 }
 
 constexpr int kInferWaves = 8;           // 512 threads: two waves per SIMD share the tile
 constexpr int kInferBlocks = 24 / kInferWaves;   // row blocks of 32 visits per wave
+constexpr int kInferVisits = (kP + 64 * kInferWaves - 1) / (64 * kInferWaves);   // feature/head visits per thread
+
+// Layer-1 k-step J: this lane's A operand, the features of its two taps
+// 4J + 2hi and 4J + 2hi + 1 (taps >= 27 are K padding: the zero row).
+template <int J>
+__device__ __forceinline__ uint4 l1_operand(const uint2 *feat, const Nbhd &n, int hi)
+{
+    const int qa = hi ? tap_row_or_zero<4 * J + 2>(n) : tap_row_or_zero<4 * J>(n);
+    const int qb = hi ? tap_row_or_zero<4 * J + 3>(n) : tap_row_or_zero<4 * J + 1>(n);
+    const uint2 lo = feat[qa], up = feat[qb];
+    return make_uint4(lo.x, lo.y, up.x, up.y);
+}
+
+template <int J, int NB>
+__device__ __forceinline__ void layer1_steps(const uint2 *feat, const Nbhd (&nb)[NB], int hi, const uint4 *w1,
+                                             int lane, f32x16 (&acc)[NB])
+{
+    if constexpr (J < kK1) {
+        const uint4 bf = w1[J * 64 + lane];
+        uint4 af[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) af[b] = l1_operand<J>(feat, nb[b], hi);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = mfma(af[b], bf, acc[b]);
+        layer1_steps<J + 1>(feat, nb, hi, w1, lane, acc);
+    }
+}
+
+// Layer-2 half-tap K (tap K/2, input channels 16 (K%2) .. +15): A operands of
+// every block, gathered from act1 (byte base `ap` includes this lane's
+// 8-channel half).
+template <int K, int NB>
+__device__ __forceinline__ void l2_operands(const char *ap, const Nbhd (&nb)[NB], uint4 (&a)[NB])
+{
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        a[b] = *reinterpret_cast<const uint4 *>(ap + tap_row<K / 2>(nb[b]) * kActRowB + 32 * (K % 2));
+}
+
+template <int K, int NB>
+__device__ __forceinline__ void layer2_steps(const char *ap, const Nbhd (&nb)[NB], const uint4 *w2, int lane,
+                                             uint4 (&cur)[NB], uint4 &bcur, f32x16 (&acc)[NB])
+{
+    if constexpr (K < kK2) {
+        uint4 nxt[NB], bnxt;
+        if constexpr (K + 1 < kK2) {
+            l2_operands<K + 1>(ap, nb, nxt);
+            bnxt = w2[(K + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = mfma(cur[b], bcur, acc[b]);
+        if constexpr (K + 1 < kK2) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) cur[b] = nxt[b];
+            bcur = bnxt;
+            layer2_steps<K + 1>(ap, nb, w2, lane, cur, bcur, acc);
+        }
+    }
+}
+
+// logit of one visit: b3 + sum over taps of G[neighbour][tap], taps in order.
+template <int T>
+__device__ __forceinline__ float head_sum(const float *g, const Nbhd &n, float s)
+{
+    if constexpr (T < 27) return head_sum<T + 1>(g, n, s + g[tap_row<T>(n) * 27 + T]);
+    else return s;
+}
 
 __global__ void __launch_bounds__(64 * kInferWaves, 1)
 k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restrict__ theta, Frags f,
         float *__restrict__ mask)
 {
-    constexpr int NT = 64 * kInferWaves, NB = kInferBlocks;
+    constexpr int NT = 64 * kInferWaves, NB = kInferBlocks, NV = kInferVisits;
     __shared__ LdsInfer L;
     __shared__ float red[kInferWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -354,31 +460,49 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     if (tid == 0) L.feat[kZeroRow] = make_uint2(0u, 0u);
     const float b1 = theta[kOffB1 + i], b2 = theta[kOffB2 + i], b3 = theta[kOffB3];
     const uint4 w3a = f.w3t[lane], w3b = f.w3t[64 + lane];
-    // coordinates of this lane's A rows (one per block)
-    int pz[NB], py[NB], px[NB];
+    // neighbourhoods of this lane's A rows (one per block) and of the visits
+    // whose features and logits this thread computes
+    Nbhd nb[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int p = (wave + kInferWaves * b) * 32 + i;
-        const int zy = p / 3;
-        px[b] = p - 3 * zy;
-        py[b] = zy & 31;
-        pz[b] = zy >> 5;
+    for (int b = 0; b < NB; ++b) nb[b] = make_nbhd((wave + kInferWaves * b) * 32 + i);
+    int ia[NV], ib[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const int p = tid + NT * j;
+        ia[j] = ib[j] = 0;
+        if (p < kP) visit_bins(p, ia[j], ib[j]);
     }
+    const char *ap = reinterpret_cast<const char *>(L.u.act) + 16 * hi;
+    // spectrum bins of this thread's visits, loaded one tile ahead
+    float2 sa[NV], sb[NV];
+    auto fetch = [&](int64_t t) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            sa[j] = sb[j] = make_float2(0.0f, 0.0f);
+            if (t < n_tiles && tid + NT * j < kP) {
+                sa[j] = spec[t * kSpecC + ia[j]];
+                sb[j] = spec[t * kSpecC + ib[j]];
+            }
+        }
+    };
+    fetch(blockIdx.x);
     __syncthreads();
 
     for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        // ---- features (as k_features) ----------------------------------
-        const float2 *S = spec + t * kSpecC;
-        float f0[2] = {0.0f, 0.0f}, f1[2] = {0.0f, 0.0f}, s = 0.0f;
+        // Keep the neighbourhood rows opaque per tile: otherwise every tap's
+        // row (81 per lane) is hoisted out of the tile loop and spills.
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int p = tid + NT * j;
-            if (p < kP) {
-                int ia, ib;
-                visit_bins(p, ia, ib);
-                const float2 a = S[ia], b = S[ib];
-                f0[j] = (logf(sqrtf(a.x * a.x + a.y * a.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
-                f1[j] = (logf(sqrtf(b.x * b.x + b.y * b.y) + 1.0f) - 10.0f) * (1.0f / 3.0f);
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int j = 0; j < 9; ++j) asm volatile("" : "+v"(nb[b].zy[j]));
+        // ---- features (as k_features) ----------------------------------
+        float f0[NV], f1[NV], s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            f0[j] = f1[j] = 0.0f;
+            if (tid + NT * j < kP) {
+                f0[j] = mag_feature(sa[j]);
+                f1[j] = mag_feature(sb[j]);
                 s += f0[j];
             }
         }
@@ -390,7 +514,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         for (int w = 0; w < kInferWaves; ++w) tot += red[w];
         const float mean = tot * (1.0f / kP);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < NV; ++j) {
             const int p = tid + NT * j;
             if (p < kP) {
                 const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
@@ -398,8 +522,10 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
                 L.feat[p] = make_uint2(pack2(f0[j], f1[j]), pack2(mean, sm));
             }
         }
+        // the act zero row lies inside G's extent: clear it again every tile
         if (tid < kActStride / 2) reinterpret_cast<uint32_t *>(&L.u.act[kZeroRow * kActStride])[tid] = 0u;
         __syncthreads();
+        fetch(t + gridDim.x);   // next tile's bins, in flight through both layers
 
         f32x16 acc[NB];
         // ---- layer 1: K = t*4 + ci (112) --------------------------------
@@ -407,24 +533,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
-#pragma unroll
-        for (int j = 0; j < kK1; ++j) {
-            const uint4 bf = f.w1[j * 64 + lane];
-            // this lane's taps: 4j + 2hi and 4j + 2hi + 1 (>= 27: padding)
-            const int ta = 4 * j + 2 * hi, tb = ta + 1;
-            const int daz = ta / 9 - 1, day = (ta / 3) % 3 - 1, dax = ta % 3 - 1;
-            const int dbz = tb / 9 - 1, dby = (tb / 3) % 3 - 1, dbx = tb % 3 - 1;
-            uint4 af[NB];
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int qa = ta < 27 ? nbr_row(pz[b], py[b], px[b], daz, day, dax) : kZeroRow;
-                const int qb = tb < 27 ? nbr_row(pz[b], py[b], px[b], dbz, dby, dbx) : kZeroRow;
-                const uint2 lo = L.feat[qa], up = L.feat[qb];
-                af[b] = make_uint4(lo.x, lo.y, up.x, up.y);
-            }
-#pragma unroll
-            for (int b = 0; b < NB; ++b) acc[b] = mfma(af[b], bf, acc[b]);
-        }
+        layer1_steps<0>(L.feat, nb, hi, f.w1, lane, acc);
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -434,39 +543,16 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
             }
         __syncthreads();
 
-        // ---- layer 2: K = t*32 + ci (864), loads one half-tap ahead ------
+        // ---- layer 2: K = t*32 + ci (864), operands one half-tap ahead ----
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
         {
-            int row[NB];
             uint4 cur[NB], bcur;
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                row[b] = nbr_row(pz[b], py[b], px[b], -1, -1, -1) * kActStride + 8 * hi;
-                cur[b] = *reinterpret_cast<const uint4 *>(&L.u.act[row[b]]);
-            }
+            l2_operands<0>(ap, nb, cur);
             bcur = L.w2[lane];
-            for (int k = 0; k < 2 * 27; ++k) {
-                // operands of half-tap k + 1 (tap = k' / 2, channel half = k' % 2)
-                const int kn = k + 1 < 2 * 27 ? k + 1 : k;
-                const int tn = kn >> 1, hn = kn & 1;
-                int rn[NB];
-#pragma unroll
-                for (int b = 0; b < NB; ++b)
-                    rn[b] = nbr_row(pz[b], py[b], px[b], tn / 9 - 1, (tn / 3) % 3 - 1, tn % 3 - 1) * kActStride +
-                            8 * hi + 16 * hn;
-                uint4 nxt[NB];
-#pragma unroll
-                for (int b = 0; b < NB; ++b) nxt[b] = *reinterpret_cast<const uint4 *>(&L.u.act[rn[b]]);
-                const uint4 bnxt = L.w2[kn * 64 + lane];
-#pragma unroll
-                for (int b = 0; b < NB; ++b) acc[b] = mfma(cur[b], bcur, acc[b]);
-#pragma unroll
-                for (int b = 0; b < NB; ++b) cur[b] = nxt[b];
-                bcur = bnxt;
-            }
+            layer2_steps<0>(ap, nb, L.w2, lane, cur, bcur, acc);
         }
         __syncthreads();  // every wave has read act1
 #pragma unroll
@@ -484,8 +570,8 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int p = (wave + kInferWaves * b) * 32 + i;
-                a0[b] = *reinterpret_cast<const uint4 *>(&L.u.act[p * kActStride + 8 * hi]);
-                a1[b] = *reinterpret_cast<const uint4 *>(&L.u.act[p * kActStride + 16 + 8 * hi]);
+                a0[b] = *reinterpret_cast<const uint4 *>(ap + p * kActRowB);
+                a1[b] = *reinterpret_cast<const uint4 *>(ap + p * kActRowB + 32);
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -506,15 +592,15 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
                 }
         }
         __syncthreads();
-        for (int p = tid; p < kP; p += NT) {
-            const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
-            float sum = b3;
 #pragma unroll
-            for (int tap = 0; tap < 27; ++tap)
-                sum += L.u.g[nbr_row(z, y, x, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1) * 27 + tap];
-            mask[t * kP + p] = 1.0f / (1.0f + expf(-sum));
+        for (int j = 0; j < NV; ++j) {
+            const int p = tid + NT * j;
+            if (p < kP) {
+                const float sum = head_sum<0>(L.u.g, make_nbhd(p), b3);
+                mask[t * kP + p] = 1.0f / (1.0f + expf(-sum));
+            }
         }
-        __syncthreads();  // G is read before the next tile's features
+        __syncthreads();  // G is read before the next tile's features and act1 overwrite it
     }
 }
 

@@ -1284,6 +1284,247 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
 }
 
 // ---------------------------------------------------------------------------
+// 4fSC demodulation, column-blocked.  One 256-thread workgroup = kQ4Lines
+// field lines x the whole active line; each thread owns 4 adjacent output
+// columns.  The thread keeps the 7 chroma rows its columns need (line n-3 ..
+// n+3) in registers and slides them down the block's lines, so each partial
+// plane is read once plus a 6-row halo per block, as float4.  The FIR inputs
+// t0..t3 of a line (same definitions and summation order as pal_demod_quad,
+// so results are identical) pass through LDS once, double-buffered: one
+// barrier per line.
+// ---------------------------------------------------------------------------
+
+constexpr int kQ4Lines = 32;
+constexpr int kQ4Cols = 4 * kDemodThreads;             // 1024 >= active width
+constexpr int kQ4Row4 = (kQ4Cols + 2 * kQHalo) / 4;     // float4 per t row, zero halos of 8
+
+struct LdsQ4 {
+    float4 t[2][4][kQ4Row4];
+    float bp[kQ4Lines], bq[kQ4Lines], vsw[kQ4Lines];
+};
+
+__device__ __forceinline__ float4 f4add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+__device__ __forceinline__ float f4get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
+
+__global__ void __launch_bounds__(kDemodThreads)
+pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
+                const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks)
+{
+    __shared__ LdsQ4 L;
+    const int tid = threadIdx.x;
+    const int lb = blockIdx.x % n_lblocks;
+    const int fsel = (blockIdx.x / n_lblocks) & 1;
+    const FrameJob &job = jobs[blockIdx.x / (2 * n_lblocks)];
+    const int off = job.offset[fsel];
+    const uint16_t *comp = job.field[fsel];
+    const int firstLine = (g.first + 1 - off) / 2;
+    const int lastLine = ((g.last - off) / 2) + 1;
+    const int n0 = firstLine + lb * kQ4Lines;
+    if (n0 >= lastLine) return;
+    const int nl = min(kQ4Lines, lastLine - n0);
+    const int fs = g.filter_size;
+    const int c0 = 4 * tid;                  // this thread's first column, from active_video_start
+    const bool any_live = c0 < g.aw;
+    const int nc = job.nchroma;
+    const float *const *planes = job.chroma[fsel];
+
+    // zero halos of the t rows (columns -8..-1 and 1024..1031), both buffers
+    if (tid < 32) {
+        const int buf = tid >> 4, k = (tid >> 2) & 3, h = tid & 3;
+        L.t[buf][k][h < 2 ? h : kQ4Row4 - 4 + h] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+
+    // ---- burst detection per line (palcolour.cpp:344-409) -------------------
+    {
+        const int wave = tid >> 6, lane = tid & 63;
+        const int blen = g.cbe - g.cbs;
+        for (int li = wave; li < nl; li += kDemodThreads / 64) {
+            const int number = n0 + li;
+            float bp = 0, bq = 0, bpo = 0, bqo = 0;
+            for (int i = g.cbs + lane; i < g.cbe; i += 64) {
+                auto px = [&](int ln) -> float {
+                    if (ln < 0 || ln >= g.fh) return 0.0f;  // blackLine (zeros)
+                    if (comp == nullptr) return g.black;
+                    return (float)comp[(int64_t)ln * g.pitch + i + g.lmargin];
+                };
+                const float in0 = px(number), in1 = px(number - 1), in2 = px(number + 1);
+                const float in3 = px(number - 2), in4 = px(number + 2);
+                const float a = (in0 - (in3 + in4) * 0.5f) * 0.5f;
+                const float b = (in2 - in1) * 0.5f;
+                bp += a * sine[i];
+                bq += a * cosine[i];
+                bpo += b * sine[i];
+                bqo += b * cosine[i];
+            }
+            bp = wave_sum(bp) / blen;
+            bq = wave_sum(bq) / blen;
+            bpo = wave_sum(bpo) / blen;
+            bqo = wave_sum(bqo) / blen;
+            if (lane == 0) {
+                const float vs = (((bp - bpo) * (bp - bpo) + (bq - bqo) * (bq - bqo)) < (bp * bp + bq * bq) * 2) ? 1.0f : -1.0f;
+                float lbp = (bp - bqo) * 0.5f, lbq = (bq + bpo) * 0.5f;
+                const float norm = fmaxf(sqrtf(lbp * lbp + lbq * lbq), g.burst_floor);
+                lbp /= norm;
+                lbq /= norm;
+                L.bp[li] = (lbp * g.chroma_gain_cos - lbq * g.chroma_gain_sin);
+                L.bq[li] = (lbp * g.chroma_gain_sin + lbq * g.chroma_gain_cos);
+                L.vsw[li] = vs;
+            }
+        }
+    }
+
+    // Chroma of this thread's columns on field line `line`: the partial planes
+    // summed in order, zero outside the active area.
+    auto load_row = [&](int line) -> float4 {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (any_live && line >= firstLine && line < lastLine) {
+            const int64_t o = (int64_t)line * g.cpitch + kColOrigin + c0;
+            v = *reinterpret_cast<const float4 *>(planes[0] + o);
+            if (nc > 1) v = f4add(v, *reinterpret_cast<const float4 *>(planes[1] + o));
+            if (nc > 2) v = f4add(v, *reinterpret_cast<const float4 *>(planes[2] + o));
+            if (nc > 3) v = f4add(v, *reinterpret_cast<const float4 *>(planes[3] + o));
+            if (c0 + 4 > g.aw) {
+                if (c0 + 1 >= g.aw) v.y = 0.0f;
+                if (c0 + 2 >= g.aw) v.z = 0.0f;
+                v.w = 0.0f;
+            }
+        }
+        return v;
+    };
+
+    // carrier (exactly one of sine/cosine is non-zero, +-1) of the 4 columns
+    float w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int col = g.avs + c0 + e;
+        w[e] = (any_live && col < g.fw) ? sine[col] + cosine[col] : 0.0f;
+    }
+    float cf[kQHalo + 1][4];
+#pragma unroll
+    for (int b = 0; b <= kQHalo; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cf[b][j] = (b <= fs) ? cfilt[b * 4 + j] : 0.0f;
+    // columns c0 + e + b of the sine parity: ((e + b) & 1) == sel for every thread
+    const int sel = g.spar ^ ((g.avs + c0) & 1);
+
+    const bool is_float = (g.pixel_format == 1 /*YUV444PS*/ || g.pixel_format == 5 /*GRAYS*/);
+    const bool luma_only = (g.pixel_format == 4 /*GRAY16*/ || g.pixel_format == 5 /*GRAYS*/);
+    const int64_t plane = (int64_t)g.out_w * g.out_h;
+    float eyLo = -INFINITY, eyHi = INFINITY, ecLo = -INFINITY, ecHi = INFINITY;
+    float yLo = 0.0f, yHi = 65535.0f, cLo = 0.0f, cHi = 65535.0f;
+    if (g.clamp_mode == 1) {
+        eyLo = 0.0f; eyHi = 1.0f; ecLo = -0.5f; ecHi = 0.5f;
+        yLo = 16.0f * 256.0f; yHi = 235.0f * 256.0f; cLo = 16.0f * 256.0f; cHi = 240.0f * 256.0f;
+    } else if (g.clamp_mode == 3) {
+        eyLo = -15.0f / 219.0f; eyHi = 238.75f / 219.0f; ecLo = -127.0f / 224.0f; ecHi = 126.75f / 224.0f;
+        yLo = 256.0f; yHi = 254.75f * 256.0f; cLo = 256.0f; cHi = 254.75f * 256.0f;
+    }
+
+    // rows n0-3 .. n0+3 of this thread's columns
+    float4 r[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r[k] = load_row(n0 - 3 + k);
+
+    for (int li = 0; li < nl; ++li) {
+        const int number = n0 + li;
+        const int buf = li & 1;
+        const float4 nxt = load_row(number + 4);
+        {
+            float4 t[4];
+            float *t0 = &t[0].x, *t1 = &t[1].x, *t2 = &t[2].x, *t3 = &t[3].x;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                t0[e] = f4get(r[3], e) * w[e];
+                t1[e] = -(f4get(r[1], e) + f4get(r[5], e)) * w[e];
+                t2[e] = (f4get(r[2], e) - f4get(r[4], e)) * w[e];
+                t3[e] = (f4get(r[6], e) - f4get(r[0], e)) * w[e];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) L.t[buf][k][2 + tid] = t[k];
+        }
+        __syncthreads();
+        if (any_live) {
+            // FIR over t_k at columns c0-8 .. c0+11, in two passes (t0,t1 then
+            // t2,t3) to halve the live registers; per output the additions
+            // keep pal_demod_quad's order.
+            float A01[2][4], A23[2][4];   // [tap parity (e + b) & 1][column e]
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                float T[2][20];
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) {
+                        const float4 q = L.t[buf][2 * half + k][tid + m];
+                        T[k][4 * m] = q.x; T[k][4 * m + 1] = q.y; T[k][4 * m + 2] = q.z; T[k][4 * m + 3] = q.w;
+                    }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float ev = 0.0f, od = 0.0f;
+#pragma unroll
+                    for (int b = 0; b <= kQHalo; ++b) {
+                        const int ci = 8 + e;
+                        const float sa = T[0][ci + b] + T[0][ci - b];
+                        const float sb = T[1][ci + b] + T[1][ci - b];
+                        const float a = fmaf(sb, cf[b][2 * half + 1], sa * cf[b][2 * half]);
+                        if (((e + b) & 1) == 0) ev += a;
+                        else od += a;
+                    }
+                    (half == 0 ? A01 : A23)[0][e] = ev;
+                    (half == 0 ? A01 : A23)[1][e] = od;
+                }
+            }
+            const float bp = L.bp[li], bq = L.bq[li], vsw = L.vsw[li];
+            float compv[4];
+            if (comp != nullptr) {
+                const uint2 cw = *reinterpret_cast<const uint2 *>(comp + (int64_t)number * g.pitch + g.lmargin + g.avs + c0);
+                compv[0] = (float)(cw.x & 0xFFFFu); compv[1] = (float)(cw.x >> 16);
+                compv[2] = (float)(cw.y & 0xFFFFu); compv[3] = (float)(cw.y >> 16);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) compv[e] = g.black;
+            }
+            const int frame_line = number * 2 + off;
+            const int orow = frame_line - g.first + g.pad_top;
+            const int64_t obase = (int64_t)orow * g.out_w + g.pad_left + c0;
+            // taps with e + b even sit on sine columns (m terms: P01, Q23) when sel == 0
+            const int ps = sel, pc = sel ^ 1;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float P01 = A01[ps][e], Q01 = A01[pc][e], Q23 = A23[ps][e], P23 = A23[pc][e];
+                const float PU = P01 + P23, PV = P01 - P23, QU = Q01 - Q23, QV = Q01 + Q23;
+                const float Y = compv[e] - f4get(r[3], e);
+                const float U = -(PU * bp + QU * bq) * 2.0f;
+                const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
+                const float ey = (Y - g.black) * g.ey_scale;
+                const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
+                if (c0 + e < g.aw) {
+                    const int64_t o = obase + e;
+                    if (is_float) {
+                        float *out = reinterpret_cast<float *>(job.out);
+                        out[o] = fminf(fmaxf(ey, eyLo), eyHi);
+                        if (!luma_only) {
+                            out[plane + o] = fminf(fmaxf(ecb, ecLo), ecHi);
+                            out[2 * plane + o] = fminf(fmaxf(ecr, ecLo), ecHi);
+                        }
+                    } else {
+                        uint16_t *out = reinterpret_cast<uint16_t *>(job.out);
+                        out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), yLo), yHi);
+                        if (!luma_only) {
+                            out[plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecb + 128.0f * 256.0f), cLo), cHi);
+                            out[2 * plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecr + 128.0f * 256.0f), cLo), cHi);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r[k] = r[k + 1];
+        r[6] = nxt;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Packing and border fill.
 // ---------------------------------------------------------------------------
 
@@ -1387,7 +1628,15 @@ void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *
                   const float *cfilt, int n_lblocks, hipStream_t s)
 {
     const int n_segs = (g.aw + kDemodThreads - 1) / kDemodThreads;
-    if (g.quad) {
+    if (g.quad && g.aw <= kQ4Cols) {
+        const int nlb = (n_lblocks * kDemodLines + kQ4Lines - 1) / kQ4Lines;
+        const int blocks = n_jobs * 2 * nlb;
+        if (blocks > 0)
+            hipLaunchKernelGGL(pal_demod_quad4, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt,
+                               nlb);
+        return;
+    }
+    if (g.quad) {  // active lines wider than one column-blocked workgroup
         const int nlb = (n_lblocks * kDemodLines + kQLines - 1) / kQLines;
         const int blocks = n_jobs * 2 * nlb * n_segs;
         if (blocks > 0)

@@ -55,7 +55,10 @@ def test_transform2d_full_frames(orc, bars_capture, fmt):
 @pytest.mark.parametrize("crop", [dict(first_active_frame_line=101, last_active_frame_line=170,
                                        first_active_sample=333, last_active_sample=470),
                                   dict(first_active_frame_line=0, last_active_frame_line=624,
-                                       first_active_sample=0, last_active_sample=1134)])
+                                       first_active_sample=0, last_active_sample=1134),
+                                  # 4fSC demod of a line wider than one column-blocked workgroup
+                                  dict(first_active_frame_line=40, last_active_frame_line=600,
+                                       first_active_sample=9, last_active_sample=1120)])
 def test_crop_padding_clamp(orc, patch_capture, kind, crop):
     path, fields = patch_capture
     v = abi.Video(path)
