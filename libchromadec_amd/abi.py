@@ -25,6 +25,8 @@ STATUS = {0: "CHD_OK", 1: "CHD_E_INVALID_ARG", 2: "CHD_E_FILE_NOT_FOUND", 3: "CH
           10: "CHD_E_NN_MODEL_LOAD", 11: "CHD_E_NN_BACKEND_UNAVAILABLE", 12: "CHD_E_NN_INFERENCE",
           13: "CHD_E_OUT_OF_RANGE", 14: "CHD_E_CANCELLED", 15: "CHD_E_INTERNAL", 16: "CHD_E_OOM",
           17: "CHD_E_UNSUPPORTED"}
+CHD_DEC_AUTO = 0
+CHD_DEC_PAL_2D = 6
 CHD_DEC_TRANSFORM_2D = 7
 CHD_DEC_TRANSFORM_3D = 8
 CHD_PIXEL_YUV444P16, CHD_PIXEL_YUV444PS, CHD_PIXEL_GRAY16, CHD_PIXEL_GRAYS = 0, 1, 4, 5

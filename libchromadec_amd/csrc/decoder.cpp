@@ -686,9 +686,9 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
         return CHD_OK;
     }
 
-    if (resolved != CHD_DEC_TRANSFORM_2D && resolved != CHD_DEC_TRANSFORM_3D)
-        return fail(CHD_E_DECODER_UNKNOWN,
-                    fn + ": unknown or unsupported decoder kind (this build decodes CHD_DEC_TRANSFORM_2D/3D)");
+    if (resolved != CHD_DEC_PAL_2D && resolved != CHD_DEC_TRANSFORM_2D && resolved != CHD_DEC_TRANSFORM_3D)
+        return fail(CHD_E_DECODER_UNKNOWN, fn + ": unknown or unsupported decoder kind (this build decodes "
+                                                "CHD_DEC_PAL_2D and CHD_DEC_TRANSFORM_2D/3D)");
     if (fmt_ok == 0)
         return fail(CHD_E_UNSUPPORTED, fn + ": output_format not produced by this build "
                                             "(yuv444p16, yuv444ps, gray16, grays are)");
@@ -742,7 +742,7 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
             while (in >> v) thr.push_back(v);
         }
         const size_t want = (resolved == CHD_DEC_TRANSFORM_3D) ? 768 : 80;
-        if (!thr.empty() && thr.size() != want)
+        if (resolved != CHD_DEC_PAL_2D && !thr.empty() && thr.size() != want)
             return fail(CHD_E_INVALID_ARG, fn + ": transform_thresholds_file holds " + std::to_string(thr.size()) +
                                                " values, the decoder needs " + std::to_string(want));
     }

@@ -94,8 +94,9 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
 {
     const std::string W = who;
     if (p == nullptr || out == nullptr) return chd::fail(CHD_E_INVALID_ARG, W + ": null argument");
-    if (p->kind != CHD_DEC_TRANSFORM_2D && p->kind != CHD_DEC_TRANSFORM_3D)
-        return chd::fail(CHD_E_DECODER_UNKNOWN, W + ": kind must be CHD_DEC_TRANSFORM_2D or CHD_DEC_TRANSFORM_3D");
+    if (p->kind != CHD_DEC_TRANSFORM_2D && p->kind != CHD_DEC_TRANSFORM_3D && p->kind != CHD_DEC_PAL_2D)
+        return chd::fail(CHD_E_DECODER_UNKNOWN,
+                         W + ": kind must be CHD_DEC_PAL_2D, CHD_DEC_TRANSFORM_2D or CHD_DEC_TRANSFORM_3D");
     if (p->pixel_format != CHD_PIXEL_YUV444PS && p->pixel_format != CHD_PIXEL_YUV444P16 &&
         p->pixel_format != CHD_PIXEL_GRAYS && p->pixel_format != CHD_PIXEL_GRAY16)
         return chd::fail(CHD_E_UNSUPPORTED, W + ": pixel format not supported by the GPU decoder");
@@ -178,10 +179,13 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
         return chd::fail(CHD_E_UNSUPPORTED, W + ": sample rate needs a chroma filter wider than the GPU kernel supports");
     }
     g.filter_size = fs;
-    std::vector<float> cfilt((tpal::kMaxFilter + 1) * 4, 0.0f);
+    // chroma taps cfilt and luma taps yfilt (the luma filter has the same
+    // width: ya = ca), interleaved per tap with stride kFiltStride
+    constexpr int FS = tpal::kFiltStride;
+    std::vector<float> cfilt((tpal::kMaxFilter + 1) * FS, 0.0f);
     {
-        std::vector<double> cf((fs + 1) * 4, 0.0);
-        double cdiv = 0;
+        std::vector<double> cf((fs + 1) * 4, 0.0), yf((fs + 1) * 2, 0.0);
+        double cdiv = 0, ydiv = 0;
         for (int f = 0; f <= fs; ++f) {
             const double fc = std::min(ca, (double)f);
             const double ff = std::min(ca, std::sqrt((double)f * f + 2 * 2));
@@ -193,8 +197,15 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
             cf[f * 4 + 1] = (1 + std::cos(M_PI * fff / ca)) / d;
             cf[f * 4 + 3] = (1 + std::cos(M_PI * ffff / ca)) / d;
             cdiv += 2 * (cf[f * 4 + 0] + 2 * cf[f * 4 + 2] + 2 * cf[f * 4 + 1] + 2 * cf[f * 4 + 3]);
+            // Y filter: lines 0 and +-2 only, the outer taps weighted 0.2
+            yf[f * 2 + 0] = (1 + std::cos(M_PI * fc / ca)) / d;
+            yf[f * 2 + 1] = 0.2 * (1 + std::cos(M_PI * fff / ca)) / d;
+            ydiv += 2 * (yf[f * 2 + 0] + 2 * yf[f * 2 + 1]);
         }
-        for (int i = 0; i < (fs + 1) * 4; ++i) cfilt[i] = (float)(cf[i] / cdiv);
+        for (int f = 0; f <= fs; ++f) {
+            for (int j = 0; j < 4; ++j) cfilt[f * FS + j] = (float)(cf[f * 4 + j] / cdiv);
+            for (int j = 0; j < 2; ++j) cfilt[f * FS + 4 + j] = (float)(yf[f * 2 + j] / ydiv);
+        }
     }
     std::vector<float> sine(g.fw), cosine(g.fw);
     // At sampleRate = 4 fSC the carrier samples are exactly 0 and +-1 (the
@@ -232,7 +243,9 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     if (st != CHD_OK) { delete plan; return st; }
 
     plan->max_frames = p->max_batch_frames > 0 ? p->max_batch_frames : 64;
-    // workspace units: 3D slabs (frames + 2 per run), 2D fields
+    // workspace units: 3D slabs (frames + 2 per run), 2D fields; PAL_2D
+    // decodes straight from the fields but keeps field units for the
+    // separated-chroma entry point
     if (p->kind == CHD_DEC_TRANSFORM_3D) {
         plan->max_units = plan->max_frames + 2 * std::min(kMaxRunsPerChunk, plan->max_frames);
         plan->unit_floats = 8 * (size_t)g.part_plane;
@@ -281,11 +294,11 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
 static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hipStream_t s)
 {
     const Geom &g = plan->g;
-    hipError_t e;
+    hipError_t e = hipSuccess;
     if (plan->kind == CHD_DEC_TRANSFORM_3D) {
         e = hipMemcpyAsync(plan->d_slab_jobs, plan->h_slab_jobs, n_units * sizeof(tpal::SlabJob),
                            hipMemcpyHostToDevice, s);
-    } else {
+    } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
         e = hipMemcpyAsync(plan->d_field_jobs, plan->h_field_jobs, n_units * sizeof(tpal::FieldJob2D),
                            hipMemcpyHostToDevice, s);
     }
@@ -311,11 +324,11 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hi
                             plan->d_spec, plan->d_mask);
     } else if (plan->kind == CHD_DEC_TRANSFORM_3D)
         tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s);
-    else
+    else if (plan->kind == CHD_DEC_TRANSFORM_2D)
         tpal::launch_tpal2d(g, plan->d_field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
     if (plan->timing) hipEventRecord(plan->ev[1], s);
     tpal::launch_demod(g, plan->d_frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
-                       plan->n_lblocks, s);
+                       plan->n_lblocks, s, plan->kind == CHD_DEC_PAL_2D);
     if (plan->timing) hipEventRecord(plan->ev[2], s);
     e = hipGetLastError();
     if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: kernel launch", e));
@@ -386,7 +399,7 @@ chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n
                     fj.chroma[fsel][3] = b + g.part_plane;
                 }
                 fj.nchroma = 4;
-            } else {
+            } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
                 for (int fsel = 0; fsel < 2; ++fsel) {
                     tpal::FieldJob2D &jj = plan->h_field_jobs[2 * i + fsel];
                     jj.field = win.field[fsel];
@@ -398,6 +411,12 @@ chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n
                     fj.chroma[fsel][1] = jj.partial + g.part_plane;
                 }
                 fj.nchroma = 2;
+            } else {
+                for (int fsel = 0; fsel < 2; ++fsel) {
+                    fj.field[fsel] = win.field[fsel];
+                    fj.offset[fsel] = win.offset[fsel];
+                }
+                fj.nchroma = 0;
             }
         }
         chd_status_t st = run_chunk(plan, 2 * cnt, cnt, s);
@@ -525,7 +544,7 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
                     sj.partial = plan->d_partials + (size_t)(units + k) * plan->unit_floats;
                 }
                 units += cnt + 2;
-            } else {
+            } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
                 for (int k = 0; k < 2 * cnt; ++k) {
                     tpal::FieldJob2D &jj = plan->h_field_jobs[units + k];
                     jj.field = field_ptr(q, 2 * f + k);
@@ -533,6 +552,8 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
                     jj.partial = plan->d_partials + (size_t)(units + k) * plan->unit_floats;
                 }
                 units += 2 * cnt;
+            } else {
+                units += 2 * cnt;   // PAL_2D: no transform work, the budget stays per field
             }
             for (int i = 0; i < cnt; ++i) {
                 tpal::FrameJob &fj = plan->h_frame_jobs[frames + i];
@@ -549,13 +570,13 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
                         fj.chroma[fsel][1] = a + g.part_plane;
                         fj.chroma[fsel][2] = b;
                         fj.chroma[fsel][3] = b + g.part_plane;
-                    } else {
+                    } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
                         const float *a = plan->h_field_jobs[unit0 + 2 * i + fsel].partial;
                         fj.chroma[fsel][0] = a;
                         fj.chroma[fsel][1] = a + g.part_plane;
                     }
                 }
-                fj.nchroma = is3d ? 4 : 2;
+                fj.nchroma = is3d ? 4 : (plan->kind == CHD_DEC_TRANSFORM_2D ? 2 : 0);
             }
             frames += cnt;
             out_index += cnt;
@@ -678,7 +699,7 @@ chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: job upload", e));
         plan->staging_pending = true;
         tpal::launch_demod(g, plan->d_frame_jobs, cnt, plan->d_sine, plan->d_cosine, plan->d_cfilt, plan->n_lblocks,
-                           s);
+                           s, false);
         e = hipGetLastError();
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: launch", e));
     }
@@ -727,6 +748,7 @@ void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
 {
     chd_tpal_plan *plan = b->plan;
     const Geom &g = plan->g;
+    if (plan->kind == CHD_DEC_PAL_2D) return;   // the demodulator filters the composite itself
     if (plan->kind != CHD_DEC_TRANSFORM_3D) {
         tpal::launch_tpal2d(g, b->d_field, b->n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
         return;
@@ -755,7 +777,7 @@ void batch_all(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
     if (b->border) tpal::launch_fill_border(plan->g, b->d_out, b->n_frames, s);
     batch_tiles(b, s, strip, groups);
     tpal::launch_demod(plan->g, b->d_frame, b->n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
-                       plan->n_lblocks, s);
+                       plan->n_lblocks, s, plan->kind == CHD_DEC_PAL_2D);
 }
 
 hipError_t batch_capture(chd_tpal_batch *b)
@@ -818,7 +840,7 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
                 sj.partial = reinterpret_cast<float *>((unit0 + k) * plan->unit_floats);  // offset for now
                 slabs.push_back(sj);
             }
-        } else {
+        } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
             for (int64_t k = 0; k < 2 * q.count; ++k) {
                 tpal::FieldJob2D jj{};
                 jj.field = field_ptr(2 * q.first_frame + k);
@@ -840,13 +862,13 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
                     fj.chroma[fsel][1] = reinterpret_cast<float *>(a + g.part_plane);
                     fj.chroma[fsel][2] = reinterpret_cast<float *>(bb);
                     fj.chroma[fsel][3] = reinterpret_cast<float *>(bb + g.part_plane);
-                } else {
+                } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
                     const size_t a = (unit0 + 2 * i + fsel) * plan->unit_floats;
                     fj.chroma[fsel][0] = reinterpret_cast<float *>(a);
                     fj.chroma[fsel][1] = reinterpret_cast<float *>(a + g.part_plane);
                 }
             }
-            fj.nchroma = is3d ? 4 : 2;
+            fj.nchroma = is3d ? 4 : (plan->kind == CHD_DEC_TRANSFORM_2D ? 2 : 0);
             frames.push_back(fj);
         }
     }
@@ -857,7 +879,9 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
     b->n_frames = (int)frames.size();
     b->d_out = d_out;
     b->border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
-    hipError_t e = hipMalloc((void **)&b->d_partials, (size_t)b->n_units * plan->unit_floats * sizeof(float));
+    hipError_t e = hipSuccess;
+    if (b->n_units > 0)
+        e = hipMalloc((void **)&b->d_partials, (size_t)b->n_units * plan->unit_floats * sizeof(float));
     auto rebase = [&](const float *off) { return b->d_partials + reinterpret_cast<size_t>(off); };
     if (e == hipSuccess) {
         for (auto &sj : slabs) sj.partial = rebase(sj.partial);
@@ -869,11 +893,12 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
         }
     }
     if (e == hipSuccess && is3d) e = hipMalloc((void **)&b->d_slab, slabs.size() * sizeof(tpal::SlabJob));
-    if (e == hipSuccess && !is3d) e = hipMalloc((void **)&b->d_field, fjobs.size() * sizeof(tpal::FieldJob2D));
+    if (e == hipSuccess && !fjobs.empty())
+        e = hipMalloc((void **)&b->d_field, fjobs.size() * sizeof(tpal::FieldJob2D));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_frame, frames.size() * sizeof(tpal::FrameJob));
     if (e == hipSuccess && is3d)
         e = hipMemcpy(b->d_slab, slabs.data(), slabs.size() * sizeof(tpal::SlabJob), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !is3d)
+    if (e == hipSuccess && !fjobs.empty())
         e = hipMemcpy(b->d_field, fjobs.data(), fjobs.size() * sizeof(tpal::FieldJob2D), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(b->d_frame, frames.data(), frames.size() * sizeof(tpal::FrameJob), hipMemcpyHostToDevice);

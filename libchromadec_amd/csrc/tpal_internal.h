@@ -76,9 +76,14 @@ struct FrameJob {
     const uint16_t *field[2];
     int32_t offset[2];
     const float *chroma[2][4];    // per field: partial planes summed in order
-    int32_t nchroma;
+    int32_t nchroma;              // 0: CHD_DEC_PAL_2D, the filter runs on the composite itself
     void *out;                    // planes x out_h x out_w
 };
+
+// Demodulator filter table: per tap b, kFiltStride floats: the chroma taps
+// cfilt[b][0..3] (lines 0, +-2, +-1, +-3), then the luma taps yfilt[b][0..1]
+// (lines 0, +-2) used when decoding the raw composite.
+constexpr int kFiltStride = 8;
 
 // Constant tables of the tile transforms (device __constant__ copy c_tab).
 struct Tables {
@@ -111,8 +116,9 @@ void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *
                    int strip = kStrip3, int max_groups = 0);
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
                    const double *thr64, hipStream_t s);
+// composite: every job has nchroma == 0 (CHD_DEC_PAL_2D filters the composite)
 void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
-                  const float *cfilt, int n_lblocks, hipStream_t s);
+                  const float *cfilt, int n_lblocks, hipStream_t s, bool composite);
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
                  hipStream_t s);
 void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s);

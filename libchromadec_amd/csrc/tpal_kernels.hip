@@ -958,16 +958,24 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
         const int line = n0 - 3 + r;
         const int col = s0 + cc;
         float v = 0.0f;
-        if (line >= firstLine && line < lastLine && col >= g.avs && col < g.ave) {
-            const int64_t o = (int64_t)line * g.cpitch + (col - g.avs + kColOrigin);
-            const int nc = job.nchroma;
-            v = job.chroma[fsel][0][o];
-            if (nc > 1) v += job.chroma[fsel][1][o];
-            if (nc > 2) v += job.chroma[fsel][2][o];
-            if (nc > 3) v += job.chroma[fsel][3][o];
+        const int nc = job.nchroma;
+        if (line >= firstLine && line < lastLine) {
+            if (nc == 0) {
+                // CHD_DEC_PAL_2D: the filter input is the composite itself
+                // (decodeLine<uint16_t, false>), real samples in the halo too
+                if (col >= 0 && col < g.fw)
+                    v = comp != nullptr ? (float)comp[(int64_t)line * g.pitch + col + g.lmargin] : g.black;
+            } else if (col >= g.avs && col < g.ave) {
+                const int64_t o = (int64_t)line * g.cpitch + (col - g.avs + kColOrigin);
+                v = job.chroma[fsel][0][o];
+                if (nc > 1) v += job.chroma[fsel][1][o];
+                if (nc > 2) v += job.chroma[fsel][2][o];
+                if (nc > 3) v += job.chroma[fsel][3][o];
+            }
         }
         L.chroma[r][cc] = v;
     }
+    const bool pal2d = job.nchroma == 0;
 
     // ---- burst detection per line (palcolour.cpp:344-409) -------------------
     {
@@ -1049,13 +1057,13 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
         __syncthreads();
         if (col_live) {
             const float bp = L.bp[li], bq = L.bq[li], vsw = L.vsw[li];
-            float PU = 0, QU = 0, PV = 0, QV = 0;
+            float PU = 0, QU = 0, PV = 0, QV = 0, PY = 0, QY = 0;
             for (int b = 0; b <= fs; ++b) {
                 // taps clamp into the populated halo (palcolour.cpp:587-588)
                 const int l = max(fillStart, i - b) - s0;
                 const int r = min(fillEnd - 1, i + b) - s0;
-                const float c0 = cfilt[b * 4 + 0], c1 = cfilt[b * 4 + 1];
-                const float c2 = cfilt[b * 4 + 2], c3 = cfilt[b * 4 + 3];
+                const float c0 = cfilt[b * kFiltStride + 0], c1 = cfilt[b * kFiltStride + 1];
+                const float c2 = cfilt[b * kFiltStride + 2], c3 = cfilt[b * kFiltStride + 3];
                 const float m0 = L.mn[0][r] + L.mn[0][l], m1 = L.mn[1][r] + L.mn[1][l];
                 const float m2 = L.mn[2][r] + L.mn[2][l], m3 = L.mn[3][r] + L.mn[3][l];
                 const float q0 = L.mn[4][r] + L.mn[4][l], q1 = L.mn[5][r] + L.mn[5][l];
@@ -1066,9 +1074,16 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
                 QU += common_q - m2 * c2 - m3 * c3;
                 PV += common_p - q2 * c2 - q3 * c3;
                 QV += common_q + m2 * c2 + m3 * c3;
+                if (pal2d) {
+                    const float y0 = cfilt[b * kFiltStride + 4], y1 = cfilt[b * kFiltStride + 5];
+                    PY += m0 * y0 + m1 * y1;
+                    QY += q0 * y0 + q1 * y1;
+                }
             }
             const float compv = (comp != nullptr) ? (float)comp[(int64_t)number * g.pitch + i + g.lmargin] : g.black;
-            const float Y = compv - L.chroma[r0][i - s0];
+            // luma: composite minus the pre-filtered chroma, or for PAL_2D minus
+            // the chroma re-synthesised from the Y filter (palcolour.cpp:622-630)
+            const float Y = pal2d ? compv - (PY * sine[i] + QY * cosine[i]) * 2.0f : compv - L.chroma[r0][i - s0];
             const float U = -(PU * bp + QU * bq) * 2.0f;
             const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
             const int frame_line = number * 2 + off;
@@ -1215,7 +1230,7 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
 #pragma unroll
     for (int b = 0; b <= kQHalo; ++b)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cf[b][j] = (b <= fs) ? cfilt[b * 4 + j] : 0.0f;
+        for (int j = 0; j < 4; ++j) cf[b][j] = (b <= fs) ? cfilt[b * kFiltStride + j] : 0.0f;
 
     const int i = i0 + tid;
     const bool col_live = i < g.ave;
@@ -1285,18 +1300,21 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
 
 // ---------------------------------------------------------------------------
 // 4fSC demodulation, column-blocked.  One 256-thread workgroup = kQ4Lines
-// field lines x the whole active line; each thread owns 4 adjacent output
-// columns.  The thread keeps the 7 chroma rows its columns need (line n-3 ..
-// n+3) in registers and slides them down the block's lines, so each partial
-// plane is read once plus a 6-row halo per block, as float4.  The FIR inputs
-// t0..t3 of a line (same definitions and summation order as pal_demod_quad,
-// so results are identical) pass through LDS once, double-buffered: one
-// barrier per line.
+// field lines x the whole active line; thread t owns the 4 columns
+// 4t - 8 .. 4t - 5 (from active_video_start), so threads 0-1 hold the left
+// filter halo.  The thread keeps the 7 input rows its columns need (line
+// n-3 .. n+3) in registers and slides them down the block's lines, so each
+// partial plane is read once plus a 6-row halo per block, as float4.  The
+// FIR inputs t0..t3 of a line (the same definitions and summation order as
+// pal_demod_quad) pass through LDS once, double-buffered: one barrier per
+// line.  FrameJob.nchroma == 0 (CHD_DEC_PAL_2D) filters the composite itself
+// and takes luma from the re-synthesised chroma.
 // ---------------------------------------------------------------------------
 
 constexpr int kQ4Lines = 32;
-constexpr int kQ4Cols = 4 * kDemodThreads;             // 1024 >= active width
-constexpr int kQ4Row4 = (kQ4Cols + 2 * kQHalo) / 4;     // float4 per t row, zero halos of 8
+constexpr int kQ4Cols = 4 * kDemodThreads;             // columns held by the threads
+constexpr int kQ4MaxWidth = kQ4Cols - 2 * kQHalo;       // widest active line: halos on both sides
+constexpr int kQ4Row4 = kDemodThreads + 2;              // float4 per t row (+ 8 zero columns)
 
 struct LdsQ4 {
     float4 t[2][4][kQ4Row4];
@@ -1306,6 +1324,7 @@ struct LdsQ4 {
 __device__ __forceinline__ float4 f4add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 __device__ __forceinline__ float f4get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
 
+template <bool PAL2D>
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
                 const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks)
@@ -1323,15 +1342,19 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     if (n0 >= lastLine) return;
     const int nl = min(kQ4Lines, lastLine - n0);
     const int fs = g.filter_size;
-    const int c0 = 4 * tid;                  // this thread's first column, from active_video_start
-    const bool any_live = c0 < g.aw;
+    const int c0 = 4 * tid - kQHalo;         // this thread's first column, from active_video_start
     const int nc = job.nchroma;
+    constexpr bool pal2d = PAL2D;   // every job of the launch has nchroma == 0
+    // columns whose input this thread loads: the active line for separated
+    // chroma (zero elsewhere), the line plus the filter halo for PAL_2D
+    const bool loads = pal2d ? (c0 + 4 > -kQHalo && c0 < g.aw + kQHalo) : (c0 >= 0 && c0 < g.aw);
+    const bool outputs = c0 >= 0 && c0 < g.aw;
     const float *const *planes = job.chroma[fsel];
 
-    // zero halos of the t rows (columns -8..-1 and 1024..1031), both buffers
-    if (tid < 32) {
-        const int buf = tid >> 4, k = (tid >> 2) & 3, h = tid & 3;
-        L.t[buf][k][h < 2 ? h : kQ4Row4 - 4 + h] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // zero columns beyond the threads' range (both buffers, all 4 rows)
+    if (tid < 16) {
+        const int buf = tid >> 3, k = (tid >> 1) & 3, h = tid & 1;
+        L.t[buf][k][kDemodThreads + h] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 
     // ---- burst detection per line (palcolour.cpp:344-409) -------------------
@@ -1373,20 +1396,32 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
         }
     }
 
-    // Chroma of this thread's columns on field line `line`: the partial planes
-    // summed in order, zero outside the active area.
+    // Filter input of this thread's columns on field line `line`: the partial
+    // planes summed in order (zero outside the active area), or the composite
+    // (PAL_2D); zero on lines outside the field's active lines.
     auto load_row = [&](int line) -> float4 {
         float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (any_live && line >= firstLine && line < lastLine) {
-            const int64_t o = (int64_t)line * g.cpitch + kColOrigin + c0;
-            v = *reinterpret_cast<const float4 *>(planes[0] + o);
-            if (nc > 1) v = f4add(v, *reinterpret_cast<const float4 *>(planes[1] + o));
-            if (nc > 2) v = f4add(v, *reinterpret_cast<const float4 *>(planes[2] + o));
-            if (nc > 3) v = f4add(v, *reinterpret_cast<const float4 *>(planes[3] + o));
-            if (c0 + 4 > g.aw) {
-                if (c0 + 1 >= g.aw) v.y = 0.0f;
-                if (c0 + 2 >= g.aw) v.z = 0.0f;
-                v.w = 0.0f;
+        if (loads && line >= firstLine && line < lastLine) {
+            if (pal2d) {
+                if (comp == nullptr) {
+                    v = make_float4(g.black, g.black, g.black, g.black);
+                } else {
+                    const uint2 cw = *reinterpret_cast<const uint2 *>(comp + (int64_t)line * g.pitch + g.lmargin +
+                                                                     g.avs + c0);
+                    v = make_float4((float)(cw.x & 0xFFFFu), (float)(cw.x >> 16), (float)(cw.y & 0xFFFFu),
+                                    (float)(cw.y >> 16));
+                }
+            } else {
+                const int64_t o = (int64_t)line * g.cpitch + kColOrigin + c0;
+                v = *reinterpret_cast<const float4 *>(planes[0] + o);
+                if (nc > 1) v = f4add(v, *reinterpret_cast<const float4 *>(planes[1] + o));
+                if (nc > 2) v = f4add(v, *reinterpret_cast<const float4 *>(planes[2] + o));
+                if (nc > 3) v = f4add(v, *reinterpret_cast<const float4 *>(planes[3] + o));
+                if (c0 + 4 > g.aw) {
+                    if (c0 + 1 >= g.aw) v.y = 0.0f;
+                    if (c0 + 2 >= g.aw) v.z = 0.0f;
+                    v.w = 0.0f;
+                }
             }
         }
         return v;
@@ -1397,13 +1432,14 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int col = g.avs + c0 + e;
-        w[e] = (any_live && col < g.fw) ? sine[col] + cosine[col] : 0.0f;
+        w[e] = (loads && col >= 0 && col < g.fw) ? sine[col] + cosine[col] : 0.0f;
     }
-    float cf[kQHalo + 1][4];
+    constexpr int NCF = PAL2D ? 6 : 4;   // chroma taps, plus the luma taps for PAL_2D
+    float cf[kQHalo + 1][NCF];
 #pragma unroll
     for (int b = 0; b <= kQHalo; ++b)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cf[b][j] = (b <= fs) ? cfilt[b * 4 + j] : 0.0f;
+        for (int j = 0; j < NCF; ++j) cf[b][j] = (b <= fs) ? cfilt[b * kFiltStride + j] : 0.0f;
     // columns c0 + e + b of the sine parity: ((e + b) & 1) == sel for every thread
     const int sel = g.spar ^ ((g.avs + c0) & 1);
 
@@ -1440,14 +1476,14 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 t3[e] = (f4get(r[6], e) - f4get(r[0], e)) * w[e];
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) L.t[buf][k][2 + tid] = t[k];
+            for (int k = 0; k < 4; ++k) L.t[buf][k][tid] = t[k];
         }
         __syncthreads();
-        if (any_live) {
-            // FIR over t_k at columns c0-8 .. c0+11, in two passes (t0,t1 then
-            // t2,t3) to halve the live registers; per output the additions
-            // keep pal_demod_quad's order.
-            float A01[2][4], A23[2][4];   // [tap parity (e + b) & 1][column e]
+        if (outputs) {
+            // FIR over t_k at columns c0-8 .. c0+11 (float4 tid-2 .. tid+2), in
+            // two passes (t0,t1 then t2,t3) to halve the live registers; per
+            // output the additions keep pal_demod_quad's order.
+            float A01[2][4], A23[2][4], AY[2][4];   // [tap parity (e + b) & 1][column e]
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
                 float T[2][20];
@@ -1455,12 +1491,12 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 for (int k = 0; k < 2; ++k)
 #pragma unroll
                     for (int m = 0; m < 5; ++m) {
-                        const float4 q = L.t[buf][2 * half + k][tid + m];
+                        const float4 q = L.t[buf][2 * half + k][tid - 2 + m];
                         T[k][4 * m] = q.x; T[k][4 * m + 1] = q.y; T[k][4 * m + 2] = q.z; T[k][4 * m + 3] = q.w;
                     }
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    float ev = 0.0f, od = 0.0f;
+                    float ev = 0.0f, od = 0.0f, yev = 0.0f, yod = 0.0f;
 #pragma unroll
                     for (int b = 0; b <= kQHalo; ++b) {
                         const int ci = 8 + e;
@@ -1469,9 +1505,21 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                         const float a = fmaf(sb, cf[b][2 * half + 1], sa * cf[b][2 * half]);
                         if (((e + b) & 1) == 0) ev += a;
                         else od += a;
+                        if constexpr (PAL2D) {
+                            if (half == 0) {
+                                // Y filter: lines 0 and +-2 (t0, t1)
+                                const float ay = fmaf(sb, cf[b][5], sa * cf[b][4]);
+                                if (((e + b) & 1) == 0) yev += ay;
+                                else yod += ay;
+                            }
+                        }
                     }
                     (half == 0 ? A01 : A23)[0][e] = ev;
                     (half == 0 ? A01 : A23)[1][e] = od;
+                    if (half == 0) {
+                        AY[0][e] = yev;
+                        AY[1][e] = yod;
+                    }
                 }
             }
             const float bp = L.bp[li], bq = L.bq[li], vsw = L.vsw[li];
@@ -1487,13 +1535,17 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
             const int frame_line = number * 2 + off;
             const int orow = frame_line - g.first + g.pad_top;
             const int64_t obase = (int64_t)orow * g.out_w + g.pad_left + c0;
-            // taps with e + b even sit on sine columns (m terms: P01, Q23) when sel == 0
+            // taps with e + b of parity sel sit on sine columns (m terms: P01, Q23)
             const int ps = sel, pc = sel ^ 1;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float P01 = A01[ps][e], Q01 = A01[pc][e], Q23 = A23[ps][e], P23 = A23[pc][e];
                 const float PU = P01 + P23, PV = P01 - P23, QU = Q01 - Q23, QV = Q01 + Q23;
-                const float Y = compv[e] - f4get(r[3], e);
+                // PAL_2D luma: composite minus (PY sine + QY cosine) * 2, and
+                // column e is itself a sine column when (e & 1) == sel
+                float Y;
+                if constexpr (PAL2D) Y = compv[e] - ((e & 1) == sel ? AY[ps][e] : AY[pc][e]) * w[e] * 2.0f;
+                else Y = compv[e] - f4get(r[3], e);
                 const float U = -(PU * bp + QU * bq) * 2.0f;
                 const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
                 const float ey = (Y - g.black) * g.ey_scale;
@@ -1625,18 +1677,23 @@ void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_m
 }
 
 void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
-                  const float *cfilt, int n_lblocks, hipStream_t s)
+                  const float *cfilt, int n_lblocks, hipStream_t s, bool composite)
 {
     const int n_segs = (g.aw + kDemodThreads - 1) / kDemodThreads;
-    if (g.quad && g.aw <= kQ4Cols) {
+    if (g.quad && g.aw <= kQ4MaxWidth) {
         const int nlb = (n_lblocks * kDemodLines + kQ4Lines - 1) / kQ4Lines;
         const int blocks = n_jobs * 2 * nlb;
-        if (blocks > 0)
-            hipLaunchKernelGGL(pal_demod_quad4, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt,
-                               nlb);
+        if (blocks > 0) {
+            if (composite)
+                hipLaunchKernelGGL(pal_demod_quad4<true>, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine,
+                                   cosine, cfilt, nlb);
+            else
+                hipLaunchKernelGGL(pal_demod_quad4<false>, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine,
+                                   cosine, cfilt, nlb);
+        }
         return;
     }
-    if (g.quad) {  // active lines wider than one column-blocked workgroup
+    if (g.quad && !composite) {  // active lines wider than one column-blocked workgroup
         const int nlb = (n_lblocks * kDemodLines + kQLines - 1) / kQLines;
         const int blocks = n_jobs * 2 * nlb * n_segs;
         if (blocks > 0)

@@ -49,6 +49,7 @@ typedef struct orc_params {
     double ecb_from_u, ecr_from_v;
     int32_t clamp_mode; /* 0 none, 1 legal_rgb_sdr, 2 legal_rgb_hdr, 3 legal_ycbcr_bt601 */
     int32_t pad_left, pad_top, out_width, out_height;
+    double kr, kb;      /* luma coefficients of the colour-difference precision */
 } orc_params;
 
 /* ------------------------------------------------------------------------ */
@@ -552,9 +553,13 @@ void orc_detect_burst(const orc_params *p, int32_t number, const uint16_t *field
 }
 
 /* decodeField + decodeLine<double, true> (palcolour.cpp:307-335, 456-646).
- * Y/U/V are ComponentFrame planes of field_width x (2*field_height-1). */
+ * Y/U/V are ComponentFrame planes of field_width x (2*field_height-1).
+ * With pal2d set, `chroma` holds the composite field itself (as doubles) and
+ * the line is decoded as decodeLine<uint16_t, false> (the PALcolour 2D
+ * filter on the raw composite, palcolour.cpp:327-328, 525-609, 622-630):
+ * luma subtracts the chroma re-synthesised from the Y filter's P/Q. */
 static void decode_field(const orc_params *p, const pal_luts *L, const uint16_t *comp, int off,
-                         const double *chroma, double *Yp, double *Up, double *Vp)
+                         const double *chroma, int pal2d, double *Yp, double *Up, double *Vp)
 {
     static const double blackLine[ORC_MAX_WIDTH] = {0};
     const int fw = p->field_width;
@@ -597,10 +602,12 @@ static void decode_field(const orc_params *p, const pal_luts *L, const uint16_t 
         const int lineNumber = number * 2 + off;
         double *oY = Yp + lineNumber * fw, *oU = Up + lineNumber * fw, *oV = Vp + lineNumber * fw;
         for (int i = p->active_video_start; i < endPos; i++) {
-            double PU = 0, QU = 0, PV = 0, QV = 0;
+            double PU = 0, QU = 0, PV = 0, QV = 0, PY = 0, QY = 0;
             for (int b = 0; b <= fs; b++) {
                 const int l = imax(fillStart, i - b);
                 const int r = imin(fillEnd - 1, i + b);
+                PY += (m[0][r] + m[0][l]) * L->yfilt[b][0] + (m[1][r] + m[1][l]) * L->yfilt[b][1];
+                QY += (n[0][r] + n[0][l]) * L->yfilt[b][0] + (n[1][r] + n[1][l]) * L->yfilt[b][1];
                 PU += (m[0][r] + m[0][l]) * L->cfilt[b][0] + (m[1][r] + m[1][l]) * L->cfilt[b][1]
                     + (n[2][r] + n[2][l]) * L->cfilt[b][2] + (n[3][r] + n[3][l]) * L->cfilt[b][3];
                 QU += (n[0][r] + n[0][l]) * L->cfilt[b][0] + (n[1][r] + n[1][l]) * L->cfilt[b][1]
@@ -610,7 +617,7 @@ static void decode_field(const orc_params *p, const pal_luts *L, const uint16_t 
                 QV += (n[0][r] + n[0][l]) * L->cfilt[b][0] + (n[1][r] + n[1][l]) * L->cfilt[b][1]
                     + (m[2][r] + m[2][l]) * L->cfilt[b][2] + (m[3][r] + m[3][l]) * L->cfilt[b][3];
             }
-            oY[i] = cl[i] - in0[i];
+            oY[i] = pal2d ? cl[i] - ((PY * S[i] + QY * C[i]) * 2.0) : cl[i] - in0[i];
             oU[i] = -(PU * bp + QU * bq) * 2.0;
             oV[i] = Vsw * -(QV * bp - PV * bq) * 2.0;
         }
@@ -630,8 +637,28 @@ void orc_palcolour_decode_frame(const orc_params *p, const uint16_t *field0, int
     memset(V, 0, n * sizeof(double));
     pal_luts *L = malloc(sizeof(pal_luts));
     build_luts(p, L);
-    decode_field(p, L, field0, off0, chroma0, Y, U, V);
-    decode_field(p, L, field1, off1, chroma1, Y, U, V);
+    decode_field(p, L, field0, off0, chroma0, 0, Y, U, V);
+    decode_field(p, L, field1, off1, chroma1, 0, Y, U, V);
+    free(L);
+}
+
+/* PalColour::decodeFrames with separation palColourFilter (CHD_DEC_PAL_2D):
+ * each field decoded from its own composite (palcolour.cpp:279-335). */
+void orc_pal2d_decode_frame(const orc_params *p, const uint16_t *field0, int32_t off0, const uint16_t *field1,
+                            int32_t off1, double *Y, double *U, double *V)
+{
+    const size_t n = (size_t)p->field_width * (2 * p->field_height - 1);
+    const size_t fa = (size_t)p->field_width * p->field_height;
+    memset(Y, 0, n * sizeof(double));
+    memset(U, 0, n * sizeof(double));
+    memset(V, 0, n * sizeof(double));
+    pal_luts *L = malloc(sizeof(pal_luts));
+    double *c = malloc(2 * fa * sizeof(double));
+    build_luts(p, L);
+    for (size_t i = 0; i < fa; i++) { c[i] = field0[i]; c[fa + i] = field1[i]; }
+    decode_field(p, L, field0, off0, c, 1, Y, U, V);
+    decode_field(p, L, field1, off1, c + fa, 1, Y, U, V);
+    free(c);
     free(L);
 }
 
@@ -707,13 +734,113 @@ void orc_output_u16(const orc_params *p, const double *Y, const double *U, const
     }
 }
 
+/* R'G'B' from the composite-domain Y/U/V (color_conversion.cpp:33-67): the
+ * composite U/V reductions follow from ecbFromU = 1 / (2 (1 - kb) uReduction)
+ * and ecrFromV = 1 / (2 (1 - kr) vReduction). */
+static void rgb_coeffs(const orc_params *p, double *rV, double *gU, double *gV, double *bU)
+{
+    const double kg = 1.0 - p->kr - p->kb;
+    const double ured = 1.0 / (2.0 * (1.0 - p->kb) * p->ecb_from_u);
+    const double vred = 1.0 / (2.0 * (1.0 - p->kr) * p->ecr_from_v);
+    *rV = 1.0 / vred;
+    *gU = -(p->kb / kg) / ured;
+    *gV = -(p->kr / kg) / vred;
+    *bU = 1.0 / ured;
+}
+
+/* RGB48: interleaved R'G'B' u16, full range, truncated after the [0, 65535]
+ * clamp; black border (OutputWriter::convertLine, output_writer.cpp:431-452;
+ * fillBlack :371-378). */
+void orc_output_rgb48(const orc_params *p, const double *Y, const double *U, const double *V, uint16_t *out)
+{
+    const int W = p->out_width, H = p->out_height, fw = p->field_width;
+    memset(out, 0, (size_t)3 * W * H * sizeof(uint16_t));
+    double rV, gU, gV, bU;
+    rgb_coeffs(p, &rV, &gU, &gV, &bU);
+    const double yRange = p->white16b - p->black16b;
+    const double yScale = 65535.0 / yRange, uvScale = 65535.0 / yRange;
+    const int aw = p->active_video_end - p->active_video_start;
+    const int ah = p->last_active_frame_line - p->first_active_frame_line + 1;
+    for (int y = 0; y < ah; y++) {
+        const int il = p->first_active_frame_line + y;
+        const size_t ro = (size_t)W * (p->pad_top + y) + p->pad_left;
+        const double *iy = Y + (size_t)il * fw + p->active_video_start;
+        const double *iu = U + (size_t)il * fw + p->active_video_start;
+        const double *iv = V + (size_t)il * fw + p->active_video_start;
+        for (int x = 0; x < aw; x++) {
+            const double ry = dclamp((iy[x] - p->black16b) * yScale, 0.0, 65535.0);
+            const double ru = iu[x] * uvScale, rv = iv[x] * uvScale;
+            uint16_t *o = out + (ro + x) * 3;
+            o[0] = (uint16_t)dclamp(ry + rV * rv, 0.0, 65535.0);
+            o[1] = (uint16_t)dclamp(ry + gU * ru + gV * rv, 0.0, 65535.0);
+            o[2] = (uint16_t)dclamp(ry + bU * ru, 0.0, 65535.0);
+        }
+    }
+}
+
+/* RGBS: planar E'R/E'G/E'B floats, black 0, clamped to the mode's R'G'B'
+ * box (OutputWriter::convertToFloatRGB, output_writer.cpp:729-777; bounds
+ * floatRGBBounds :149-190, with the BT.601 Y'CbCr box projected through the
+ * inverse matrix). */
+void orc_output_rgbs(const orc_params *p, const double *Y, const double *U, const double *V, float *out)
+{
+    const int W = p->out_width, H = p->out_height, fw = p->field_width;
+    const size_t ps = (size_t)W * H;
+    memset(out, 0, 3 * ps * sizeof(float));
+    double rV, gU, gV, bU;
+    rgb_coeffs(p, &rV, &gU, &gV, &bU);
+    float lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
+    if (p->clamp_mode == 1) {
+        for (int c = 0; c < 3; c++) { lo[c] = 0.0f; hi[c] = 1.0f; }
+    } else if (p->clamp_mode == 2) {
+        for (int c = 0; c < 3; c++) lo[c] = 0.0f;
+    } else if (p->clamp_mode == 3) {
+        const float eyLo = -15.0f / 219.0f, eyHi = 238.75f / 219.0f;
+        const float ecLo = -127.0f / 224.0f, ecHi = 126.75f / 224.0f;
+        const double kg = 1.0 - p->kr - p->kb;
+        const double rds = 2.0 * (1.0 - p->kr), bds = 2.0 * (1.0 - p->kb);
+        const double g = (p->kb * bds) / kg + (p->kr * rds) / kg;
+        lo[0] = (float)(eyLo + rds * ecLo); hi[0] = (float)(eyHi + rds * ecHi);
+        lo[1] = (float)(eyLo - g * ecHi);   hi[1] = (float)(eyHi - g * ecLo);
+        lo[2] = (float)(eyLo + bds * ecLo); hi[2] = (float)(eyHi + bds * ecHi);
+    }
+    const double yRange = p->white16b - p->black16b;
+    const double eyScale = 1.0 / yRange, uvScale = 1.0 / yRange;
+    const int aw = p->active_video_end - p->active_video_start;
+    const int ah = p->last_active_frame_line - p->first_active_frame_line + 1;
+    for (int y = 0; y < ah; y++) {
+        const int il = p->first_active_frame_line + y;
+        const size_t ro = (size_t)W * (p->pad_top + y) + p->pad_left;
+        const double *iy = Y + (size_t)il * fw + p->active_video_start;
+        const double *iu = U + (size_t)il * fw + p->active_video_start;
+        const double *iv = V + (size_t)il * fw + p->active_video_start;
+        for (int x = 0; x < aw; x++) {
+            const double ry = (iy[x] - p->black16b) * eyScale;
+            const double ru = iu[x] * uvScale, rv = iv[x] * uvScale;
+            out[ro + x] = fclamp((float)(ry + rV * rv), lo[0], hi[0]);
+            out[ps + ro + x] = fclamp((float)(ry + gU * ru + gV * rv), lo[1], hi[1]);
+            out[2 * ps + ro + x] = fclamp((float)(ry + bU * ru), lo[2], hi[2]);
+        }
+    }
+}
+
+/* out_fmt 0 = float Y'CbCr planes, 1 = u16 Y'CbCr planes, 2 = RGB48, 3 = RGBS */
+static void write_output(const orc_params *p, const double *yuv, size_t ca, int32_t out_fmt, void *out)
+{
+    if (out == NULL) return;
+    if (out_fmt == 0) orc_output_float(p, yuv, yuv + ca, yuv + 2 * ca, (float *)out);
+    else if (out_fmt == 1) orc_output_u16(p, yuv, yuv + ca, yuv + 2 * ca, (uint16_t *)out);
+    else if (out_fmt == 2) orc_output_rgb48(p, yuv, yuv + ca, yuv + 2 * ca, (uint16_t *)out);
+    else orc_output_rgbs(p, yuv, yuv + ca, yuv + 2 * ca, (float *)out);
+}
+
 /* ------------------------------------------------------------------------ */
 /* End-to-end: one frame from a field window, as chd_decode_frame composes    */
 /* SourceField::loadFields -> PalColour::decodeFrames -> OutputWriter         */
 /* (c_api_decoder.cpp:183-332, 461-479, 497-515).                            */
 /* kind 7 = TRANSFORM_2D (window = the frame's 2 fields, start 0, end 2);     */
 /* kind 8 = TRANSFORM_3D (window of 14 fields, start 4, end 6).              */
-/* out_fmt 0 = float planes, 1 = u16 planes.  Optional chroma_out receives   */
+/* out_fmt as write_output.  Optional chroma_out receives                     */
 /* the two chroma fields (2 x field area doubles) and yuv_out the component  */
 /* frame (3 x field_width x (2*field_height-1) doubles).                      */
 /* ------------------------------------------------------------------------ */
@@ -727,6 +854,18 @@ static int decode_window(const orc_params *p, int32_t kind, const double *thr_sq
     const size_t ca = (size_t)p->field_width * (2 * p->field_height - 1);
     double *chroma = malloc(2 * fa * sizeof(double));
     double *outs[2] = {chroma, chroma + fa};
+    if (kind == 6) {
+        /* CHD_DEC_PAL_2D: no transform; the decoder filters the composite */
+        double *yuv = malloc(3 * ca * sizeof(double));
+        orc_pal2d_decode_frame(p, fields[start], offsets[start], fields[start + 1], offsets[start + 1], yuv,
+                               yuv + ca, yuv + 2 * ca);
+        write_output(p, yuv, ca, out_fmt, out);
+        if (chroma_out) memset(chroma_out, 0, 2 * fa * sizeof(double));
+        if (yuv_out) memcpy(yuv_out, yuv, 3 * ca * sizeof(double));
+        free(yuv);
+        free(chroma);
+        return 0;
+    }
     if (kind == 8) {
         if (start < 4 || nfields - end < 4) { free(chroma); return -2; }
         tpal3d_filter(p, fields, offsets, start, end, thr_sq, masks, outs);
@@ -739,10 +878,7 @@ static int decode_window(const orc_params *p, int32_t kind, const double *thr_sq
     double *yuv = malloc(3 * ca * sizeof(double));
     orc_palcolour_decode_frame(p, fields[start], offsets[start], fields[start + 1], offsets[start + 1],
                                outs[0], outs[1], yuv, yuv + ca, yuv + 2 * ca);
-    if (out != NULL) {
-        if (out_fmt == 0) orc_output_float(p, yuv, yuv + ca, yuv + 2 * ca, (float *)out);
-        else orc_output_u16(p, yuv, yuv + ca, yuv + 2 * ca, (uint16_t *)out);
-    }
+    write_output(p, yuv, ca, out_fmt, out);
     if (chroma_out) memcpy(chroma_out, chroma, 2 * fa * sizeof(double));
     if (yuv_out) memcpy(yuv_out, yuv, 3 * ca * sizeof(double));
     free(yuv);
@@ -796,7 +932,7 @@ int orc_decode_sequence(const orc_params *p, int32_t kind, const double *thr_sq,
             win[w] = (fi >= 0 && fi < n_seq_fields) ? seq + (size_t)fi * fa : blackf;
             offs[w] = w % 2;
         }
-        void *o = (out_fmt == 0) ? (void *)((float *)out + ps * i) : (void *)((uint16_t *)out + ps * i);
+        void *o = (out_fmt == 0 || out_fmt == 3) ? (void *)((float *)out + ps * i) : (void *)((uint16_t *)out + ps * i);
         const int rc = orc_decode_window(p, kind, thr_sq, win, offs, nwin, 2 * behind, 2 * behind + 2, out_fmt, o,
                                          NULL, NULL);
         if (rc != 0) rc_all = rc;

@@ -27,7 +27,19 @@ class OrcParams(C.Structure):
                 ("ecb_from_u", C.c_double), ("ecr_from_v", C.c_double),
                 ("clamp_mode", C.c_int32),
                 ("pad_left", C.c_int32), ("pad_top", C.c_int32),
-                ("out_width", C.c_int32), ("out_height", C.c_int32)]
+                ("out_width", C.c_int32), ("out_height", C.c_int32),
+                ("kr", C.c_double), ("kb", C.c_double)]
+
+
+# output formats of the oracle's OutputWriter: float Y'CbCr planes, u16
+# Y'CbCr planes, RGB48 (interleaved u16), RGBS (float R'G'B' planes)
+OUT_PS, OUT_P16, OUT_RGB48, OUT_RGBS = 0, 1, 2, 3
+
+
+def _out_array(p, out_fmt, lead=()):
+    dt = np.float32 if out_fmt in (OUT_PS, OUT_RGBS) else np.uint16
+    shape = (p.out_height, p.out_width, 3) if out_fmt == OUT_RGB48 else (3, p.out_height, p.out_width)
+    return np.zeros(tuple(lead) + shape, dtype=dt)
 
 
 def build(force: bool = False) -> str:
@@ -103,10 +115,11 @@ def pal_params(first_line=44, last_line=619, avs=185, ave=1107, black=0x4000, wh
                broadcast_scaling="scientific", field_width=1135, field_height=313,
                burst=(98, 138), fsc=4433618.75, sample_rate=17734475.0) -> OrcParams:
     ecb, ecr = color_conversion(color_difference, broadcast_scaling)
+    kr, kb = (0.30, 0.11) if color_difference == "classic" else (0.299, 0.114)
     aw, ah = ave - avs, last_line - first_line + 1
     return OrcParams(field_width, field_height, avs, ave, first_line, last_line, burst[0], burst[1],
                      black, white, fsc, sample_rate, chroma_gain, chroma_phase_deg, ecb, ecr,
-                     clamp_mode, pad_left, pad_top, out_width or aw, out_height or ah)
+                     clamp_mode, pad_left, pad_top, out_width or aw, out_height or ah, kr, kb)
 
 
 def uniform_thresholds(kind: int, threshold: float = 0.4) -> np.ndarray:
@@ -143,8 +156,7 @@ def decode_window(p: OrcParams, kind: int, fields, offsets, start, end, thr_sq, 
     arr, keep = _field_ptrs(fields)
     offs = np.ascontiguousarray(offsets, dtype=np.int32)
     thr = np.ascontiguousarray(thr_sq, dtype=np.float64)
-    shape = (3, p.out_height, p.out_width)
-    out = np.zeros(shape, dtype=np.float32 if out_fmt == 0 else np.uint16)
+    out = _out_array(p, out_fmt)
     chroma = np.zeros((2, p.field_height, p.field_width)) if want_chroma else None
     yuv = np.zeros((3, 2 * p.field_height - 1, p.field_width)) if want_yuv else None
     rc = L.orc_decode_window(C.byref(p), kind, _dp(thr), arr, offs.ctypes.data_as(C.POINTER(C.c_int32)),
@@ -190,7 +202,7 @@ def decode_frame_masked(p: OrcParams, seq_fields, frame, masks, out_fmt=0):
     arr, keep = _field_ptrs(fields)
     o = np.ascontiguousarray(offs, dtype=np.int32)
     m = np.ascontiguousarray(masks, dtype=np.float32)
-    out = np.zeros((3, p.out_height, p.out_width), dtype=np.float32 if out_fmt == 0 else np.uint16)
+    out = _out_array(p, out_fmt)
     rc = L.orc_decode_window_masked(C.byref(p), m.ctypes.data_as(C.POINTER(C.c_float)), arr,
                                     o.ctypes.data_as(C.POINTER(C.c_int32)), len(fields), s, e, out_fmt,
                                     out.ctypes.data_as(C.c_void_p), None)
@@ -226,7 +238,7 @@ def decode_sequence(p: OrcParams, kind: int, seq_fields, frames, thr_sq, out_fmt
     L = lib()
     seq = np.ascontiguousarray(seq_fields, dtype=np.uint16)
     fr = np.ascontiguousarray(frames, dtype=np.int32)
-    out = np.zeros((len(fr), 3, p.out_height, p.out_width), dtype=np.float32 if out_fmt == 0 else np.uint16)
+    out = _out_array(p, out_fmt, (len(fr),))
     thr = np.ascontiguousarray(thr_sq, dtype=np.float64)
     rc = L.orc_decode_sequence(C.byref(p), kind, _dp(thr), _u16p(seq), seq.shape[0],
                                fr.ctypes.data_as(C.POINTER(C.c_int32)), len(fr), out_fmt,

@@ -16,6 +16,8 @@ float planes (full scale: E'Y 0..1, E'Cb/E'Cr +-0.5) and in yuv444p16 LSB:
   stress   24 frames of the stress capture (30 dB, phase jitter, fast
         motion, dropouts concealed by the library, whose concealment is also
         compared bit for bit with the oracle's), plain and learned
+  PAL 2D   the 64-frame sequence with CHD_DEC_PAL_2D (the PALcolour 2D
+        filter on the raw composite, no transform)
 
 The oracle is the checker here; nothing it computes feeds the GPU path.
 """
@@ -121,6 +123,16 @@ def config_2d():
     return "2D, 1 frame (bars + gradient)", [0], res
 
 
+def config_pal2d(fields):
+    frames = [0, 1, 31, 62, 63]
+    res = {}
+    for fmt in ("ps", "p16"):
+        plan = _plan(abi.CHD_DEC_PAL_2D, fmt, 64)
+        got = _decode(plan, _pack(plan, fields), 64, fmt)
+        res[fmt] = _stats(got[frames], _oracle_plain(6, fields, frames, fmt), fmt == "p16")
+    return "PAL 2D (PALcolour filter, no transform), 64-frame sequence", frames, res
+
+
 def config_3d(fields):
     frames = [0, 1, 2, 31, 32, 61, 62, 63]
     res = {}
@@ -201,7 +213,8 @@ def main():
     import bench
     fields = bench.make_workload(64)[0]
     lines = ["| Config | Frames checked | Output | E'Y max | U/V max | U/V RMS |", "|---|---|---|---|---|---|"]
-    for fn in (config_2d, lambda: config_3d(fields), config_batch, lambda: config_learned(fields), config_stress):
+    for fn in (config_2d, lambda: config_3d(fields), config_batch, lambda: config_learned(fields), config_stress,
+               lambda: config_pal2d(fields)):
         name, frames, res = fn()
         for fmt, r in res.items():
             if r["u16"]:

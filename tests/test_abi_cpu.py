@@ -195,6 +195,12 @@ def test_commit_kind_checks(bars_capture):
     with pytest.raises(abi.ChdError) as e:
         abi.Decoder(v, 4).commit()  # CHD_DEC_NTSC_3D: not built
     assert e.value.name == "CHD_E_DECODER_UNKNOWN"
+    # CHD_DEC_PAL_2D, and CHD_DEC_AUTO (resolves to it for PAL), are built:
+    # here they fail only for want of a HIP device
+    for kind in (abi.CHD_DEC_PAL_2D, abi.CHD_DEC_AUTO):
+        with pytest.raises(abi.ChdError) as e:
+            abi.Decoder(v, kind).commit()
+        assert e.value.name == "CHD_E_UNSUPPORTED"
     none = abi.Decoder(v, 13).commit()  # CHD_DEC_NONE: geometry only
     oi = none.output_info()
     assert (oi.width, oi.height, oi.num_planes, oi.num_frames) == (922, 576, 3, 3)
@@ -229,7 +235,10 @@ def test_device_plan_without_gpu_fails_loudly():
     with pytest.raises(abi.ChdError) as e:
         abi.TpalPlan(abi.pal_tpal_params())
     assert e.value.name == "CHD_E_UNSUPPORTED"
-    bad = abi.pal_tpal_params(kind=6)
+    with pytest.raises(abi.ChdError) as e:
+        abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_PAL_2D))
+    assert e.value.name == "CHD_E_UNSUPPORTED"
+    bad = abi.pal_tpal_params(kind=4)   # CHD_DEC_NTSC_3D
     with pytest.raises(abi.ChdError) as e:
         abi.TpalPlan(bad)
     assert e.value.name == "CHD_E_DECODER_UNKNOWN"

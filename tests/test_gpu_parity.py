@@ -43,6 +43,21 @@ def test_transform3d_full_frames(orc, patch_capture, fmt):
 
 
 @pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
+@pytest.mark.parametrize("capture", ["bars_capture", "patch_capture"])
+def test_pal2d_full_frames(orc, request, capture, fmt):
+    """CHD_DEC_PAL_2D: the PALcolour 2D filter on the raw composite
+    (decodeLine<uint16_t, false>), through the same demodulator kernels."""
+    path, fields = request.getfixturevalue(capture)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_PAL_2D, output_format=fmt).commit()
+    for f in (0, 2):
+        _check(dec.decode(f), _oracle(orc, 6, fields, f, fmt), fmt)
+    # CHD_DEC_AUTO resolves to CHD_DEC_PAL_2D for PAL (registry.cpp:96-107)
+    auto = abi.Decoder(v, abi.CHD_DEC_AUTO, output_format=fmt).commit()
+    assert np.array_equal(auto.decode(1), dec.decode(1))
+
+
+@pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
 def test_transform2d_full_frames(orc, bars_capture, fmt):
     path, fields = bars_capture
     v = abi.Video(path)
@@ -51,7 +66,7 @@ def test_transform2d_full_frames(orc, bars_capture, fmt):
         _check(dec.decode(f), _oracle(orc, 7, fields, f, fmt), fmt)
 
 
-@pytest.mark.parametrize("kind", [7, 8])
+@pytest.mark.parametrize("kind", [6, 7, 8])
 @pytest.mark.parametrize("crop", [dict(first_active_frame_line=101, last_active_frame_line=170,
                                        first_active_sample=333, last_active_sample=470),
                                   dict(first_active_frame_line=0, last_active_frame_line=624,
@@ -140,6 +155,10 @@ def test_gain_phase_gray(orc, patch_capture):
     _check(g, _oracle(orc, 8, fields, 1, "yuv444ps", **geo)[:1], "grays")
     g16 = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_2D, output_format="gray16", **crop).commit().decode(1)
     _check(g16, _oracle(orc, 7, fields, 1, "yuv444p16", **geo)[:1], "gray16")
+    p2 = abi.Decoder(v, abi.CHD_DEC_PAL_2D, output_format="yuv444ps", chroma_gain=0.6, chroma_phase_deg=33.0,
+                     **crop).commit()
+    _check(p2.decode(1), _oracle(orc, 6, fields, 1, "yuv444ps", chroma_gain=0.6, chroma_phase_deg=33.0, **geo),
+           "yuv444ps")
 
 
 def test_out_of_range_and_reuse(patch_capture):
@@ -161,7 +180,7 @@ def test_out_of_range_and_reuse(patch_capture):
         dec.set_option("transform_threshold", 0.3)
 
 
-@pytest.mark.parametrize("kind", [7, 8])
+@pytest.mark.parametrize("kind", [6, 7, 8])
 def test_async_matches_sync(patch_capture, kind):
     path, _ = patch_capture
     v = abi.Video(path)
@@ -179,7 +198,7 @@ def test_async_matches_sync(patch_capture, kind):
     abi.lib().chd_cancel_free(c)
 
 
-@pytest.mark.parametrize("kind", [7, 8])
+@pytest.mark.parametrize("kind", [6, 7, 8])
 def test_device_api_matches_abi_bitwise(patch_capture, kind):
     """chd_tpal_decode shares each 8-field slab between frames f and f+2;
     the result must be bit-identical to per-frame chd_decode_frame."""

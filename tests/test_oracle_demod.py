@@ -23,12 +23,14 @@ def test_filter_size_and_quadrature_reference(orc):
     assert cf[7, 0] > 0 and np.all(cf >= 0)
 
 
-@pytest.mark.parametrize("kind", [7, 8])
+@pytest.mark.parametrize("kind", [6, 7, 8])
 @pytest.mark.parametrize("yuv", [(0.5, 0.2, -0.15), (0.3, -0.25, 0.3), (0.7, 0.0, 0.0)])
 def test_flat_colour_round_trip(orc, synth, kind, yuv):
     """Encode a flat Y'UV picture, decode: the centre region returns the
     encoded colour (pins burst phase, V-switch, demod signs, the 2x gain,
-    filter normalisation and the E' scaling)."""
+    filter normalisation and the E' scaling).  Kind 6 (CHD_DEC_PAL_2D)
+    filters the raw composite: its luma also pins the Y filter and the
+    chroma re-synthesis (palcolour.cpp:525-630)."""
     y, u, v = yuv
     pic = np.zeros((synth.PIC_HEIGHT, synth.PIC_WIDTH))
     fields = []
