@@ -128,8 +128,8 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true",
                     help="skip comparing the GPU frames with the oracle's in the CPU-baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the learned-mask and batched legs")
-    ap.add_argument("--extras", default="learned,batched",
-                    help="comma-separated legs to run beside the headline: learned, batched")
+    ap.add_argument("--extras", default="learned,batched,abi",
+                    help="comma-separated legs to run beside the headline: learned, batched, abi")
     ap.add_argument("--stress", type=int, default=0,
                     help="instead of the benchmark, run the dropout stress workload over this many frames "
                          "(libchromadec_amd/stress.py) and print its JSON line")
@@ -242,6 +242,8 @@ def main() -> int:
         res["learned"]["per_frame_time_vs_plain"] = round(fps / res["learned"]["value"], 2)
     if "batched" in extras:
         res["batched"] = batched_leg(args.steps)
+    if "abi" in extras:
+        res["abi_end_to_end"] = abi_leg(fields)
     if args.cpu_sample > 0:
         gpu_frames = None
         if not args.no_parity:
@@ -251,6 +253,37 @@ def main() -> int:
         res["cpu_baseline"], res["parity_vs_fp64_oracle"] = cpu_baseline(fields, args.cpu_sample, gpu_frames)
     print(json.dumps(res))
     return 0
+
+
+def abi_leg(fields, threads: int = 8):
+    """End to end through the public C ABI: chd_decode_frames_async over the
+    64 frames from a .tbc file, host planes out (PCIe and host copies
+    included), frames freed unread in the callback."""
+    import tempfile
+    from libchromadec_amd import abi, synth
+    d = tempfile.mkdtemp(prefix="chd_bench_")
+    path = os.path.join(d, "c2.tbc")
+    synth.write_tbc(path, fields)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps", transform_threshold=0.4,
+                      thread_count=threads).commit()
+    frames = list(range(N_FRAMES))
+    dec.decode_async(frames, keep=False)   # warm-up: worker plans, field caches
+    t0 = time.perf_counter()
+    res = dec.decode_async(frames, keep=False)
+    dt = time.perf_counter() - t0
+    ok = sum(1 for r in res.values() if r == "CHD_OK")
+    t1 = time.perf_counter()
+    for f in range(8):
+        dec.decode(f)
+    sync = 8 / (time.perf_counter() - t1)
+    dec.close()
+    v.close()
+    import shutil
+    shutil.rmtree(d, ignore_errors=True)
+    return {"async_frames_per_s": round(ok / dt, 1), "threads": threads, "sync_frames_per_s": round(sync, 1),
+            "note": "chd_decode_frames_async from a .tbc file, 6.4 MB of host planes per frame (PCIe-inclusive); "
+                    "the sync figure is the chd_decode_frame loop, which also copies the planes into numpy"}
 
 
 def learned_leg(plain_plan, packed, fields, truth, tu, tv, out, seqs, stream, steps):

@@ -30,7 +30,7 @@ extern "C" {
 typedef struct chd_tpal_plan chd_tpal_plan_t;
 
 typedef struct chd_tpal_params {
-    int32_t kind;                        /* CHD_DEC_TRANSFORM_2D (7) or _3D (8) */
+    int32_t kind;                        /* CHD_DEC_PAL_2D (6), CHD_DEC_TRANSFORM_2D (7) or _3D (8) */
     int32_t field_width, field_height;
     int32_t active_video_start, active_video_end;            /* samples, half-open */
     int32_t first_active_frame_line, last_active_frame_line; /* inclusive */
@@ -41,12 +41,14 @@ typedef struct chd_tpal_params {
     const double *thresholds;            /* 768 (3D) or 80 (2D) per-bin thresholds, or NULL */
     double  chroma_gain, chroma_phase_deg;
     double  ecb_from_u, ecr_from_v;      /* composite U/V -> E'Cb/E'Cr per unit signal range */
-    int32_t pixel_format;                /* CHD_PIXEL_YUV444PS, _YUV444P16, _GRAYS or _GRAY16 */
+    int32_t pixel_format;                /* CHD_PIXEL_YUV444PS, _YUV444P16, _GRAYS, _GRAY16, _RGB48 or _RGBS */
     int32_t clamp;                       /* chd_clamp_t */
     int32_t out_width, out_height;       /* output frame incl. padding */
     int32_t pad_left, pad_top;           /* position of the active picture */
     int32_t max_batch_frames;            /* frames per internal launch; 0 = 64 */
     int32_t device;                      /* HIP device ordinal */
+    double  kr, kb;                      /* luma coefficients of the colour-difference
+                                            precision (R'G'B' output); 0 = 0.299, 0.114 */
 } chd_tpal_params_t;
 
 /* Build a plan: constant tables, thresholds and a workspace sized for

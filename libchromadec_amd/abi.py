@@ -30,7 +30,10 @@ CHD_DEC_PAL_2D = 6
 CHD_DEC_TRANSFORM_2D = 7
 CHD_DEC_TRANSFORM_3D = 8
 CHD_PIXEL_YUV444P16, CHD_PIXEL_YUV444PS, CHD_PIXEL_GRAY16, CHD_PIXEL_GRAYS = 0, 1, 4, 5
+CHD_PIXEL_RGB48, CHD_PIXEL_RGBS = 2, 3
 CHD_PLANE_Y, CHD_PLANE_CB, CHD_PLANE_CR = 0, 1, 2
+CHD_PLANE_R, CHD_PLANE_G, CHD_PLANE_B = 3, 4, 5
+FLOAT_FORMATS = (CHD_PIXEL_YUV444PS, CHD_PIXEL_GRAYS, CHD_PIXEL_RGBS)
 
 
 class VideoInfo(C.Structure):
@@ -76,7 +79,8 @@ class TpalParams(C.Structure):
                 ("pixel_format", C.c_int32), ("clamp", C.c_int32),
                 ("out_width", C.c_int32), ("out_height", C.c_int32),
                 ("pad_left", C.c_int32), ("pad_top", C.c_int32),
-                ("max_batch_frames", C.c_int32), ("device", C.c_int32)]
+                ("max_batch_frames", C.c_int32), ("device", C.c_int32),
+                ("kr", C.c_double), ("kb", C.c_double)]
 
 
 class TpalSequence(C.Structure):
@@ -275,14 +279,23 @@ class Frame:
         return i
 
     def planes(self) -> np.ndarray:
+        """[planes, height, width]; RGB48 (interleaved) as [height, width, 3]."""
         inf = self.info()
-        is_float = inf.format in (CHD_PIXEL_YUV444PS, CHD_PIXEL_GRAYS)
+        is_float = inf.format in FLOAT_FORMATS
+        if inf.format == CHD_PIXEL_RGB48:
+            ptr, stride = C.c_void_p(), C.c_ssize_t()
+            check(lib().chd_frame_get_plane(self.h, CHD_PLANE_R, C.byref(ptr), C.byref(stride)),
+                  "chd_frame_get_plane")
+            buf = (C.c_char * (stride.value * inf.height)).from_address(ptr.value)
+            rows = np.frombuffer(buf, dtype=np.uint16).reshape(inf.height, stride.value // 2)
+            return rows[:, :3 * inf.width].reshape(inf.height, inf.width, 3).copy()
+        first = CHD_PLANE_R if inf.format == CHD_PIXEL_RGBS else CHD_PLANE_Y
         out = np.zeros((inf.num_planes, inf.height, inf.width), dtype=np.float32 if is_float else np.uint16)
         for p in range(inf.num_planes):
             ptr = C.c_void_p()
             stride = C.c_ssize_t()
             fn = lib().chd_frame_get_plane_float if is_float else lib().chd_frame_get_plane
-            check(fn(self.h, p, C.byref(ptr), C.byref(stride)), fn.__name__)
+            check(fn(self.h, first + p, C.byref(ptr), C.byref(stride)), fn.__name__)
             es = 4 if is_float else 2
             buf = (C.c_char * (stride.value * inf.height)).from_address(ptr.value)
             arr = np.frombuffer(buf, dtype=np.float32 if is_float else np.uint16)
@@ -349,12 +362,18 @@ class Decoder:
         finally:
             fr.free()
 
-    def decode_async(self, indices, cancel=None):
-        """Returns {index: planes or status name}."""
+    def decode_async(self, indices, cancel=None, keep=True):
+        """Returns {index: planes or status name}; with keep=False each frame
+        is freed in the callback unread and the value is "CHD_OK"."""
         res = {}
         mu = threading.Lock()
 
         def done(user, status, idx, frame):
+            if status == CHD_OK and not keep:
+                lib().chd_frame_free(C.c_void_p(frame))
+                with mu:
+                    res[idx] = "CHD_OK"
+                return
             with mu:
                 if status == CHD_OK:
                     fr = Frame(C.c_void_p(frame))
@@ -390,12 +409,12 @@ def color_scales(color_difference="modern", broadcast_scaling="scientific"):
 
 def pal_tpal_params(kind=CHD_DEC_TRANSFORM_3D, pixel_format=CHD_PIXEL_YUV444PS, threshold=0.4,
                     max_batch_frames=64, first_line=44, last_line=619, avs=185, ave=1107, device=0,
-                    black=0x4000, white=0xD300):
+                    black=0x4000, white=0xD300, clamp=0):
     """Device-plan parameters for the line-locked 4fSC PAL geometry."""
     ecb, ecr = color_scales()
     return TpalParams(kind, 1135, 313, avs, ave, first_line, last_line, 98, 138, black, white,
-                      4433618.75, 17734475.0, threshold, None, 1.0, 0.0, ecb, ecr, pixel_format, 0,
-                      ave - avs, last_line - first_line + 1, 0, 0, max_batch_frames, device)
+                      4433618.75, 17734475.0, threshold, None, 1.0, 0.0, ecb, ecr, pixel_format, clamp,
+                      ave - avs, last_line - first_line + 1, 0, 0, max_batch_frames, device, 0.299, 0.114)
 
 
 class TpalPlan:

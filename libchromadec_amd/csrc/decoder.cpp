@@ -481,7 +481,7 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
     frame->planes = d->planes;
     const size_t n = (size_t)d->planes * d->outW * d->outH;
     hipError_t e;
-    if (d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS) {
+    if (d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS || d->format == CHD_PIXEL_RGBS) {
         frame->f32.resize(n);
         e = hipMemcpyAsync(frame->f32.data(), w.d_out, n * sizeof(float), hipMemcpyDeviceToHost, w.stream);
     } else {
@@ -506,8 +506,8 @@ int parse_format(const std::string &n, chd_pixel_format_t &f)
     if (n == "yuv444ps" || n == "YUV444PS") { f = CHD_PIXEL_YUV444PS; return 1; }
     if (n == "gray16" || n == "GRAY16") { f = CHD_PIXEL_GRAY16; return 1; }
     if (n == "grays" || n == "GRAYS") { f = CHD_PIXEL_GRAYS; return 1; }
-    if (n == "rgb48" || n == "RGB48") { f = CHD_PIXEL_RGB48; return 0; }
-    if (n == "rgbs" || n == "RGBS") { f = CHD_PIXEL_RGBS; return 0; }
+    if (n == "rgb48" || n == "RGB48") { f = CHD_PIXEL_RGB48; return 1; }
+    if (n == "rgbs" || n == "RGBS") { f = CHD_PIXEL_RGBS; return 1; }
     if (n == "yuv440p16" || n == "YUV440P16") { f = CHD_PIXEL_YUV440P16; return 0; }
     if (n == "yuv440ps" || n == "YUV440PS") { f = CHD_PIXEL_YUV440PS; return 0; }
     return -1;
@@ -523,9 +523,9 @@ int parse_clamp(const std::string &n)
 }
 
 // color_conversion.cpp:16-67
-bool color_scales(const std::string &cdp, const std::string &bsp, double &ecb, double &ecr)
+bool color_scales(const std::string &cdp, const std::string &bsp, double &ecb, double &ecr, double &kr, double &kb)
 {
-    double kr, kb, ur, vr;
+    double ur, vr;
     if (cdp.empty() || cdp == "modern") { kr = 0.299; kb = 0.114; }
     else if (cdp == "classic") { kr = 0.30; kb = 0.11; }
     else return false;
@@ -691,7 +691,7 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
                                                 "CHD_DEC_PAL_2D and CHD_DEC_TRANSFORM_2D/3D)");
     if (fmt_ok == 0)
         return fail(CHD_E_UNSUPPORTED, fn + ": output_format not produced by this build "
-                                            "(yuv444p16, yuv444ps, gray16, grays are)");
+                                            "(yuv444p16, yuv444ps, gray16, grays, rgb48, rgbs are)");
     // PalDecoder::configure (pal_decoder.cpp:38-51)
     if (system != chd::System::PAL && system != chd::System::PAL_M)
         return fail(CHD_E_DECODER_INCOMPATIBLE, fn + ": This decoder is for PAL video sources only");
@@ -746,8 +746,8 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
             return fail(CHD_E_INVALID_ARG, fn + ": transform_thresholds_file holds " + std::to_string(thr.size()) +
                                                " values, the decoder needs " + std::to_string(want));
     }
-    double ecb = 0.0, ecr = 0.0;
-    color_scales(cdp, bsp, ecb, ecr);
+    double ecb = 0.0, ecr = 0.0, kr = 0.0, kb = 0.0;
+    color_scales(cdp, bsp, ecb, ecr, kr, kb);
 
     chd_tpal_params_t &p = d->params;
     std::memset(&p, 0, sizeof(p));
@@ -781,6 +781,8 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
     p.pad_top = padT;
     p.max_batch_frames = 1;
     p.device = 0;
+    p.kr = kr;
+    p.kb = kb;
 
     d->resolved = resolved;
     d->lookBehind = (resolved == CHD_DEC_TRANSFORM_3D) ? 2 : 0;  // (HALFZTILE + 1) / 2
@@ -884,6 +886,8 @@ chd_status_t chd_frame_get_plane_info(const chd_frame_t *f, chd_plane_t p, chd_p
         break;
     case CHD_PIXEL_GRAY16:
     case CHD_PIXEL_GRAYS: valid = p == CHD_PLANE_Y; break;
+    case CHD_PIXEL_RGB48:
+    case CHD_PIXEL_RGBS: valid = p == CHD_PLANE_R || p == CHD_PLANE_G || p == CHD_PLANE_B; break;
     default: break;
     }
     if (!valid) return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_info: plane not valid for the frame's pixel format");
@@ -912,8 +916,17 @@ chd_status_t chd_frame_get_plane(const chd_frame_t *f, chd_plane_t p, const void
         *out_data = f->u16.data();
         *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(uint16_t);
         return CHD_OK;
+    case CHD_PIXEL_RGB48:
+        // interleaved R'G'B': the three planes share the buffer at sample
+        // offsets 0/1/2 with the row pitch as stride (c_api_frame.cpp:161-179)
+        if (p != CHD_PLANE_R && p != CHD_PLANE_G && p != CHD_PLANE_B)
+            return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: plane not valid for RGB48");
+        *out_data = f->u16.data() + (size_t)(p - CHD_PLANE_R);
+        *out_stride = (ptrdiff_t)f->width * 3 * (ptrdiff_t)sizeof(uint16_t);
+        return CHD_OK;
     case CHD_PIXEL_YUV444PS:
     case CHD_PIXEL_GRAYS:
+    case CHD_PIXEL_RGBS:
         return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: use chd_frame_get_plane_float for float pixel formats");
     default: break;
     }
@@ -925,13 +938,21 @@ chd_status_t chd_frame_get_plane_float(const chd_frame_t *f, chd_plane_t p, cons
 {
     if (f == nullptr || out_data == nullptr || out_stride == nullptr)
         return fail(CHD_E_INVALID_ARG, "chromadec: chd_frame_get_plane_float: null argument");
-    if (f->format != CHD_PIXEL_YUV444PS && f->format != CHD_PIXEL_GRAYS)
+    if (f->format != CHD_PIXEL_YUV444PS && f->format != CHD_PIXEL_GRAYS && f->format != CHD_PIXEL_RGBS)
         return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: frame is not a float pixel format");
     if (f->format == CHD_PIXEL_GRAYS && p != CHD_PLANE_Y)
         return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: CHD_PIXEL_GRAYS only has CHD_PLANE_Y");
-    if (p != CHD_PLANE_Y && p != CHD_PLANE_CB && p != CHD_PLANE_CR)
-        return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: Y'CbCr float frame plane must be Y, Cb, or Cr");
-    *out_data = f->f32.data() + (size_t)f->width * f->height * (size_t)p;
+    size_t idx;
+    if (f->format == CHD_PIXEL_RGBS) {
+        if (p != CHD_PLANE_R && p != CHD_PLANE_G && p != CHD_PLANE_B)
+            return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: CHD_PIXEL_RGBS plane must be R, G, or B");
+        idx = (size_t)(p - CHD_PLANE_R);
+    } else {
+        if (p != CHD_PLANE_Y && p != CHD_PLANE_CB && p != CHD_PLANE_CR)
+            return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: Y'CbCr float frame plane must be Y, Cb, or Cr");
+        idx = (size_t)p;
+    }
+    *out_data = f->f32.data() + (size_t)f->width * f->height * idx;
     *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(float);
     return CHD_OK;
 }

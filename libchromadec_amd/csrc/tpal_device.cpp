@@ -98,7 +98,8 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
         return chd::fail(CHD_E_DECODER_UNKNOWN,
                          W + ": kind must be CHD_DEC_PAL_2D, CHD_DEC_TRANSFORM_2D or CHD_DEC_TRANSFORM_3D");
     if (p->pixel_format != CHD_PIXEL_YUV444PS && p->pixel_format != CHD_PIXEL_YUV444P16 &&
-        p->pixel_format != CHD_PIXEL_GRAYS && p->pixel_format != CHD_PIXEL_GRAY16)
+        p->pixel_format != CHD_PIXEL_GRAYS && p->pixel_format != CHD_PIXEL_GRAY16 &&
+        p->pixel_format != CHD_PIXEL_RGB48 && p->pixel_format != CHD_PIXEL_RGBS)
         return chd::fail(CHD_E_UNSUPPORTED, W + ": pixel format not supported by the GPU decoder");
     if (p->field_width <= 0 || p->field_height <= 0 || p->active_video_start < 0 ||
         p->active_video_end > p->field_width || p->active_video_end <= p->active_video_start ||
@@ -169,6 +170,44 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     const double theta = (p->chroma_phase_deg * M_PI) / 180.0;
     g.chroma_gain_cos = (float)(std::cos(theta) * p->chroma_gain);
     g.chroma_gain_sin = (float)(std::sin(theta) * p->chroma_gain);
+    {
+        // Y'UV -> R'G'B' (color_conversion.cpp:33-67): the composite U/V
+        // reductions follow from ecbFromU = 1 / (2 (1 - kb) uReduction)
+        const double kr = p->kr > 0.0 ? p->kr : 0.299, kb = p->kb > 0.0 ? p->kb : 0.114;
+        const double kg = 1.0 - kr - kb;
+        const double ured = 1.0 / (2.0 * (1.0 - kb) * p->ecb_from_u);
+        const double vred = 1.0 / (2.0 * (1.0 - kr) * p->ecr_from_v);
+        g.rgb[0] = (float)(1.0 / vred);
+        g.rgb[1] = (float)(-(kb / kg) / ured);
+        g.rgb[2] = (float)(-(kr / kg) / vred);
+        g.rgb[3] = (float)(1.0 / ured);
+        // floatRGBBounds (output_writer.cpp:149-190)
+        const float inf = INFINITY;
+        for (int c = 0; c < 3; ++c) {
+            g.rgb_lo[c] = -inf;
+            g.rgb_hi[c] = inf;
+        }
+        if (p->clamp == CHD_CLAMP_LEGAL_RGB_SDR) {
+            for (int c = 0; c < 3; ++c) {
+                g.rgb_lo[c] = 0.0f;
+                g.rgb_hi[c] = 1.0f;
+            }
+        } else if (p->clamp == CHD_CLAMP_LEGAL_RGB_HDR) {
+            for (int c = 0; c < 3; ++c) g.rgb_lo[c] = 0.0f;
+        } else if (p->clamp == CHD_CLAMP_LEGAL_YCBCR_BT601) {
+            // the BT.601 legal Y'CbCr box carried through the inverse matrix
+            const float eyLo = -15.0f / 219.0f, eyHi = 238.75f / 219.0f;
+            const float ecLo = -127.0f / 224.0f, ecHi = 126.75f / 224.0f;
+            const double rds = 2.0 * (1.0 - kr), bds = 2.0 * (1.0 - kb);
+            const double gc = (kb * bds) / kg + (kr * rds) / kg;
+            g.rgb_lo[0] = (float)(eyLo + rds * ecLo);
+            g.rgb_hi[0] = (float)(eyHi + rds * ecHi);
+            g.rgb_lo[1] = (float)(eyLo - gc * ecHi);
+            g.rgb_hi[1] = (float)(eyHi - gc * ecLo);
+            g.rgb_lo[2] = (float)(eyLo + bds * ecLo);
+            g.rgb_hi[2] = (float)(eyHi + bds * ecHi);
+        }
+    }
 
     // PalColour::buildLookUpTables (palcolour.cpp:150-277)
     const double bw = 1100000.0 / 0.93;
@@ -358,7 +397,8 @@ static int64_t frame_bytes(const chd_tpal_plan *plan)
 {
     const Geom &g = plan->g;
     const int planes = (g.pixel_format == CHD_PIXEL_GRAY16 || g.pixel_format == CHD_PIXEL_GRAYS) ? 1 : 3;
-    const int es = (g.pixel_format == CHD_PIXEL_YUV444PS || g.pixel_format == CHD_PIXEL_GRAYS) ? 4 : 2;
+    const int es = (g.pixel_format == CHD_PIXEL_YUV444PS || g.pixel_format == CHD_PIXEL_GRAYS ||
+                    g.pixel_format == CHD_PIXEL_RGBS) ? 4 : 2;
     return (int64_t)planes * g.out_w * g.out_h * es;
 }
 

@@ -58,6 +58,11 @@ struct Geom {
     int32_t clamp_mode;
     float ey_scale, ecb_scale, ecr_scale;
     float chroma_gain_cos, chroma_gain_sin;  // gain * cos/sin(phase)
+    // R'G'B' output: E'R = E'Y + rgb[0] V, E'G = E'Y + rgb[1] U + rgb[2] V,
+    // E'B = E'Y + rgb[3] U (U, V per unit signal range), and the RGBS clamp
+    // box of the clamp mode
+    float rgb[4];
+    float rgb_lo[3], rgb_hi[3];
 };
 
 struct SlabJob {

@@ -921,6 +921,73 @@ struct LdsD {
 
 __device__ __forceinline__ float h273_round(float x) { return copysignf(floorf(fabsf(x) + 0.5f), x); }
 
+// OutputWriter (output_writer.cpp:46-76, 124-190, 402-486, 670-777): the
+// pixel format and the clamp box of the clamp mode, per launch.
+struct OutSpec {
+    int fmt;                              // chd_pixel_format_t
+    int64_t plane;                        // samples per plane
+    float eyLo, eyHi, ecLo, ecHi;         // float Y'CbCr
+    float yLo, yHi, cLo, cHi;             // u16 Y'CbCr (H.273 narrow range)
+};
+
+__device__ __forceinline__ OutSpec out_spec(const Geom &g)
+{
+    OutSpec o;
+    o.fmt = g.pixel_format;
+    o.plane = (int64_t)g.out_w * g.out_h;
+    o.eyLo = -INFINITY; o.eyHi = INFINITY; o.ecLo = -INFINITY; o.ecHi = INFINITY;
+    o.yLo = 0.0f; o.yHi = 65535.0f; o.cLo = 0.0f; o.cHi = 65535.0f;
+    if (g.clamp_mode == 1) {
+        o.eyLo = 0.0f; o.eyHi = 1.0f; o.ecLo = -0.5f; o.ecHi = 0.5f;
+        o.yLo = 16.0f * 256.0f; o.yHi = 235.0f * 256.0f; o.cLo = 16.0f * 256.0f; o.cHi = 240.0f * 256.0f;
+    } else if (g.clamp_mode == 3) {
+        o.eyLo = -15.0f / 219.0f; o.eyHi = 238.75f / 219.0f; o.ecLo = -127.0f / 224.0f; o.ecHi = 126.75f / 224.0f;
+        o.yLo = 256.0f; o.yHi = 254.75f * 256.0f; o.cLo = 256.0f; o.cHi = 254.75f * 256.0f;
+    }
+    return o;
+}
+
+// One output pixel o (row-major index within a plane) from the
+// composite-domain Y/U/V of the demodulator.
+__device__ __forceinline__ void store_pixel(const Geom &g, const OutSpec &os, void *outp, int64_t o, float Y, float U,
+                                            float V)
+{
+    const float ey = (Y - g.black) * g.ey_scale;
+    if (os.fmt == 1 /*YUV444PS*/ || os.fmt == 5 /*GRAYS*/) {
+        float *out = reinterpret_cast<float *>(outp);
+        out[o] = fminf(fmaxf(ey, os.eyLo), os.eyHi);
+        if (os.fmt == 1) {
+            out[os.plane + o] = fminf(fmaxf(U * g.ecb_scale, os.ecLo), os.ecHi);
+            out[2 * os.plane + o] = fminf(fmaxf(V * g.ecr_scale, os.ecLo), os.ecHi);
+        }
+    } else if (os.fmt == 0 /*YUV444P16*/ || os.fmt == 4 /*GRAY16*/) {
+        uint16_t *out = reinterpret_cast<uint16_t *>(outp);
+        out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), os.yLo), os.yHi);
+        if (os.fmt == 0) {
+            const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
+            out[os.plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecb + 128.0f * 256.0f), os.cLo), os.cHi);
+            out[2 * os.plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecr + 128.0f * 256.0f), os.cLo), os.cHi);
+        }
+    } else if (os.fmt == 3 /*RGBS*/) {
+        // E'R/E'G/E'B planes (convertToFloatRGB)
+        const float u = U * g.ey_scale, v = V * g.ey_scale;
+        float *out = reinterpret_cast<float *>(outp);
+        out[o] = fminf(fmaxf(ey + g.rgb[0] * v, g.rgb_lo[0]), g.rgb_hi[0]);
+        out[os.plane + o] = fminf(fmaxf(ey + g.rgb[1] * u + g.rgb[2] * v, g.rgb_lo[1]), g.rgb_hi[1]);
+        out[2 * os.plane + o] = fminf(fmaxf(ey + g.rgb[3] * u, g.rgb_lo[2]), g.rgb_hi[2]);
+    } else {
+        // RGB48: interleaved full-range u16, luma clamped first, every
+        // component clamped to [0, 65535] and truncated (convertLine)
+        const float s = 65535.0f * g.ey_scale;
+        const float ry = fminf(fmaxf((Y - g.black) * s, 0.0f), 65535.0f);
+        const float ru = U * s, rv = V * s;
+        uint16_t *out = reinterpret_cast<uint16_t *>(outp) + 3 * o;
+        out[0] = (uint16_t)fminf(fmaxf(ry + g.rgb[0] * rv, 0.0f), 65535.0f);
+        out[1] = (uint16_t)fminf(fmaxf(ry + g.rgb[1] * ru + g.rgb[2] * rv, 0.0f), 65535.0f);
+        out[2] = (uint16_t)fminf(fmaxf(ry + g.rgb[3] * ru, 0.0f), 65535.0f);
+    }
+}
+
 // Grid: jobs x 2 fields x line blocks x column segments.
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
@@ -1018,18 +1085,7 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
     }
     __syncthreads();
 
-    const bool is_float = (g.pixel_format == 1 /*YUV444PS*/ || g.pixel_format == 5 /*GRAYS*/);
-    const bool luma_only = (g.pixel_format == 4 /*GRAY16*/ || g.pixel_format == 5 /*GRAYS*/);
-    const int64_t plane = (int64_t)g.out_w * g.out_h;
-    float eyLo = -INFINITY, eyHi = INFINITY, ecLo = -INFINITY, ecHi = INFINITY;
-    float yLo = 0.0f, yHi = 65535.0f, cLo = 0.0f, cHi = 65535.0f;
-    if (g.clamp_mode == 1) {
-        eyLo = 0.0f; eyHi = 1.0f; ecLo = -0.5f; ecHi = 0.5f;
-        yLo = 16.0f * 256.0f; yHi = 235.0f * 256.0f; cLo = 16.0f * 256.0f; cHi = 240.0f * 256.0f;
-    } else if (g.clamp_mode == 3) {
-        eyLo = -15.0f / 219.0f; eyHi = 238.75f / 219.0f; ecLo = -127.0f / 224.0f; ecHi = 126.75f / 224.0f;
-        yLo = 256.0f; yHi = 254.75f * 256.0f; cLo = 256.0f; cHi = 254.75f * 256.0f;
-    }
+    const OutSpec os = out_spec(g);
 
     const int i = i0 + tid;              // this thread's output column
     const bool col_live = i < endPos;
@@ -1089,23 +1145,7 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
             const int frame_line = number * 2 + off;
             const int orow = frame_line - g.first + g.pad_top;
             const int64_t o = (int64_t)orow * g.out_w + g.pad_left + (i - g.avs);
-            const float ey = (Y - g.black) * g.ey_scale;
-            const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
-            if (is_float) {
-                float *out = reinterpret_cast<float *>(job.out);
-                out[o] = fminf(fmaxf(ey, eyLo), eyHi);
-                if (!luma_only) {
-                    out[plane + o] = fminf(fmaxf(ecb, ecLo), ecHi);
-                    out[2 * plane + o] = fminf(fmaxf(ecr, ecLo), ecHi);
-                }
-            } else {
-                uint16_t *out = reinterpret_cast<uint16_t *>(job.out);
-                out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), yLo), yHi);
-                if (!luma_only) {
-                    out[plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecb + 128.0f * 256.0f), cLo), cHi);
-                    out[2 * plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecr + 128.0f * 256.0f), cLo), cHi);
-                }
-            }
+            store_pixel(g, os, job.out, o, Y, U, V);
         }
         __syncthreads();
     }
@@ -1214,18 +1254,7 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
     }
     __syncthreads();
 
-    const bool is_float = (g.pixel_format == 1 /*YUV444PS*/ || g.pixel_format == 5 /*GRAYS*/);
-    const bool luma_only = (g.pixel_format == 4 /*GRAY16*/ || g.pixel_format == 5 /*GRAYS*/);
-    const int64_t plane = (int64_t)g.out_w * g.out_h;
-    float eyLo = -INFINITY, eyHi = INFINITY, ecLo = -INFINITY, ecHi = INFINITY;
-    float yLo = 0.0f, yHi = 65535.0f, cLo = 0.0f, cHi = 65535.0f;
-    if (g.clamp_mode == 1) {
-        eyLo = 0.0f; eyHi = 1.0f; ecLo = -0.5f; ecHi = 0.5f;
-        yLo = 16.0f * 256.0f; yHi = 235.0f * 256.0f; cLo = 16.0f * 256.0f; cHi = 240.0f * 256.0f;
-    } else if (g.clamp_mode == 3) {
-        eyLo = -15.0f / 219.0f; eyHi = 238.75f / 219.0f; ecLo = -127.0f / 224.0f; ecHi = 126.75f / 224.0f;
-        yLo = 256.0f; yHi = 254.75f * 256.0f; cLo = 256.0f; cHi = 254.75f * 256.0f;
-    }
+    const OutSpec os = out_spec(g);
     float cf[kQHalo + 1][4];
 #pragma unroll
     for (int b = 0; b <= kQHalo; ++b)
@@ -1276,23 +1305,7 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
             const int frame_line = number * 2 + off;
             const int orow = frame_line - g.first + g.pad_top;
             const int64_t o = (int64_t)orow * g.out_w + g.pad_left + (i - g.avs);
-            const float ey = (Y - g.black) * g.ey_scale;
-            const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
-            if (is_float) {
-                float *out = reinterpret_cast<float *>(job.out);
-                out[o] = fminf(fmaxf(ey, eyLo), eyHi);
-                if (!luma_only) {
-                    out[plane + o] = fminf(fmaxf(ecb, ecLo), ecHi);
-                    out[2 * plane + o] = fminf(fmaxf(ecr, ecLo), ecHi);
-                }
-            } else {
-                uint16_t *out = reinterpret_cast<uint16_t *>(job.out);
-                out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), yLo), yHi);
-                if (!luma_only) {
-                    out[plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecb + 128.0f * 256.0f), cLo), cHi);
-                    out[2 * plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecr + 128.0f * 256.0f), cLo), cHi);
-                }
-            }
+            store_pixel(g, os, job.out, o, Y, U, V);
         }
         __syncthreads();
     }
@@ -1443,18 +1456,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     // columns c0 + e + b of the sine parity: ((e + b) & 1) == sel for every thread
     const int sel = g.spar ^ ((g.avs + c0) & 1);
 
-    const bool is_float = (g.pixel_format == 1 /*YUV444PS*/ || g.pixel_format == 5 /*GRAYS*/);
-    const bool luma_only = (g.pixel_format == 4 /*GRAY16*/ || g.pixel_format == 5 /*GRAYS*/);
-    const int64_t plane = (int64_t)g.out_w * g.out_h;
-    float eyLo = -INFINITY, eyHi = INFINITY, ecLo = -INFINITY, ecHi = INFINITY;
-    float yLo = 0.0f, yHi = 65535.0f, cLo = 0.0f, cHi = 65535.0f;
-    if (g.clamp_mode == 1) {
-        eyLo = 0.0f; eyHi = 1.0f; ecLo = -0.5f; ecHi = 0.5f;
-        yLo = 16.0f * 256.0f; yHi = 235.0f * 256.0f; cLo = 16.0f * 256.0f; cHi = 240.0f * 256.0f;
-    } else if (g.clamp_mode == 3) {
-        eyLo = -15.0f / 219.0f; eyHi = 238.75f / 219.0f; ecLo = -127.0f / 224.0f; ecHi = 126.75f / 224.0f;
-        yLo = 256.0f; yHi = 254.75f * 256.0f; cLo = 256.0f; cHi = 254.75f * 256.0f;
-    }
+    const OutSpec os = out_spec(g);
 
     // rows n0-3 .. n0+3 of this thread's columns
     float4 r[7];
@@ -1548,26 +1550,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 else Y = compv[e] - f4get(r[3], e);
                 const float U = -(PU * bp + QU * bq) * 2.0f;
                 const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
-                const float ey = (Y - g.black) * g.ey_scale;
-                const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
-                if (c0 + e < g.aw) {
-                    const int64_t o = obase + e;
-                    if (is_float) {
-                        float *out = reinterpret_cast<float *>(job.out);
-                        out[o] = fminf(fmaxf(ey, eyLo), eyHi);
-                        if (!luma_only) {
-                            out[plane + o] = fminf(fmaxf(ecb, ecLo), ecHi);
-                            out[2 * plane + o] = fminf(fmaxf(ecr, ecLo), ecHi);
-                        }
-                    } else {
-                        uint16_t *out = reinterpret_cast<uint16_t *>(job.out);
-                        out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), yLo), yHi);
-                        if (!luma_only) {
-                            out[plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecb + 128.0f * 256.0f), cLo), cHi);
-                            out[2 * plane + o] = (uint16_t)fminf(fmaxf(h273_round(224.0f * 256.0f * ecr + 128.0f * 256.0f), cLo), cHi);
-                        }
-                    }
-                }
+                if (c0 + e < g.aw) store_pixel(g, os, job.out, obase + e, Y, U, V);
             }
         }
 #pragma unroll
@@ -1616,7 +1599,9 @@ __global__ void chroma_to_planes(const Geom g, const float *__restrict__ src, fl
 // padding adds a border around the active picture).
 __global__ void fill_border(const Geom g, void *out, int64_t n_frames)
 {
-    const bool is_float = (g.pixel_format == 1 || g.pixel_format == 5);
+    // black: 0.0 for float formats, 0 for RGB48, H.273 digital zero for Y'CbCr
+    const bool is_float = (g.pixel_format == 1 || g.pixel_format == 3 || g.pixel_format == 5);
+    const bool rgb48 = g.pixel_format == 2;
     const int planes = (g.pixel_format == 4 || g.pixel_format == 5) ? 1 : 3;
     const int64_t plane = (int64_t)g.out_w * g.out_h;
     const int64_t total = n_frames * planes * plane;
@@ -1624,6 +1609,8 @@ __global__ void fill_border(const Geom g, void *out, int64_t n_frames)
         const int p = (int)((i / plane) % planes);
         if (is_float) {
             reinterpret_cast<float *>(out)[i] = 0.0f;
+        } else if (rgb48) {
+            reinterpret_cast<uint16_t *>(out)[i] = 0;
         } else {
             reinterpret_cast<uint16_t *>(out)[i] = (p == 0) ? (uint16_t)(16 * 256) : (uint16_t)(128 * 256);
         }

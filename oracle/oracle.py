@@ -73,6 +73,8 @@ def lib():
                                                     P(d), P(d), P(d), P(d), P(d)]
         _lib.orc_output_float.argtypes = [P(OrcParams), P(d), P(d), P(d), P(C.c_float)]
         _lib.orc_output_u16.argtypes = [P(OrcParams), P(d), P(d), P(d), P(C.c_uint16)]
+        _lib.orc_output_rgb48.argtypes = [P(OrcParams), P(d), P(d), P(d), P(C.c_uint16)]
+        _lib.orc_output_rgbs.argtypes = [P(OrcParams), P(d), P(d), P(d), P(C.c_float)]
         _lib.orc_decode_window.argtypes = [P(OrcParams), i32, P(d), P(P(C.c_uint16)), P(i32), i32, i32, i32,
                                            i32, C.c_void_p, P(d), P(d)]
         _lib.orc_decode_window.restype = C.c_int
@@ -310,3 +312,19 @@ def detect_burst(p: OrcParams, field: np.ndarray, line: int):
     bp, bq, v = C.c_double(), C.c_double(), C.c_double()
     lib().orc_detect_burst(C.byref(p), line, _u16p(f), C.byref(bp), C.byref(bq), C.byref(v))
     return bp.value, bq.value, v.value
+
+
+def output_stage(p: OrcParams, yuv: np.ndarray, out_fmt: int) -> np.ndarray:
+    """OutputWriter of a component frame yuv [3, 2*field_height-1,
+    field_width] (composite-domain Y/U/V) in any of the OUT_* formats."""
+    L = lib()
+    yuv = np.ascontiguousarray(yuv, dtype=np.float64)
+    ca = yuv.shape[1] * yuv.shape[2]
+    flat = yuv.reshape(-1)
+    out = _out_array(p, out_fmt)
+    args = (C.byref(p), _dp(flat), _dp(flat[ca:]), _dp(flat[2 * ca:]))
+    fn = {OUT_PS: L.orc_output_float, OUT_P16: L.orc_output_u16, OUT_RGB48: L.orc_output_rgb48,
+          OUT_RGBS: L.orc_output_rgbs}[out_fmt]
+    ptype = C.c_float if out.dtype == np.float32 else C.c_uint16
+    fn(*args, out.ctypes.data_as(C.POINTER(ptype)))
+    return out

@@ -101,6 +101,47 @@ def test_crop_padding_clamp(orc, patch_capture, kind, crop):
     _check(got, want, "yuv444ps")
 
 
+CLAMPS = {"none": 0, "legal_rgb_sdr": 1, "legal_rgb_hdr": 2, "legal_ycbcr_bt601": 3}
+
+
+@pytest.mark.parametrize("kind", [6, 8])
+@pytest.mark.parametrize("fmt", ["rgbs", "rgb48"])
+@pytest.mark.parametrize("clamp", ["none", "legal_rgb_sdr", "legal_rgb_hdr", "legal_ycbcr_bt601"])
+def test_rgb_outputs(orc, patch_capture, kind, fmt, clamp):
+    """R'G'B' output formats (OutputWriter RGB48 / convertToFloatRGB): RGBS
+    within 1e-4, RGB48 (truncated) within 1 LSB, black border, and the
+    interleaved RGB48 planes borrowed at sample offsets 0/1/2."""
+    import ctypes as C
+    path, fields = patch_capture
+    crop = dict(first_active_frame_line=101, last_active_frame_line=170, first_active_sample=333,
+                last_active_sample=470)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format=fmt, output_clamp=clamp, padding_multiple=16, **crop).commit()
+    oi = dec.output_info()
+    got = dec.decode(3)
+    # active 138 x 70 padded to 144 x 80: odd remainders go left/top first
+    # (OutputWriter::updateConfiguration, output_writer.cpp:214-238)
+    assert (oi.width, oi.height) == (144, 80)
+    p = orc.pal_params(first_line=101, last_line=170, avs=333, ave=471, clamp_mode=CLAMPS[clamp], pad_left=3,
+                       pad_top=5, out_width=144, out_height=80)
+    want = orc.decode_frame(p, kind, fields, 3, orc.uniform_thresholds(kind), orc.OUT_RGBS if fmt == "rgbs"
+                            else orc.OUT_RGB48)
+    assert got.shape == want.shape
+    d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+    assert d.max() <= (F32_TOL if fmt == "rgbs" else 1), d.max()
+    if fmt == "rgb48":
+        f = C.c_void_p()
+        abi.check(abi.lib().chd_decode_frame(dec.h, 3, C.byref(f)), "chd_decode_frame")
+        ptrs = []
+        for plane in (abi.CHD_PLANE_R, abi.CHD_PLANE_G, abi.CHD_PLANE_B):
+            ptr, stride = C.c_void_p(), C.c_ssize_t()
+            abi.check(abi.lib().chd_frame_get_plane(f, plane, C.byref(ptr), C.byref(stride)), "chd_frame_get_plane")
+            assert stride.value == 144 * 6
+            ptrs.append(ptr.value)
+        assert ptrs[1] - ptrs[0] == 2 and ptrs[2] - ptrs[0] == 4
+        abi.lib().chd_frame_free(f)
+
+
 @pytest.mark.parametrize("kind", [7, 8])
 @pytest.mark.parametrize("t", [0.0, 0.25, 1.5])
 def test_threshold_values(orc, patch_capture, kind, t):
