@@ -60,10 +60,10 @@ void launch_arrange(const float *theta, const Frags &f, hipStream_t s);
 void launch_features(const float2 *spec, int64_t n_tiles, uint16_t *feat, hipStream_t s);
 void launch_targets(const float2 *spec, const float2 *truth, int64_t n_tiles, float *w, float *num, hipStream_t s);
 void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n_tiles, float *mask, hipStream_t s);
-// Fused, persistent inference (features through sigmoid) for decoding:
-// n_groups workgroups (one per CU fits: 144 KB of LDS each) loop over tiles.
+// Fused, persistent inference (features through sigmoid) for decoding: two
+// workgroups per CU (68 KB of LDS each) loop over tiles.
 void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
-                  int n_groups, hipStream_t s);
+                  int cus, hipStream_t s);
 void launch_backward(const float *theta, const Frags &f, const Acts &a, const float *mask, int64_t n_tiles,
                      float scale, float *partial, float *grad, hipStream_t s);
 void launch_adam(float *theta, const float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,

@@ -320,168 +320,204 @@ __global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ act2,
 
 // ---------------------------------------------------------------------------
 // Fused inference: features, the three layers and the sigmoid of a tile in
-// one workgroup, activations in LDS.  Persistent: each workgroup keeps the
-// layer-2 weight fragments in LDS and loops over tiles.  Same arithmetic as
-// k_features + k_conv<kFwd1/kFwd2> + k_head (BF16 operands, FP32
-// accumulation, BF16 activations), so masks agree with the layered forward
-// up to FP32 summation order.
+// one workgroup, activations in LDS.  Persistent, two 4-wave workgroups per
+// CU on different tiles, so that one's feature, epilogue and head phases
+// overlap the other's matrix-core phases.  Layer-2 weight fragments stream
+// from the L1/L2 caches (55 KB, read by every workgroup), loaded three
+// half-taps ahead.  Same arithmetic as k_features + k_conv<kFwd1/kFwd2> +
+// k_head (BF16 operands, FP32 accumulation, BF16 activations), so masks
+// agree with the layered forward up to FP32 summation order.
 // ---------------------------------------------------------------------------
 
 constexpr int kActStride = 40;  // bf16 per activation row in LDS (32 + 8: spreads gathered rows over banks)
 constexpr int kActRowB = 2 * kActStride;
 constexpr int kZeroRow = kP;    // an all-zero row: the neighbour of a visit across the x edge
+constexpr int kHeadTaps = 9;    // taps per head round (one dz plane)
 
 struct LdsInfer {
-    uint4 w2[kK2 * 64];                        // layer-2 B fragments
     union {
         uint16_t act[(kP + 1) * kActStride];   // act1, then act2 (+ zero row)
-        float g[(kP + 1) * 27];                // head partial sums G[q][t] (+ zero row)
+        float g[(kP + 1) * kHeadTaps];         // head partial sums G[q][t] of one round (+ zero row)
     } u;
     uint2 feat[kP + 1];                        // 4 bf16 features per visit (+ zero row)
 };
 
-// Tile-invariant neighbourhood of one visit: the rows of (z+dz, y+dy, x) for
-// the 9 (dz, dy) pairs, and whether x sits on either edge.  Tap t's row is
-// then a compile-time pick plus at most one add and one select.
-struct Nbhd {
-    int zy[9];
-    bool lo, hi;
-};
+// Row blocks: wave w owns the 6 blocks (z, x) with z in {2w, 2w+1} and x in
+// {0, 1, 2}; an MFMA row is y = 0..31, so visit p = (z*32 + y)*3 + x.  A
+// neighbour at tap (dz, dy, dx) is then row ((z+dz)&7)*96 + ((y+dy)&31)*3 +
+// x+dx: a wave-uniform z term, a per-lane y term and a compile-time x, and
+// x+dx outside 0..2 is the zero row, known at compile time.
+constexpr int kInferWaves = 4;                   // 256 threads; two workgroups per CU
+constexpr int kInferBlocks = 24 / kInferWaves;   // (z, x) row blocks per wave
+constexpr int kInferVisits = 3;                  // feature/head visits per thread: x = 0..2
+constexpr int kInferAhead = 3;                   // layer-2 weight fragments in flight
+constexpr int kGRowB = kHeadTaps * 4;            // bytes per G row
 
-__device__ __forceinline__ Nbhd make_nbhd(int p)
+// Byte offset of act row (zt, lane y term, x) for tap T of block B (compile
+// time); zt[k] = ((z0 + k - 1) & 7) * 96 * rowbytes for k = 0..3 covers
+// z0-1 .. z0+2 (z0 = 2w, blocks z0 and z0+1).
+template <int B, int T, int ROWB>
+__device__ __forceinline__ int tap_addr(const int (&zt)[4], const int (&yt)[3], int zero)
 {
-    Nbhd n;
-    const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
-#pragma unroll
-    for (int j = 0; j < 9; ++j) n.zy[j] = ((((z + j / 3 - 1) & 7) << 5) | ((y + j % 3 - 1) & 31)) * 3 + x;
-    n.lo = x == 0;
-    n.hi = x == 2;
-    return n;
+    constexpr int dz = T / 9 - 1, dy = (T / 3) % 3 - 1, dx = T % 3 - 1;
+    constexpr int x = B % 3 + dx, zi = B / 3 + dz + 1;
+    if constexpr (x < 0 || x > 2) return zero;
+    else return zt[zi] + yt[dy + 1] + x * ROWB;
 }
-
-template <int T>
-__device__ __forceinline__ int tap_row(const Nbhd &n)
-{
-    constexpr int dx = T % 3 - 1;
-    const int r = n.zy[T / 3];
-    if constexpr (dx < 0) return n.lo ? kZeroRow : r - 1;
-    else if constexpr (dx > 0) return n.hi ? kZeroRow : r + 1;
-    else return r;
-}
-
-template <int T>
-__device__ __forceinline__ int tap_row_or_zero(const Nbhd &n)
-{
-    if constexpr (T >= 27) return kZeroRow;
-    else return tap_row<T>(n);
-}
-
-constexpr int kInferWaves = 8;           // 512 threads: two waves per SIMD share the tile
-constexpr int kInferBlocks = 24 / kInferWaves;   // row blocks of 32 visits per wave
-constexpr int kInferVisits = (kP + 64 * kInferWaves - 1) / (64 * kInferWaves);   // feature/head visits per thread
 
 // Layer-1 k-step J: this lane's A operand, the features of its two taps
 // 4J + 2hi and 4J + 2hi + 1 (taps >= 27 are K padding: the zero row).
-template <int J>
-__device__ __forceinline__ uint4 l1_operand(const uint2 *feat, const Nbhd &n, int hi)
+template <int B, int T>
+__device__ __forceinline__ int l1_addr(const int (&zt)[4], const int (&yt)[3], int zero)
 {
-    const int qa = hi ? tap_row_or_zero<4 * J + 2>(n) : tap_row_or_zero<4 * J>(n);
-    const int qb = hi ? tap_row_or_zero<4 * J + 3>(n) : tap_row_or_zero<4 * J + 1>(n);
-    const uint2 lo = feat[qa], up = feat[qb];
+    if constexpr (T >= 27) return zero;
+    else return tap_addr<B, T, 8>(zt, yt, zero);
+}
+
+template <int J, int B>
+__device__ __forceinline__ uint4 l1_operand(const char *fb, const int (&zt)[4], const int (&yt)[3], int zero, int hi)
+{
+    const int qa = hi ? l1_addr<B, 4 * J + 2>(zt, yt, zero) : l1_addr<B, 4 * J>(zt, yt, zero);
+    const int qb = hi ? l1_addr<B, 4 * J + 3>(zt, yt, zero) : l1_addr<B, 4 * J + 1>(zt, yt, zero);
+    const uint2 lo = *reinterpret_cast<const uint2 *>(fb + qa), up = *reinterpret_cast<const uint2 *>(fb + qb);
     return make_uint4(lo.x, lo.y, up.x, up.y);
 }
 
-template <int J, int NB>
-__device__ __forceinline__ void layer1_steps(const uint2 *feat, const Nbhd (&nb)[NB], int hi, const uint4 *w1,
-                                             int lane, f32x16 (&acc)[NB])
+template <int J>
+__device__ __forceinline__ void layer1_steps(const char *fb, const int (&zt)[4], const int (&yt)[3], int zero, int hi,
+                                             const uint4 *w1, int lane, f32x16 (&acc)[kInferBlocks])
 {
     if constexpr (J < kK1) {
         const uint4 bf = w1[J * 64 + lane];
-        uint4 af[NB];
+        uint4 af[kInferBlocks];
+        af[0] = l1_operand<J, 0>(fb, zt, yt, zero, hi);
+        af[1] = l1_operand<J, 1>(fb, zt, yt, zero, hi);
+        af[2] = l1_operand<J, 2>(fb, zt, yt, zero, hi);
+        af[3] = l1_operand<J, 3>(fb, zt, yt, zero, hi);
+        af[4] = l1_operand<J, 4>(fb, zt, yt, zero, hi);
+        af[5] = l1_operand<J, 5>(fb, zt, yt, zero, hi);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) af[b] = l1_operand<J>(feat, nb[b], hi);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc[b] = mfma(af[b], bf, acc[b]);
-        layer1_steps<J + 1>(feat, nb, hi, w1, lane, acc);
+        for (int b = 0; b < kInferBlocks; ++b) acc[b] = mfma(af[b], bf, acc[b]);
+        // keep each step's loads next to its matrix ops: hoisting later
+        // steps' operands would exceed the register budget
+        __builtin_amdgcn_sched_barrier(0);
+        layer1_steps<J + 1>(fb, zt, yt, zero, hi, w1, lane, acc);
     }
 }
 
-// Layer-2 half-tap K (tap K/2, input channels 16 (K%2) .. +15): A operands of
-// every block, gathered from act1 (byte base `ap` includes this lane's
-// 8-channel half).
-template <int K, int NB>
-__device__ __forceinline__ void l2_operands(const char *ap, const Nbhd (&nb)[NB], uint4 (&a)[NB])
+// Layer-2 half-tap K (tap K/2, input channels 16 (K%2) .. +15): A operand of
+// block B, gathered from act1 (byte addresses include this lane's 8-channel
+// half).
+template <int K, int B>
+__device__ __forceinline__ uint4 l2_operand(const char *ab, const int (&zt)[4], const int (&yt)[3], int zero)
 {
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-        a[b] = *reinterpret_cast<const uint4 *>(ap + tap_row<K / 2>(nb[b]) * kActRowB + 32 * (K % 2));
+    return *reinterpret_cast<const uint4 *>(ab + tap_addr<B, K / 2, kActRowB>(zt, yt, zero) + 32 * (K % 2));
 }
 
-template <int K, int NB>
-__device__ __forceinline__ void layer2_steps(const char *ap, const Nbhd (&nb)[NB], const uint4 *w2, int lane,
-                                             uint4 (&cur)[NB], uint4 &bcur, f32x16 (&acc)[NB])
+// Half-tap K: the matrix ops of blocks 0-2 and 3-5, each group followed by
+// the gathers of its operands for half-tap K+1 into the same registers
+// (one buffer; the other group's ops cover the load latency).  Weight
+// fragments of half-taps K .. K + kInferAhead - 1 sit in bq[K % A].
+template <int K>
+__device__ __forceinline__ void layer2_steps(const char *ab, const int (&zt)[4], const int (&yt)[3], int zero,
+                                             const char *w2, int w2off, uint4 (&a)[kInferBlocks],
+                                             uint4 (&bq)[kInferAhead], f32x16 (&acc)[kInferBlocks])
 {
+    // w2 + w2off: this lane's fragment word of half-tap 0 (1 KB per half-tap)
     if constexpr (K < kK2) {
-        uint4 nxt[NB], bnxt;
+        const uint4 bcur = bq[K % kInferAhead];
+        if constexpr (K + kInferAhead < kK2)
+            bq[K % kInferAhead] = *reinterpret_cast<const uint4 *>(w2 + w2off + (K + kInferAhead) * 1024);
+        acc[0] = mfma(a[0], bcur, acc[0]);
+        acc[1] = mfma(a[1], bcur, acc[1]);
+        acc[2] = mfma(a[2], bcur, acc[2]);
         if constexpr (K + 1 < kK2) {
-            l2_operands<K + 1>(ap, nb, nxt);
-            bnxt = w2[(K + 1) * 64 + lane];
+            a[0] = l2_operand<K + 1, 0>(ab, zt, yt, zero);
+            a[1] = l2_operand<K + 1, 1>(ab, zt, yt, zero);
+            a[2] = l2_operand<K + 1, 2>(ab, zt, yt, zero);
         }
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc[b] = mfma(cur[b], bcur, acc[b]);
+        acc[3] = mfma(a[3], bcur, acc[3]);
+        acc[4] = mfma(a[4], bcur, acc[4]);
+        acc[5] = mfma(a[5], bcur, acc[5]);
         if constexpr (K + 1 < kK2) {
-#pragma unroll
-            for (int b = 0; b < NB; ++b) cur[b] = nxt[b];
-            bcur = bnxt;
-            layer2_steps<K + 1>(ap, nb, w2, lane, cur, bcur, acc);
+            a[3] = l2_operand<K + 1, 3>(ab, zt, yt, zero);
+            a[4] = l2_operand<K + 1, 4>(ab, zt, yt, zero);
+            a[5] = l2_operand<K + 1, 5>(ab, zt, yt, zero);
         }
+        // keep each half-tap's loads next to its matrix ops: hoisting later
+        // operands would exceed the register budget
+        __builtin_amdgcn_sched_barrier(0);
+        layer2_steps<K + 1>(ab, zt, yt, zero, w2, w2off, a, bq, acc);
     }
 }
 
-// logit of one visit: b3 + sum over taps of G[neighbour][tap], taps in order.
-template <int T>
-__device__ __forceinline__ float head_sum(const float *g, const Nbhd &n, float s)
+// Partial logit of visit (z, y, X) over the taps of head round R (taps
+// 9R .. 9R+8 in order, dz = R - 1): G rows hold this round's 9 columns.
+// gz = ((z + dz) & 7) * 96 * kGRowB, gy[k] = ((y + k - 1) & 31) * 3 * kGRowB.
+template <int X, int T>
+__device__ __forceinline__ float head_sum(const char *g, int gz, const int (&gy)[3], int zero, float s)
 {
-    if constexpr (T < 27) return head_sum<T + 1>(g, n, s + g[tap_row<T>(n) * 27 + T]);
-    else return s;
+    if constexpr (T < kHeadTaps) {
+        constexpr int dy = T / 3 - 1, dx = T % 3 - 1, x = X + dx;
+        int a;
+        if constexpr (x < 0 || x > 2) a = zero;
+        else a = gz + gy[dy + 1] + x * kGRowB;
+        return head_sum<X, T + 1>(g, gz, gy, zero, s + *reinterpret_cast<const float *>(g + a + 4 * T));
+    } else {
+        return s;
+    }
 }
 
-__global__ void __launch_bounds__(64 * kInferWaves, 1)
+__global__ void __launch_bounds__(64 * kInferWaves, 2)
 k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restrict__ theta, Frags f,
         float *__restrict__ mask)
 {
     constexpr int NT = 64 * kInferWaves, NB = kInferBlocks, NV = kInferVisits;
     __shared__ LdsInfer L;
     __shared__ float red[kInferWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int i = lane & 31, hi = lane >> 5;
-    for (int k = tid; k < kK2 * 64; k += NT) L.w2[k] = f.w2[k];
-    if (tid < 27) L.u.g[kZeroRow * 27 + tid] = 0.0f;   // beyond the act region: set once
+    const int z0 = 2 * wave;
+    if (tid < kActStride / 2) reinterpret_cast<uint32_t *>(&L.u.act[kZeroRow * kActStride])[tid] = 0u;  // beyond G
     if (tid == 0) L.feat[kZeroRow] = make_uint2(0u, 0u);
     const float b1 = theta[kOffB1 + i], b2 = theta[kOffB2 + i], b3 = theta[kOffB3];
-    const uint4 w3a = f.w3t[lane], w3b = f.w3t[64 + lane];
-    // neighbourhoods of this lane's A rows (one per block) and of the visits
-    // whose features and logits this thread computes
-    Nbhd nb[NB];
+    // address terms (bytes): act rows for layer 2, feature rows for layer 1
+    int zt_act[4], zt_feat[4], yt_act[3], yt_feat[3];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) nb[b] = make_nbhd((wave + kInferWaves * b) * 32 + i);
-    int ia[NV], ib[NV];
+    for (int k = 0; k < 4; ++k) {
+        const int zr = ((z0 + k - 1) & 7) * 96;
+        zt_act[k] = zr * kActRowB;
+        zt_feat[k] = zr * 8;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int yr = ((i + k - 1) & 31) * 3;
+        yt_act[k] = yr * kActRowB + 16 * hi;
+        yt_feat[k] = yr * 8;
+    }
+    const int zero_act = kZeroRow * kActRowB + 16 * hi, zero_feat = kZeroRow * 8;
+    // this thread's feature/head visits: z = z0 + hi, y = i, x = 0..2
+    const int zv = z0 + hi;
+    const int pv = (zv * 32 + i) * 3;
+    int ia[NV], ib[NV];   // byte offsets of the visit's bin and its mirror in a tile spectrum
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-        const int p = tid + NT * j;
-        ia[j] = ib[j] = 0;
-        if (p < kP) visit_bins(p, ia[j], ib[j]);
+        visit_bins(pv + j, ia[j], ib[j]);
+        ia[j] *= (int)sizeof(float2);
+        ib[j] *= (int)sizeof(float2);
     }
-    const char *ap = reinterpret_cast<const char *>(L.u.act) + 16 * hi;
+    const char *ab = reinterpret_cast<const char *>(L.u.act);
+    const char *fb = reinterpret_cast<const char *>(L.feat);
     // spectrum bins of this thread's visits, loaded one tile ahead
     float2 sa[NV], sb[NV];
     auto fetch = [&](int64_t t) {
+        const char *st = reinterpret_cast<const char *>(spec + t * kSpecC);   // wave-uniform
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             sa[j] = sb[j] = make_float2(0.0f, 0.0f);
-            if (t < n_tiles && tid + NT * j < kP) {
-                sa[j] = spec[t * kSpecC + ia[j]];
-                sb[j] = spec[t * kSpecC + ib[j]];
+            if (t < n_tiles) {
+                sa[j] = *reinterpret_cast<const float2 *>(st + ia[j]);
+                sb[j] = *reinterpret_cast<const float2 *>(st + ib[j]);
             }
         }
     };
@@ -489,22 +525,28 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     __syncthreads();
 
     for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        // Keep the neighbourhood rows opaque per tile: otherwise every tap's
-        // row (81 per lane) is hoisted out of the tile loop and spills.
+        // Keep the address terms opaque per tile: otherwise every tap's
+        // address (hundreds per lane) is hoisted out of the tile loop.
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+        for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(yt_act[k]), "+v"(yt_feat[k]));
 #pragma unroll
-            for (int j = 0; j < 9; ++j) asm volatile("" : "+v"(nb[b].zy[j]));
+        for (int k = 0; k < 4; ++k) asm volatile("" : "+s"(zt_act[k]), "+s"(zt_feat[k]));
+#pragma unroll
+        for (int j = 0; j < NV; ++j) asm volatile("" : "+v"(ia[j]), "+v"(ib[j]));
+        // per-lane byte bases of the epilogue stores and head loads; each
+        // store adds a compile-time offset (block, value) to them
+        int ebase = ((z0 * 32 + 4 * hi) * 3) * kActRowB + 2 * i;       // act row of acc[0][0]
+        int gbase = ((z0 * 32 + 4 * hi) * 3) * kGRowB;                  // G row of acc[0][0]
+        int hbase = ((z0 * 32 + i) * 3) * kActRowB + 16 * hi;            // head A row of block 0
+        int w2off = lane * 16;                                           // weight fragment word
+        asm volatile("" : "+v"(ebase), "+v"(gbase), "+v"(hbase), "+v"(w2off));
         // ---- features (as k_features) ----------------------------------
         float f0[NV], f1[NV], s = 0.0f;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            f0[j] = f1[j] = 0.0f;
-            if (tid + NT * j < kP) {
-                f0[j] = mag_feature(sa[j]);
-                f1[j] = mag_feature(sb[j]);
-                s += f0[j];
-            }
+            f0[j] = mag_feature(sa[j]);
+            f1[j] = mag_feature(sb[j]);
+            s += f0[j];
         }
         s = wave_sum(s);
         if (lane == 0) red[wave] = s;
@@ -515,17 +557,11 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         const float mean = tot * (1.0f / kP);
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            const int p = tid + NT * j;
-            if (p < kP) {
-                const int zy = p / 3, x = p - 3 * zy, y = zy & 31, z = zy >> 5;
-                const float sm = (x == 2 && (y == 4 || y == 20) && (z == 1 || z == 5)) ? 1.0f : 0.0f;
-                L.feat[p] = make_uint2(pack2(f0[j], f1[j]), pack2(mean, sm));
-            }
+            // self-mirror bins: x = 2 (j), y in {4, 20}, z in {1, 5}
+            const float sm = (j == 2 && (i == 4 || i == 20) && (zv == 1 || zv == 5)) ? 1.0f : 0.0f;
+            L.feat[pv + j] = make_uint2(pack2(f0[j], f1[j]), pack2(mean, sm));
         }
-        // the act zero row lies inside G's extent: clear it again every tile
-        if (tid < kActStride / 2) reinterpret_cast<uint32_t *>(&L.u.act[kZeroRow * kActStride])[tid] = 0u;
         __syncthreads();
-        fetch(t + gridDim.x);   // next tile's bins, in flight through both layers
 
         f32x16 acc[NB];
         // ---- layer 1: K = t*4 + ci (112) --------------------------------
@@ -533,14 +569,21 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
-        layer1_steps<0>(L.feat, nb, hi, f.w1, lane, acc);
+        layer1_steps<0>(fb, zt_feat, yt_feat, zero_feat, hi, f.w1, lane, acc);
+        // act row of acc[b][v]: ((z0 + b/3) * 32 + y) * 3 + b%3, y = 8(v/4) + 4hi + v%4
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int p = (wave + kInferWaves * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-                L.u.act[p * kActStride + i] = (uint16_t)f2bf(fmaxf(acc[b][v] + b1, 0.0f));
+                const int off = (((b / 3) * 32 + 8 * (v >> 2) + (v & 3)) * 3 + b % 3) * kActRowB;
+                *reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(L.u.act) + ebase + off) =
+                    (uint16_t)f2bf(fmaxf(acc[b][v] + b1, 0.0f));
             }
+        // the first layer-2 weight fragments, in flight across the barrier
+        uint4 bq[kInferAhead];
+#pragma unroll
+        for (int k = 0; k < kInferAhead; ++k)
+            bq[k] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(f.w2) + w2off + k * 1024);
         __syncthreads();
 
         // ---- layer 2: K = t*32 + ci (864), operands one half-tap ahead ----
@@ -549,29 +592,37 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
         {
-            uint4 cur[NB], bcur;
-            l2_operands<0>(ap, nb, cur);
-            bcur = L.w2[lane];
-            layer2_steps<0>(ap, nb, L.w2, lane, cur, bcur, acc);
+            uint4 a[NB];
+            a[0] = l2_operand<0, 0>(ab, zt_act, yt_act, zero_act);
+            a[1] = l2_operand<0, 1>(ab, zt_act, yt_act, zero_act);
+            a[2] = l2_operand<0, 2>(ab, zt_act, yt_act, zero_act);
+            a[3] = l2_operand<0, 3>(ab, zt_act, yt_act, zero_act);
+            a[4] = l2_operand<0, 4>(ab, zt_act, yt_act, zero_act);
+            a[5] = l2_operand<0, 5>(ab, zt_act, yt_act, zero_act);
+            layer2_steps<0>(ab, zt_act, yt_act, zero_act, reinterpret_cast<const char *>(f.w2), w2off, a, bq,
+                            acc);
         }
+        fetch(t + gridDim.x);   // next tile's bins, in flight through the head
         __syncthreads();  // every wave has read act1
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int p = (wave + kInferWaves * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-                L.u.act[p * kActStride + i] = (uint16_t)f2bf(fmaxf(acc[b][v] + b2, 0.0f));
+                const int off = (((b / 3) * 32 + 8 * (v >> 2) + (v & 3)) * 3 + b % 3) * kActRowB;
+                *reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(L.u.act) + ebase + off) =
+                    (uint16_t)f2bf(fmaxf(acc[b][v] + b2, 0.0f));
             }
         __syncthreads();
 
         // ---- head: G[q][t] = sum_c act2[q][c] W3[t][c] ------------------
         {
+            const uint4 w3a = f.w3t[lane], w3b = f.w3t[64 + lane];
             uint4 a0[NB], a1[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                const int p = (wave + kInferWaves * b) * 32 + i;
-                a0[b] = *reinterpret_cast<const uint4 *>(ap + p * kActRowB);
-                a1[b] = *reinterpret_cast<const uint4 *>(ap + p * kActRowB + 32);
+                const int off = ((b / 3) * 32 * 3 + b % 3) * kActRowB;
+                a0[b] = *reinterpret_cast<const uint4 *>(ab + hbase + off);
+                a1[b] = *reinterpret_cast<const uint4 *>(ab + hbase + off + 32);
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -581,25 +632,38 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
                 acc[b] = mfma(a1[b], w3b, acc[b]);
             }
         }
-        __syncthreads();  // act2 is read; G overwrites it
-        if (i < 27) {
+        float logit[NV];
 #pragma unroll
-            for (int b = 0; b < NB; ++b)
+        for (int j = 0; j < NV; ++j) logit[j] = b3;
+        const char *gb = reinterpret_cast<const char *>(L.u.g);
+        int gy[3];
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int q = (wave + kInferWaves * b) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-                    L.u.g[q * 27 + i] = acc[b][v];
-                }
-        }
-        __syncthreads();
+        for (int k = 0; k < 3; ++k) gy[k] = ((i + k - 1) & 31) * 3 * kGRowB;
+        const int zero_g = kZeroRow * kGRowB;
+        // three rounds of 9 taps (one dz plane each) through LDS
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            const int p = tid + NT * j;
-            if (p < kP) {
-                const float sum = head_sum<0>(L.u.g, make_nbhd(p), b3);
-                mask[t * kP + p] = 1.0f / (1.0f + expf(-sum));
+        for (int r = 0; r < 3; ++r) {
+            __syncthreads();  // act2 (round 0) or the previous round's G is read
+            const int c = i - kHeadTaps * r;
+            if (c >= 0 && c < kHeadTaps) {
+                char *gw = reinterpret_cast<char *>(L.u.g) + gbase + 4 * c;
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int off = (((b / 3) * 32 + 8 * (v >> 2) + (v & 3)) * 3 + b % 3) * kGRowB;
+                        *reinterpret_cast<float *>(gw + off) = acc[b][v];
+                    }
             }
+            if (tid < kHeadTaps) L.u.g[kZeroRow * kHeadTaps + tid] = 0.0f;
+            __syncthreads();
+            const int gz = ((zv + r - 1) & 7) * 96 * kGRowB;
+            logit[0] = head_sum<0, 0>(gb, gz, gy, zero_g, logit[0]);
+            logit[1] = head_sum<1, 0>(gb, gz, gy, zero_g, logit[1]);
+            logit[2] = head_sum<2, 0>(gb, gz, gy, zero_g, logit[2]);
         }
+#pragma unroll
+        for (int j = 0; j < NV; ++j) mask[t * kP + pv + j] = 1.0f / (1.0f + expf(-logit[j]));
         __syncthreads();  // G is read before the next tile's features and act1 overwrite it
     }
 }
@@ -839,10 +903,10 @@ void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n
 }
 
 void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
-                  int n_groups, hipStream_t s)
+                  int cus, hipStream_t s)
 {
     if (n_tiles <= 0) return;
-    const int64_t g = std::min<int64_t>(n_tiles, std::max(1, n_groups));
+    const int64_t g = std::min<int64_t>(n_tiles, 2 * std::max(1, cus));
     hipLaunchKernelGGL(k_infer, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, spec, n_tiles, theta, f, mask);
 }
 
