@@ -393,6 +393,23 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
 #pragma unroll
     for (int i = 0; i < 8; ++i) carry[i] = 0.0f;
 
+    // apply mode: the next tile's spectrum and mask are loaded into registers
+    // while the current tile is inverted (its loads would otherwise wait on
+    // HBM once per tile)
+    constexpr int kSpecPerLane = (MODE == kTileApply) ? 1280 / 64 : 1;
+    constexpr int kMaskPerLane = (MODE == kTileApply) ? 768 / 64 : 1;
+    float2 pre_s[kSpecPerLane];
+    float pre_m[kMaskPerLane];
+    auto prefetch = [&](int64_t t) {
+        if constexpr (MODE == kTileApply) {
+#pragma unroll
+            for (int j = 0; j < kSpecPerLane; ++j) pre_s[j] = spec[t * 1280 + lane + 64 * j];
+#pragma unroll
+            for (int j = 0; j < kMaskPerLane; ++j) pre_m[j] = mask[t * 768 + lane + 64 * j];
+        }
+    };
+    prefetch(((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx_begin);
+
     for (int tx = tx_begin; tx < tx_end; ++tx) {
         const int tileX = g.avs - kXT / 2 + (kXT / 2) * tx;
 
@@ -578,12 +595,15 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
             }
 
         } else {
-            for (int i = lane; i < 1280; i += 64) {
-                const float2 v = spec[tile * 1280 + i];
-                Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{v.x, v.y};
+#pragma unroll
+            for (int j = 0; j < kSpecPerLane; ++j) {
+                const int i = lane + 64 * j;
+                Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{pre_s[j].x, pre_s[j].y};
             }
-            for (int i = lane; i < 768; i += 64) L.m[i] = mask[tile * 768 + i];
+#pragma unroll
+            for (int j = 0; j < kMaskPerLane; ++j) L.m[lane + 64 * j] = pre_m[j];
             __syncthreads();
+            if (tx + 1 < tx_end) prefetch(tile + 1);
         }
 
         // ---- masked inverse z: outputs z = 0,1,4,5 ------------------------
