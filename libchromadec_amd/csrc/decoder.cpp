@@ -142,8 +142,10 @@ struct Worker {
     std::vector<int64_t> slot_key;      // field number * 2 + shifted, -1 = empty
     std::vector<uint64_t> slot_tick;
     uint64_t tick = 0;
-    uint16_t *h_stage = nullptr;        // pinned: 14 fields
+    uint16_t *h_stage = nullptr;        // pinned: 14 fields as read from the file
+    uint16_t *d_stage = nullptr;        // their device copies, before packing into slots
     void *d_out = nullptr;
+    void *h_out = nullptr;              // pinned: one decoded frame on its way to the host
     int device = 0;
     uint64_t seen_revision = 0;         // dropout options the cached fields were corrected with
 
@@ -157,7 +159,9 @@ struct Worker {
         chd_mask_net_free(net);
         if (d_slots) (void)hipFree(d_slots);
         if (d_out) (void)hipFree(d_out);
+        if (d_stage) (void)hipFree(d_stage);
         if (h_stage) (void)hipHostFree(h_stage);
+        if (h_out) (void)hipHostFree(h_out);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -233,8 +237,10 @@ chd_status_t worker_init(chd_decoder_t *d, Worker &w, const char *who)
     hipError_t e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&w.d_slots, (size_t)kCacheSlots * g.field_elems * sizeof(uint16_t));
     if (e == hipSuccess) e = hipMalloc(&w.d_out, chd_tpal_frame_bytes(w.plan));
-    if (e == hipSuccess)
-        e = hipHostMalloc((void **)&w.h_stage, (size_t)14 * g.fw * g.fh * sizeof(uint16_t), 0);
+    const size_t stage_bytes = (size_t)14 * g.fw * g.fh * sizeof(uint16_t);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&w.h_stage, stage_bytes, 0);
+    if (e == hipSuccess) e = hipMalloc((void **)&w.d_stage, stage_bytes);
+    if (e == hipSuccess) e = hipHostMalloc(&w.h_out, chd_tpal_frame_bytes(w.plan), 0);
     if (e == hipSuccess)
         e = hipMemsetD16Async((hipDeviceptr_t)w.d_slots, (unsigned short)d->vp.black16bIre,
                               (size_t)kCacheSlots * g.field_elems, w.stream);
@@ -288,21 +294,17 @@ const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool sh
         st = fail(CHD_E_IO, "chd_decode_frame: failed to read field " + std::to_string(number));
         return nullptr;
     }
-    uint16_t *dst = w.d_slots + (size_t)victim * g.field_elems + g.lmargin;
-    hipError_t e;
-    if (!shifted || corrected != nullptr) {
-        e = hipMemcpy2DAsync(dst, g.pitch * 2, stage, g.fw * 2, g.fw * 2, g.fh, hipMemcpyHostToDevice, w.stream);
-    } else {
-        // SourceField::shiftSecondField (source_field.cpp:51-66): drop the
-        // first two samples of the field, append two black samples.
-        e = hipMemcpy2DAsync(dst, g.pitch * 2, stage + 2, g.fw * 2, g.fw * 2, g.fh - 1, hipMemcpyHostToDevice,
-                             w.stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(dst + (size_t)(g.fh - 1) * g.pitch, stage + (size_t)(g.fh - 1) * g.fw + 2,
-                               (g.fw - 2) * 2, hipMemcpyHostToDevice, w.stream);
-        if (e == hipSuccess)
-            e = hipMemsetD16Async((hipDeviceptr_t)(dst + (size_t)(g.fh - 1) * g.pitch + g.fw - 2),
-                                  (unsigned short)d->vp.black16bIre, 2, w.stream);
+    // One contiguous copy, then the pack kernel lays the field out with its
+    // margins (row-by-row pitched copies cost a DMA command per line).
+    // SourceField::shiftSecondField (source_field.cpp:51-66) drops the first
+    // two samples of the field and appends two black ones: shift 2.
+    uint16_t *d_src = w.d_stage + (size_t)stage_idx * g.fw * g.fh;
+    hipError_t e = hipMemcpyAsync(d_src, stage, (size_t)g.fw * g.fh * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                  w.stream);
+    if (e == hipSuccess) {
+        tpal::launch_pack(g, d_src, (int64_t)g.fw * g.fh, w.d_slots + (size_t)victim * g.field_elems, 1, w.stream,
+                          shifted && corrected == nullptr ? 2 : 0);
+        e = hipGetLastError();
     }
     if (e != hipSuccess) {
         st = fail(CHD_E_INTERNAL, "chd_decode_frame: field upload failed: " + hip_err(e));
@@ -480,16 +482,20 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
     frame->height = d->outH;
     frame->planes = d->planes;
     const size_t n = (size_t)d->planes * d->outW * d->outH;
-    hipError_t e;
-    if (d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS || d->format == CHD_PIXEL_RGBS) {
-        frame->f32.resize(n);
-        e = hipMemcpyAsync(frame->f32.data(), w.d_out, n * sizeof(float), hipMemcpyDeviceToHost, w.stream);
-    } else {
-        frame->u16.resize(n);
-        e = hipMemcpyAsync(frame->u16.data(), w.d_out, n * sizeof(uint16_t), hipMemcpyDeviceToHost, w.stream);
-    }
+    const bool is_float = d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS ||
+                          d->format == CHD_PIXEL_RGBS;
+    const size_t bytes = n * (is_float ? sizeof(float) : sizeof(uint16_t));
+    // through the worker's pinned buffer: a single DMA at link rate
+    hipError_t e = hipMemcpyAsync(w.h_out, w.d_out, bytes, hipMemcpyDeviceToHost, w.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
     if (e != hipSuccess) return fail(CHD_E_INTERNAL, "chd_decode_frame: GPU decode failed: " + hip_err(e));
+    if (is_float) {
+        frame->f32.resize(n);
+        std::memcpy(frame->f32.data(), w.h_out, bytes);
+    } else {
+        frame->u16.resize(n);
+        std::memcpy(frame->u16.data(), w.h_out, bytes);
+    }
     frame->info.format = d->format;
     frame->info.width = d->outW;
     frame->info.height = d->outH;
@@ -838,27 +844,33 @@ chd_status_t chd_decode_frames_async(chd_decoder_t *d, const int64_t *indices, s
                     "chd_decode_frames_async: decoder kind is CHD_DEC_NONE (geometry/dropout only)");
     if (n == 0) return CHD_OK;
     const size_t W = std::min<size_t>({n, (size_t)std::max(1, d->threadCount), (size_t)kMaxGpuWorkers});
+    // Workers take runs of consecutive list positions: neighbouring frames of
+    // a run share most of their window fields in that worker's field cache
+    // (a 3D window spans 7 frames), where single positions would reload it.
+    const size_t run = std::max<size_t>(1, std::min<size_t>(64, (n + W - 1) / W));
     std::atomic<size_t> next{0};
     std::vector<std::thread> threads;
     threads.reserve(W);
     for (size_t wi = 0; wi < W; ++wi) {
         Worker *w = get_worker(d, wi);
-        threads.emplace_back([d, w, indices, n, cb, user, cancel, &next]() {
+        threads.emplace_back([d, w, indices, n, run, cb, user, cancel, &next]() {
             for (;;) {
-                const size_t i = next.fetch_add(1, std::memory_order_relaxed);
-                if (i >= n) return;
-                const int64_t idx = indices[i];
-                if (cancel != nullptr && cancel->requested.load(std::memory_order_acquire)) {
-                    cb(user, CHD_E_CANCELLED, idx, nullptr);
-                    continue;
+                const size_t i0 = next.fetch_add(run, std::memory_order_relaxed);
+                if (i0 >= n) return;
+                for (size_t i = i0; i < std::min(n, i0 + run); ++i) {
+                    const int64_t idx = indices[i];
+                    if (cancel != nullptr && cancel->requested.load(std::memory_order_acquire)) {
+                        cb(user, CHD_E_CANCELLED, idx, nullptr);
+                        continue;
+                    }
+                    chd_frame_t *frame = nullptr;
+                    chd_status_t rc;
+                    {
+                        std::lock_guard<std::mutex> lk(w->mu);
+                        rc = decode_locked(d, *w, idx, &frame);
+                    }
+                    cb(user, rc, idx, frame);
                 }
-                chd_frame_t *frame = nullptr;
-                chd_status_t rc;
-                {
-                    std::lock_guard<std::mutex> lk(w->mu);
-                    rc = decode_locked(d, *w, idx, &frame);
-                }
-                cb(user, rc, idx, frame);
             }
         });
     }

@@ -14,6 +14,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace chd {
@@ -79,10 +80,27 @@ struct chd_cancel {
     std::atomic<bool> requested{false};
 };
 
+// Allocator whose resize() leaves new elements uninitialised: frame storage is
+// written in full by the download right after it is sized.
+template <typename T>
+struct UninitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <typename U>
+    UninitAlloc(const UninitAlloc<U> &) noexcept {}
+    template <typename U>
+    void construct(U *p) noexcept { ::new (static_cast<void *>(p)) U; }
+    template <typename U, typename... A>
+    void construct(U *p, A &&...a) { ::new (static_cast<void *>(p)) U(std::forward<A>(a)...); }
+};
+
 struct chd_frame {
     chd_frame_info_t info{};
     chd_pixel_format_t format = CHD_PIXEL_YUV444P16;
     int32_t width = 0, height = 0, planes = 0;
-    std::vector<float> f32;      // float formats: planes x height x width
-    std::vector<uint16_t> u16;   // integer formats
+    std::vector<float, UninitAlloc<float>> f32;        // float formats: planes x height x width
+    std::vector<uint16_t, UninitAlloc<uint16_t>> u16;  // integer formats
 };

@@ -336,7 +336,7 @@ constexpr int kHeadTaps = 9;    // taps per head round (one dz plane)
 
 struct LdsInfer {
     union {
-        uint16_t act[(kP + 1) * kActStride];   // act1, then act2 (+ zero row)
+        uint16_t act[kP * kActStride];         // act1, then act2
         float g[(kP + 1) * kHeadTaps];         // head partial sums G[q][t] of one round (+ zero row)
     } u;
     uint2 feat[kP + 1];                        // 4 bf16 features per visit (+ zero row)
@@ -350,7 +350,6 @@ struct LdsInfer {
 constexpr int kInferWaves = 4;                   // 256 threads; two workgroups per CU
 constexpr int kInferBlocks = 24 / kInferWaves;   // (z, x) row blocks per wave
 constexpr int kInferVisits = 3;                  // feature/head visits per thread: x = 0..2
-constexpr int kInferAhead = 3;                   // layer-2 weight fragments in flight
 constexpr int kGRowB = kHeadTaps * 4;            // bytes per G row
 
 // Byte offset of act row (zt, lane y term, x) for tap T of block B (compile
@@ -405,49 +404,83 @@ __device__ __forceinline__ void layer1_steps(const char *fb, const int (&zt)[4],
     }
 }
 
-// Layer-2 half-tap K (tap K/2, input channels 16 (K%2) .. +15): A operand of
-// block B, gathered from act1 (byte addresses include this lane's 8-channel
-// half).
-template <int K, int B>
-__device__ __forceinline__ uint4 l2_operand(const char *ab, const int (&zt)[4], const int (&yt)[3], int zero)
+// Layer 2 runs in 18 groups G = ((dz+1)*3 + dy+1)*2 + h.  A group gathers
+// the six act rows (z0 + zb + dz, y + dy, x') for zb = 0, 1 and x' = 0..2,
+// channels 16h .. 16h+15 (byte addresses include this lane's 8-channel half),
+// and serves the three taps dx = -1, 0, 1 of both z blocks: block (zb, x)
+// takes row x' = x + dx.  Pairs with x + dx outside 0..2 would read the zero
+// row and are skipped, so a group is 6 gathers and 14 matrix ops.
+constexpr int kL2Groups = 18;
+constexpr int kL2Ring = 9;   // weight fragments in flight: three groups
+
+template <int G, int ZB, int XP>
+__device__ __forceinline__ uint4 l2_operand(const char *ab, const int (&zt)[4], const int (&yt)[3])
 {
-    return *reinterpret_cast<const uint4 *>(ab + tap_addr<B, K / 2, kActRowB>(zt, yt, zero) + 32 * (K % 2));
+    constexpr int dz = G / 6 - 1, dy = (G / 2) % 3 - 1, h = G % 2;
+    return *reinterpret_cast<const uint4 *>(ab + zt[ZB + dz + 1] + yt[dy + 1] + XP * kActRowB + 32 * h);
 }
 
-// Half-tap K: the matrix ops of blocks 0-2 and 3-5, each group followed by
-// the gathers of its operands for half-tap K+1 into the same registers
-// (one buffer; the other group's ops cover the load latency).  Weight
-// fragments of half-taps K .. K + kInferAhead - 1 sit in bq[K % A].
-template <int K>
-__device__ __forceinline__ void layer2_steps(const char *ab, const int (&zt)[4], const int (&yt)[3], int zero,
-                                             const char *w2, int w2off, uint4 (&a)[kInferBlocks],
-                                             uint4 (&bq)[kInferAhead], f32x16 (&acc)[kInferBlocks])
+// Weight half-tap of group G and dx = D - 1: tap ((dz+1)*3 + dy+1)*3 + D,
+// half G % 2 (1 KB per half-tap, tap-major).
+template <int G, int D>
+constexpr int l2_half_tap()
 {
-    // w2 + w2off: this lane's fragment word of half-tap 0 (1 KB per half-tap)
-    if constexpr (K < kK2) {
-        const uint4 bcur = bq[K % kInferAhead];
-        if constexpr (K + kInferAhead < kK2)
-            bq[K % kInferAhead] = *reinterpret_cast<const uint4 *>(w2 + w2off + (K + kInferAhead) * 1024);
-        acc[0] = mfma(a[0], bcur, acc[0]);
-        acc[1] = mfma(a[1], bcur, acc[1]);
-        acc[2] = mfma(a[2], bcur, acc[2]);
-        if constexpr (K + 1 < kK2) {
-            a[0] = l2_operand<K + 1, 0>(ab, zt, yt, zero);
-            a[1] = l2_operand<K + 1, 1>(ab, zt, yt, zero);
-            a[2] = l2_operand<K + 1, 2>(ab, zt, yt, zero);
+    return ((G / 2) * 3 + D) * 2 + G % 2;
+}
+
+template <int G>
+__device__ __forceinline__ void l2_fetch_weights(const char *w2, int w2off, uint4 (&wq)[kL2Ring])
+{
+    wq[(3 * G + 0) % kL2Ring] = *reinterpret_cast<const uint4 *>(w2 + w2off + l2_half_tap<G, 0>() * 1024);
+    wq[(3 * G + 1) % kL2Ring] = *reinterpret_cast<const uint4 *>(w2 + w2off + l2_half_tap<G, 1>() * 1024);
+    wq[(3 * G + 2) % kL2Ring] = *reinterpret_cast<const uint4 *>(w2 + w2off + l2_half_tap<G, 2>() * 1024);
+}
+
+// Group G: its matrix ops, each operand regathered for group G+1 as soon as
+// its last op is issued.  The two z blocks alternate so that no accumulator
+// is hit by consecutive ops.
+template <int G>
+__device__ __forceinline__ void layer2_groups(const char *ab, const int (&zt)[4], const int (&yt)[3],
+                                              const char *w2, int w2off, uint4 (&op)[kInferBlocks],
+                                              uint4 (&wq)[kL2Ring], f32x16 (&acc)[kInferBlocks])
+{
+    if constexpr (G < kL2Groups) {
+        if constexpr (G + 2 < kL2Groups) l2_fetch_weights<G + 2>(w2, w2off, wq);
+        const uint4 wm = wq[(3 * G + 0) % kL2Ring], w0 = wq[(3 * G + 1) % kL2Ring], wp = wq[(3 * G + 2) % kL2Ring];
+        constexpr bool more = G + 1 < kL2Groups;
+        // x' = 0: block x = 0 (dx 0), x = 1 (dx -1)
+        acc[0] = mfma(op[0], w0, acc[0]);
+        acc[3] = mfma(op[3], w0, acc[3]);
+        acc[1] = mfma(op[0], wm, acc[1]);
+        acc[4] = mfma(op[3], wm, acc[4]);
+        if constexpr (more) {
+            op[0] = l2_operand<G + 1, 0, 0>(ab, zt, yt);
+            op[3] = l2_operand<G + 1, 1, 0>(ab, zt, yt);
         }
-        acc[3] = mfma(a[3], bcur, acc[3]);
-        acc[4] = mfma(a[4], bcur, acc[4]);
-        acc[5] = mfma(a[5], bcur, acc[5]);
-        if constexpr (K + 1 < kK2) {
-            a[3] = l2_operand<K + 1, 3>(ab, zt, yt, zero);
-            a[4] = l2_operand<K + 1, 4>(ab, zt, yt, zero);
-            a[5] = l2_operand<K + 1, 5>(ab, zt, yt, zero);
+        // x' = 1: x = 2 (dx -1), x = 0 (dx +1), x = 1 (dx 0)
+        acc[2] = mfma(op[1], wm, acc[2]);
+        acc[5] = mfma(op[4], wm, acc[5]);
+        acc[0] = mfma(op[1], wp, acc[0]);
+        acc[3] = mfma(op[4], wp, acc[3]);
+        acc[1] = mfma(op[1], w0, acc[1]);
+        acc[4] = mfma(op[4], w0, acc[4]);
+        if constexpr (more) {
+            op[1] = l2_operand<G + 1, 0, 1>(ab, zt, yt);
+            op[4] = l2_operand<G + 1, 1, 1>(ab, zt, yt);
         }
-        // keep each half-tap's loads next to its matrix ops: hoisting later
+        // x' = 2: x = 2 (dx 0), x = 1 (dx +1)
+        acc[2] = mfma(op[2], w0, acc[2]);
+        acc[5] = mfma(op[5], w0, acc[5]);
+        acc[1] = mfma(op[2], wp, acc[1]);
+        acc[4] = mfma(op[5], wp, acc[4]);
+        if constexpr (more) {
+            op[2] = l2_operand<G + 1, 0, 2>(ab, zt, yt);
+            op[5] = l2_operand<G + 1, 1, 2>(ab, zt, yt);
+        }
+        // keep each group's loads next to its matrix ops: hoisting later
         // operands would exceed the register budget
         __builtin_amdgcn_sched_barrier(0);
-        layer2_steps<K + 1>(ab, zt, yt, zero, w2, w2off, a, bq, acc);
+        layer2_groups<G + 1>(ab, zt, yt, w2, w2off, op, wq, acc);
     }
 }
 
@@ -478,7 +511,6 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int i = lane & 31, hi = lane >> 5;
     const int z0 = 2 * wave;
-    if (tid < kActStride / 2) reinterpret_cast<uint32_t *>(&L.u.act[kZeroRow * kActStride])[tid] = 0u;  // beyond G
     if (tid == 0) L.feat[kZeroRow] = make_uint2(0u, 0u);
     const float b1 = theta[kOffB1 + i], b2 = theta[kOffB2 + i], b3 = theta[kOffB3];
     // address terms (bytes): act rows for layer 2, feature rows for layer 1
@@ -495,7 +527,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         yt_act[k] = yr * kActRowB + 16 * hi;
         yt_feat[k] = yr * 8;
     }
-    const int zero_act = kZeroRow * kActRowB + 16 * hi, zero_feat = kZeroRow * 8;
+    const int zero_feat = kZeroRow * 8;
     // this thread's feature/head visits: z = z0 + hi, y = i, x = 0..2
     const int zv = z0 + hi;
     const int pv = (zv * 32 + i) * 3;
@@ -579,28 +611,28 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
                 *reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(L.u.act) + ebase + off) =
                     (uint16_t)f2bf(fmaxf(acc[b][v] + b1, 0.0f));
             }
-        // the first layer-2 weight fragments, in flight across the barrier
-        uint4 bq[kInferAhead];
-#pragma unroll
-        for (int k = 0; k < kInferAhead; ++k)
-            bq[k] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(f.w2) + w2off + k * 1024);
+        // the first two groups' layer-2 weight fragments, in flight across
+        // the barrier
+        const char *w2 = reinterpret_cast<const char *>(f.w2);
+        uint4 wq[kL2Ring];
+        l2_fetch_weights<0>(w2, w2off, wq);
+        l2_fetch_weights<1>(w2, w2off, wq);
         __syncthreads();
 
-        // ---- layer 2: K = t*32 + ci (864), operands one half-tap ahead ----
+        // ---- layer 2: 27 taps x 32 channels, in groups of 3 taps x 16 ----
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
         {
-            uint4 a[NB];
-            a[0] = l2_operand<0, 0>(ab, zt_act, yt_act, zero_act);
-            a[1] = l2_operand<0, 1>(ab, zt_act, yt_act, zero_act);
-            a[2] = l2_operand<0, 2>(ab, zt_act, yt_act, zero_act);
-            a[3] = l2_operand<0, 3>(ab, zt_act, yt_act, zero_act);
-            a[4] = l2_operand<0, 4>(ab, zt_act, yt_act, zero_act);
-            a[5] = l2_operand<0, 5>(ab, zt_act, yt_act, zero_act);
-            layer2_steps<0>(ab, zt_act, yt_act, zero_act, reinterpret_cast<const char *>(f.w2), w2off, a, bq,
-                            acc);
+            uint4 op[NB];
+            op[0] = l2_operand<0, 0, 0>(ab, zt_act, yt_act);
+            op[1] = l2_operand<0, 0, 1>(ab, zt_act, yt_act);
+            op[2] = l2_operand<0, 0, 2>(ab, zt_act, yt_act);
+            op[3] = l2_operand<0, 1, 0>(ab, zt_act, yt_act);
+            op[4] = l2_operand<0, 1, 1>(ab, zt_act, yt_act);
+            op[5] = l2_operand<0, 1, 2>(ab, zt_act, yt_act);
+            layer2_groups<0>(ab, zt_act, yt_act, w2, w2off, op, wq, acc);
         }
         fetch(t + gridDim.x);   // next tile's bins, in flight through the head
         __syncthreads();  // every wave has read act1

@@ -124,8 +124,9 @@ void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_m
 // composite: every job has nchroma == 0 (CHD_DEC_PAL_2D filters the composite)
 void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
                   const float *cfilt, int n_lblocks, hipStream_t s, bool composite);
+// shift: samples dropped from the start of each field (black appended)
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
-                 hipStream_t s);
+                 hipStream_t s, int shift = 0);
 void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s);
 void launch_chroma_to_planes(const Geom &g, const float *src, float *dst, int64_t plane_stride, int n_fields,
                              hipStream_t s);

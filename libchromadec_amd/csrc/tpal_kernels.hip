@@ -1584,17 +1584,21 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
 // ---------------------------------------------------------------------------
 
 // Copy .tbc-layout fields (fh x fw uint16, contiguous) into the packed store.
+// shift > 0 drops the first `shift` samples of every field and appends as many
+// black ones (SourceField::shiftSecondField, source_field.cpp:51-66).
 __global__ void pack_fields(const Geom g, const uint16_t *__restrict__ src, int64_t src_field_elems,
-                            uint16_t *__restrict__ dst, int n_fields)
+                            uint16_t *__restrict__ dst, int n_fields, int shift)
 {
     const int64_t total = (int64_t)n_fields * g.fh * g.pitch;
+    const int64_t samples = (int64_t)g.fh * g.fw;
     const uint16_t blk = (uint16_t)g.black;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t f = i / g.field_elems;
         const int64_t r = i - f * g.field_elems;
         const int line = (int)(r / g.pitch);
         const int col = (int)(r - (int64_t)line * g.pitch) - g.lmargin;
-        dst[i] = (col >= 0 && col < g.fw) ? src[f * src_field_elems + (int64_t)line * g.fw + col] : blk;
+        const int64_t at = (int64_t)line * g.fw + col + shift;
+        dst[i] = (col >= 0 && col < g.fw && at < samples) ? src[f * src_field_elems + at] : blk;
     }
 }
 
@@ -1715,9 +1719,10 @@ void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *
 }
 
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
-                 hipStream_t s)
+                 hipStream_t s, int shift)
 {
-    if (n_fields > 0) hipLaunchKernelGGL(pack_fields, dim3(1024), dim3(256), 0, s, g, src, src_field_elems, dst, n_fields);
+    if (n_fields > 0)
+        hipLaunchKernelGGL(pack_fields, dim3(1024), dim3(256), 0, s, g, src, src_field_elems, dst, n_fields, shift);
 }
 
 void launch_chroma_to_planes(const Geom &g, const float *src, float *dst, int64_t plane_stride, int n_fields,

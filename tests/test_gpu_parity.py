@@ -42,6 +42,34 @@ def test_transform3d_full_frames(orc, patch_capture, fmt):
         _check(got, _oracle(orc, 8, fields, f, fmt), fmt)
 
 
+@pytest.mark.parametrize("kind", [6, 8])
+def test_subcarrier_locked_second_field_shift(orc, patch_capture, tmp_path, kind):
+    """Subcarrier-locked PAL: the second field of every frame is moved two
+    samples earlier, with two black samples appended, before decoding
+    (SourceField::shiftSecondField, source_field.cpp:51-66)."""
+    import json
+    from libchromadec_amd import synth
+    _, fields = patch_capture
+    path = str(tmp_path / "sclocked.tbc")
+    side = synth.write_tbc(path, fields)
+    with open(side) as fh:
+        meta = json.load(fh)
+    meta["videoParameters"]["isSubcarrierLocked"] = True
+    with open(side, "w") as fh:
+        json.dump(meta, fh)
+    shifted = fields.copy()
+    for f in range(1, fields.shape[0], 2):   # isFirstField is false for odd fields
+        flat = fields[f].reshape(-1)
+        shifted[f] = np.concatenate([flat[2:], np.full(2, synth.BLACK, flat.dtype)]).reshape(fields.shape[1:])
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444ps").commit()
+    for f in (0, 2):
+        got = dec.decode(f)
+        _check(got, _oracle(orc, kind, shifted, f, "yuv444ps"), "yuv444ps")
+        # the shift is visible: the unshifted fields decode differently
+        assert np.abs(got - _oracle(orc, kind, fields, f, "yuv444ps")).max() > 1e-2
+
+
 @pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
 @pytest.mark.parametrize("capture", ["bars_capture", "patch_capture"])
 def test_pal2d_full_frames(orc, request, capture, fmt):
