@@ -634,6 +634,12 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
             op[5] = l2_operand<0, 1, 2>(ab, zt_act, yt_act);
             layer2_groups<0>(ab, zt_act, yt_act, w2, w2off, op, wq, acc);
         }
+        // head weights before the next tile's bins: loads complete in order,
+        // so the head then waits only for its own (w2off is opaque per tile,
+        // which keeps these loads in the loop)
+        const char *w3 = reinterpret_cast<const char *>(f.w3t);
+        const uint4 w3a = *reinterpret_cast<const uint4 *>(w3 + w2off);
+        const uint4 w3b = *reinterpret_cast<const uint4 *>(w3 + w2off + 1024);
         fetch(t + gridDim.x);   // next tile's bins, in flight through the head
         __syncthreads();  // every wave has read act1
 #pragma unroll
@@ -648,7 +654,6 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
 
         // ---- head: G[q][t] = sum_c act2[q][c] W3[t][c] ------------------
         {
-            const uint4 w3a = f.w3t[lane], w3b = f.w3t[64 + lane];
             uint4 a0[NB], a1[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {

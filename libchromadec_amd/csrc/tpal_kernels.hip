@@ -153,8 +153,10 @@ __device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
     const float r1 = 4.0f * kU * (ma + mb * th), r2 = 4.0f * kU * (mb + ma * th);
     const float d1 = fmaf(db, th, da) + r1;
     const float d2 = fmaf(da, th, db) + r2;
-    if (fabsf(g1) <= d1 || fabsf(g2) <= d2) return -1;
-    return (g1 < 0.0f || g2 < 0.0f) ? 0 : 1;
+    // bitwise ors: no short-circuit branches in the unrolled mask loop
+    const bool unsure = (fabsf(g1) <= d1) | (fabsf(g2) <= d2);
+    const bool discard = (g1 < 0.0f) | (g2 < 0.0f);
+    return unsure ? -1 : (discard ? 0 : 1);
 }
 
 __device__ __forceinline__ int decide64(double ar, double ai, double br, double bi, double th)
@@ -214,11 +216,6 @@ __device__ __forceinline__ uint32_t mirror_rows(uint32_t w)
     const uint32_t r = __builtin_bitreverse32(w);  // bit y -> 31 - y
     return (r << 9) | (r >> 23);                   // + 9 (mod 32)
 }
-
-// Visit plane (x, z) handled in mask step i by lanes 0-31 (half 0) and
-// lanes 32-63 (half 1); bit y of the step's ballot is visit (z, y, x).
-__device__ __forceinline__ constexpr int visit_x(int half, int i) { return half == 0 ? (i < 8 ? 2 : 3) : (i < 4 ? 3 : 4); }
-__device__ __forceinline__ constexpr int visit_z(int half, int i) { return half == 0 ? (i < 8 ? i : i - 8) : (i < 4 ? i + 4 : i - 4); }
 
 // acc + x * c for a coefficient known at compile time after unrolling; the
 // exact 0 / +-1 cases cost nothing.
@@ -497,20 +494,43 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                 r10 = q[0];
                 r11 = q[1];
             }
-#pragma unroll 4
-            for (int i = 0; i < 12; ++i) {
-                const int x = vhalf ? visit_x(1, i) : visit_x(0, i);
-                const int z = vhalf ? visit_z(1, i) : visit_z(0, i);
-                const int y = vy;
-                const int zr = (10 - z) & 7, yr = (40 - y) & 31, xr = 8 - x;
-                const int d = decide32(Sat(L, x - 2, y, z), Sat(L, xr - 2, yr, zr), err,
-                                         g.thr_uniform >= 0.0f ? g.thr_uniform : thr32[(z * 32 + y) * 3 + (x - 2)]);
-                const uint64_t kb = __ballot(d > 0);
-                const uint64_t fb = __ballot(d < 0);
-                if (vy == 0) {
-                    const int w = (x - 2) * 8 + z;
-                    L.bal[w] = (uint32_t)(kb >> (32 * vhalf));
-                    L.flg[w] = (uint32_t)(fb >> (32 * vhalf));
+            {
+                // Step i (unrolled) decides plane (x, z) = (2 + i/4, i%4) on
+                // lanes 0-31 and plane (x, z + 4) on lanes 32-63, so the
+                // half enters every offset as a lane term and the rest is an
+                // immediate.  The mirror of plane z lies at z' = (10 - z) % 8,
+                // which is 4 above the lower half's for z < 3 and 4 below for
+                // z = 3.  The next step's bins are read while this one decides.
+                const bool uni = g.thr_uniform >= 0.0f;
+                const int yo = vy * kSpad + 4 * vhalf;
+                const int yrp = ((40 - vy) & 31) * kSpad + 4 * vhalf, yrm = yrp - 8 * vhalf;
+                const int tho = vy * 3 + 384 * vhalf;
+                auto off_a = [&](int i) { return (i >> 2) * kSk + (i & 3) + yo; };
+                auto off_b = [&](int i) {
+                    const int zr = (10 - (i & 3)) & 7;
+                    return (4 - (i >> 2)) * kSk + zr + ((i & 3) < 3 ? yrp : yrm);
+                };
+                c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
+#pragma unroll
+                for (int i = 0; i < 12; ++i) {
+                    c32 an = a, bn = b;
+                    if (i + 1 < 12) {
+                        an = L.u.S[off_a(i + 1)];
+                        bn = L.u.S[off_b(i + 1)];
+                    }
+                    const int k = i >> 2, z = i & 3;
+                    const float th = uni ? g.thr_uniform : thr32[z * 96 + k + tho];
+                    const int d = decide32(a, b, err, th);
+                    const uint64_t kb = __ballot(d > 0);
+                    const uint64_t fb = __ballot(d < 0);
+                    if (vy == 0) {
+                        const int w = k * 8 + z + 4 * vhalf;
+                        L.bal[w] = vhalf ? (uint32_t)(kb >> 32) : (uint32_t)kb;
+                        L.flg[w] = vhalf ? (uint32_t)(fb >> 32) : (uint32_t)fb;
+                    }
+                    a = an;
+                    b = bn;
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
             __syncthreads();
