@@ -330,6 +330,12 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
 }
 
 // Enqueue one chunk: copy job lists, launch the tile and demod kernels.
+// Partial chroma planes per frame job (FrameJob.nchroma) for the plan's kind.
+static int plan_nchroma(const chd_tpal_plan *plan)
+{
+    return plan->kind == CHD_DEC_TRANSFORM_3D ? 4 : plan->kind == CHD_DEC_TRANSFORM_2D ? 2 : 0;
+}
+
 static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hipStream_t s)
 {
     const Geom &g = plan->g;
@@ -367,7 +373,7 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hi
         tpal::launch_tpal2d(g, plan->d_field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
     if (plan->timing) hipEventRecord(plan->ev[1], s);
     tpal::launch_demod(g, plan->d_frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
-                       plan->n_lblocks, s, plan->kind == CHD_DEC_PAL_2D);
+                       plan->n_lblocks, s, plan_nchroma(plan));
     if (plan->timing) hipEventRecord(plan->ev[2], s);
     e = hipGetLastError();
     if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: kernel launch", e));
@@ -739,7 +745,7 @@ chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: job upload", e));
         plan->staging_pending = true;
         tpal::launch_demod(g, plan->d_frame_jobs, cnt, plan->d_sine, plan->d_cosine, plan->d_cfilt, plan->n_lblocks,
-                           s, false);
+                           s, 1);
         e = hipGetLastError();
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: launch", e));
     }
@@ -817,7 +823,7 @@ void batch_all(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
     if (b->border) tpal::launch_fill_border(plan->g, b->d_out, b->n_frames, s);
     batch_tiles(b, s, strip, groups);
     tpal::launch_demod(plan->g, b->d_frame, b->n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
-                       plan->n_lblocks, s, plan->kind == CHD_DEC_PAL_2D);
+                       plan->n_lblocks, s, plan_nchroma(plan));
 }
 
 hipError_t batch_capture(chd_tpal_batch *b)

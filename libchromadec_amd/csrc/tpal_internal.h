@@ -121,9 +121,10 @@ void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *
                    int strip = kStrip3, int max_groups = 0);
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
                    const double *thr64, hipStream_t s);
-// composite: every job has nchroma == 0 (CHD_DEC_PAL_2D filters the composite)
+// nchroma: the FrameJob.nchroma every job of the launch has (4, 2, 1, or 0
+// for CHD_DEC_PAL_2D, which filters the composite)
 void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
-                  const float *cfilt, int n_lblocks, hipStream_t s, bool composite);
+                  const float *cfilt, int n_lblocks, hipStream_t s, int nchroma);
 // shift: samples dropped from the start of each field (black appended)
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
                  hipStream_t s, int shift = 0);

@@ -42,6 +42,36 @@ namespace tpal {
 struct c32 {
     float x, y;
 };
+
+// Global-memory accesses through vector-memory instructions.  A generic
+// pointer gives flat instructions, which also count as LDS traffic, so every
+// wait for LDS (before each barrier) would wait for them too.  (Native vector
+// types: a HIP vector struct is copied as bytes and loses the address space.)
+typedef float gf32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned gu32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned gu32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 gload_f4(const float *p)
+{
+    const gf32x4 v = *(const __attribute__((address_space(1))) gf32x4 *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gload_u4(const void *p)
+{
+    const gu32x4 v = *(const __attribute__((address_space(1))) gu32x4 *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gload_u2(const uint16_t *p)
+{
+    const gu32x2 v = *(const __attribute__((address_space(1))) gu32x2 *)p;
+    return make_uint2(v.x, v.y);
+}
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) uint16_t gu16;
+__device__ __forceinline__ void gstore_f4(float *p, float a, float b, float c, float d)
+{
+    gf32x4 v = {a, b, c, d};
+    *(__attribute__((address_space(1))) gf32x4 *)p = v;
+}
 __device__ __forceinline__ c32 cadd(c32 a, c32 b) { return {a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ c32 csub(c32 a, c32 b) { return {a.x - b.x, a.y - b.y}; }
 
@@ -227,10 +257,9 @@ __device__ __forceinline__ float cmac(float acc, float x, float c)
     return fmaf(x, c, acc);
 }
 
-__device__ __forceinline__ void load16(const uint16_t *p, float black, float (&s)[16])
+// 16 samples (two uint4 of packed u16) minus black, as floats.
+__device__ __forceinline__ void unpack16(const uint4 u0, const uint4 u1, float black, float (&s)[16])
 {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
-    const uint4 u0 = q[0], u1 = q[1];
     const uint32_t w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -389,6 +418,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
     float carry[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) carry[i] = 0.0f;
+    const uint32_t blk2 = (uint32_t)g.black * 0x10001u;   // two black samples
 
     // apply mode: the next tile's spectrum and mask are loaded into registers
     // while the current tile is inverted (its loads would otherwise wait on
@@ -413,16 +443,23 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
         const int64_t tile = ((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx;
         if constexpr (MODE != kTileApply) {
             // ---- forward x: two rows per lane -> x bins 2..6 ----------------
+            // both rows' loads are issued before either is used; a row
+            // outside the tile's lines reads as black (zero after black
+            // subtraction)
+            uint4 raw[2][2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                raw[h][0] = raw[h][1] = make_uint4(blk2, blk2, blk2, blk2);
+                if (rowp[h] != nullptr) {
+                    raw[h][0] = gload_u4(rowp[h] + tileX);
+                    raw[h][1] = gload_u4(rowp[h] + tileX + 8);
+                }
+            }
             float l1 = 0.0f;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 float s[16];
-                if (rowp[h] != nullptr) {
-                    load16(rowp[h] + tileX, g.black, s);
-                } else {
-#pragma unroll
-                    for (int n = 0; n < 16; ++n) s[n] = 0.0f;
-                }
+                unpack16(raw[h][0], raw[h][1], g.black, s);
                 c32 X[5];
                 l1 += xdft16(s, X) * roww[h];
                 const int m = (lane + 64 * h) & 15;
@@ -485,14 +522,12 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
             // rows for a possible FP64 re-decision, fetched while the mask runs
             uint4 r00 = make_uint4(0u, 0u, 0u, 0u), r01 = r00, r10 = r00, r11 = r00;
             if (rowp[0] != nullptr) {
-                const uint4 *q = reinterpret_cast<const uint4 *>(rowp[0] + tileX);
-                r00 = q[0];
-                r01 = q[1];
+                r00 = gload_u4(rowp[0] + tileX);
+                r01 = gload_u4(rowp[0] + tileX + 8);
             }
             if (rowp[1] != nullptr) {
-                const uint4 *q = reinterpret_cast<const uint4 *>(rowp[1] + tileX);
-                r10 = q[0];
-                r11 = q[1];
+                r10 = gload_u4(rowp[1] + tileX);
+                r11 = gload_u4(rowp[1] + tileX + 8);
             }
             {
                 // Step i (unrolled) decides plane (x, z) = (2 + i/4, i%4) on
@@ -687,23 +722,23 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
             if (owned && orow_ok) {
                 // columns tileX .. tileX+7 are complete (tiles tx-1 and tx)
                 const int c0 = tileX - g.avs + kColOrigin;
-                float4 *dst = reinterpret_cast<float4 *>(orow + c0);
+                float *dst = orow + c0;
                 float v8[8];
 #pragma unroll
                 for (int n = 0; n < 8; ++n) {
                     const int col = tileX + n;
                     v8[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
                 }
-                dst[0] = make_float4(v8[0], v8[1], v8[2], v8[3]);
-                dst[1] = make_float4(v8[4], v8[5], v8[6], v8[7]);
+                gstore_f4(dst, v8[0], v8[1], v8[2], v8[3]);
+                gstore_f4(dst + 4, v8[4], v8[5], v8[6], v8[7]);
                 if (tx == g.n_tx3 - 1) {
 #pragma unroll
                     for (int n = 0; n < 8; ++n) {
                         const int col = tileX + 8 + n;
                         v8[n] = (col >= g.avs && col < g.ave) ? out[8 + n] : 0.0f;
                     }
-                    dst[2] = make_float4(v8[0], v8[1], v8[2], v8[3]);
-                    dst[3] = make_float4(v8[4], v8[5], v8[6], v8[7]);
+                    gstore_f4(dst + 8, v8[0], v8[1], v8[2], v8[3]);
+                    gstore_f4(dst + 12, v8[4], v8[5], v8[6], v8[7]);
                 }
             }
 #pragma unroll
@@ -730,7 +765,9 @@ struct Lds2 {
 
 __device__ __forceinline__ void load32(const uint16_t *p, float black, float (&s)[32])
 {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    uint4 q[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) q[h] = gload_u4(p + 8 * h);
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
         const uint4 u = q[h];
@@ -919,7 +956,7 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
         }
         if (live && tx >= tx_own0 && row_ok) {
             const int c0 = tileX - g.avs + kColOrigin;
-            float4 *dst = reinterpret_cast<float4 *>(orow + c0);
+            float *dst = orow + c0;
             float v16[16];
 #pragma unroll
             for (int n = 0; n < 16; ++n) {
@@ -927,7 +964,7 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
                 v16[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) dst[i] = make_float4(v16[4 * i], v16[4 * i + 1], v16[4 * i + 2], v16[4 * i + 3]);
+            for (int i = 0; i < 4; ++i) gstore_f4(dst + 4 * i, v16[4 * i], v16[4 * i + 1], v16[4 * i + 2], v16[4 * i + 3]);
             if (tx == g.n_tx2 - 1) {
 #pragma unroll
                 for (int n = 0; n < 16; ++n) {
@@ -936,7 +973,7 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
                 }
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    dst[4 + i] = make_float4(v16[4 * i], v16[4 * i + 1], v16[4 * i + 2], v16[4 * i + 3]);
+                    gstore_f4(dst + 16 + 4 * i, v16[4 * i], v16[4 * i + 1], v16[4 * i + 2], v16[4 * i + 3]);
             }
         }
 #pragma unroll
@@ -994,14 +1031,14 @@ __device__ __forceinline__ void store_pixel(const Geom &g, const OutSpec &os, vo
 {
     const float ey = (Y - g.black) * g.ey_scale;
     if (os.fmt == 1 /*YUV444PS*/ || os.fmt == 5 /*GRAYS*/) {
-        float *out = reinterpret_cast<float *>(outp);
+        gfloat *out = (gfloat *)outp;
         out[o] = fminf(fmaxf(ey, os.eyLo), os.eyHi);
         if (os.fmt == 1) {
             out[os.plane + o] = fminf(fmaxf(U * g.ecb_scale, os.ecLo), os.ecHi);
             out[2 * os.plane + o] = fminf(fmaxf(V * g.ecr_scale, os.ecLo), os.ecHi);
         }
     } else if (os.fmt == 0 /*YUV444P16*/ || os.fmt == 4 /*GRAY16*/) {
-        uint16_t *out = reinterpret_cast<uint16_t *>(outp);
+        gu16 *out = (gu16 *)outp;
         out[o] = (uint16_t)fminf(fmaxf(h273_round(219.0f * 256.0f * ey + 16.0f * 256.0f), os.yLo), os.yHi);
         if (os.fmt == 0) {
             const float ecb = U * g.ecb_scale, ecr = V * g.ecr_scale;
@@ -1011,7 +1048,7 @@ __device__ __forceinline__ void store_pixel(const Geom &g, const OutSpec &os, vo
     } else if (os.fmt == 3 /*RGBS*/) {
         // E'R/E'G/E'B planes (convertToFloatRGB)
         const float u = U * g.ey_scale, v = V * g.ey_scale;
-        float *out = reinterpret_cast<float *>(outp);
+        gfloat *out = (gfloat *)outp;
         out[o] = fminf(fmaxf(ey + g.rgb[0] * v, g.rgb_lo[0]), g.rgb_hi[0]);
         out[os.plane + o] = fminf(fmaxf(ey + g.rgb[1] * u + g.rgb[2] * v, g.rgb_lo[1]), g.rgb_hi[1]);
         out[2 * os.plane + o] = fminf(fmaxf(ey + g.rgb[3] * u, g.rgb_lo[2]), g.rgb_hi[2]);
@@ -1021,7 +1058,7 @@ __device__ __forceinline__ void store_pixel(const Geom &g, const OutSpec &os, vo
         const float s = 65535.0f * g.ey_scale;
         const float ry = fminf(fmaxf((Y - g.black) * s, 0.0f), 65535.0f);
         const float ru = U * s, rv = V * s;
-        uint16_t *out = reinterpret_cast<uint16_t *>(outp) + 3 * o;
+        gu16 *out = (gu16 *)outp + 3 * o;
         out[0] = (uint16_t)fminf(fmaxf(ry + g.rgb[0] * rv, 0.0f), 65535.0f);
         out[1] = (uint16_t)fminf(fmaxf(ry + g.rgb[1] * ru + g.rgb[2] * rv, 0.0f), 65535.0f);
         out[2] = (uint16_t)fminf(fmaxf(ry + g.rgb[3] * ru, 0.0f), 65535.0f);
@@ -1095,7 +1132,7 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
                 auto px = [&](int ln) -> float {
                     if (ln < 0 || ln >= g.fh) return 0.0f;  // blackLine (zeros)
                     if (comp == nullptr) return g.black;
-                    return (float)comp[(int64_t)ln * g.pitch + i + g.lmargin];
+                    return (float)((const gu16 *)comp)[(int64_t)ln * g.pitch + i + g.lmargin];
                 };
                 const float in0 = px(number), in1 = px(number - 1), in2 = px(number + 1);
                 const float in3 = px(number - 2), in4 = px(number + 2);
@@ -1265,7 +1302,7 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
                 auto px = [&](int ln) -> float {
                     if (ln < 0 || ln >= g.fh) return 0.0f;  // blackLine (zeros)
                     if (comp == nullptr) return g.black;
-                    return (float)comp[(int64_t)ln * g.pitch + i + g.lmargin];
+                    return (float)((const gu16 *)comp)[(int64_t)ln * g.pitch + i + g.lmargin];
                 };
                 const float in0 = px(number), in1 = px(number - 1), in2 = px(number + 1);
                 const float in3 = px(number - 2), in4 = px(number + 2);
@@ -1360,8 +1397,11 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
 // partial plane is read once plus a 6-row halo per block, as float4.  The
 // FIR inputs t0..t3 of a line (the same definitions and summation order as
 // pal_demod_quad) pass through LDS once, double-buffered: one barrier per
-// line.  FrameJob.nchroma == 0 (CHD_DEC_PAL_2D) filters the composite itself
-// and takes luma from the re-synthesised chroma.
+// line.  NC is the launch's FrameJob.nchroma: 4 (3D), 2 (2D) or 1 partial
+// planes summed per row, or 0 (CHD_DEC_PAL_2D), which filters the composite
+// itself and takes luma from the re-synthesised chroma.  A row's loads are
+// issued one line ahead and summed only after the current line's filter, so
+// HBM latency overlaps a line of work.
 // ---------------------------------------------------------------------------
 
 constexpr int kQ4Lines = 32;
@@ -1377,7 +1417,7 @@ struct LdsQ4 {
 __device__ __forceinline__ float4 f4add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 __device__ __forceinline__ float f4get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
 
-template <bool PAL2D>
+template <int NC>
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
                 const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks)
@@ -1396,8 +1436,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     const int nl = min(kQ4Lines, lastLine - n0);
     const int fs = g.filter_size;
     const int c0 = 4 * tid - kQHalo;         // this thread's first column, from active_video_start
-    const int nc = job.nchroma;
-    constexpr bool pal2d = PAL2D;   // every job of the launch has nchroma == 0
+    constexpr bool pal2d = NC == 0;
     // columns whose input this thread loads: the active line for separated
     // chroma (zero elsewhere), the line plus the filter halo for PAL_2D
     const bool loads = pal2d ? (c0 + 4 > -kQHalo && c0 < g.aw + kQHalo) : (c0 >= 0 && c0 < g.aw);
@@ -1421,7 +1460,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 auto px = [&](int ln) -> float {
                     if (ln < 0 || ln >= g.fh) return 0.0f;  // blackLine (zeros)
                     if (comp == nullptr) return g.black;
-                    return (float)comp[(int64_t)ln * g.pitch + i + g.lmargin];
+                    return (float)((const gu16 *)comp)[(int64_t)ln * g.pitch + i + g.lmargin];
                 };
                 const float in0 = px(number), in1 = px(number - 1), in2 = px(number + 1);
                 const float in3 = px(number - 2), in4 = px(number + 2);
@@ -1449,32 +1488,44 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
         }
     }
 
-    // Filter input of this thread's columns on field line `line`: the partial
-    // planes summed in order (zero outside the active area), or the composite
-    // (PAL_2D); zero on lines outside the field's active lines.
-    auto load_row = [&](int line) -> float4 {
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // Filter input of this thread's columns on field line `line`, in two
+    // steps: issue() starts the loads, sum() forms the value.  It is the
+    // partial planes summed in order (zero outside the active area), or the
+    // composite (PAL_2D: black when the field is absent); zero on lines
+    // outside the field's active lines.
+    struct Raw {
+        float4 p[NC > 0 ? NC : 1];
+        uint2 c;
+    };
+    auto issue = [&](int line) -> Raw {
+        Raw w;
+#pragma unroll
+        for (int k = 0; k < (NC > 0 ? NC : 1); ++k) w.p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        w.c = make_uint2(0u, 0u);
         if (loads && line >= firstLine && line < lastLine) {
-            if (pal2d) {
-                if (comp == nullptr) {
-                    v = make_float4(g.black, g.black, g.black, g.black);
-                } else {
-                    const uint2 cw = *reinterpret_cast<const uint2 *>(comp + (int64_t)line * g.pitch + g.lmargin +
-                                                                     g.avs + c0);
-                    v = make_float4((float)(cw.x & 0xFFFFu), (float)(cw.x >> 16), (float)(cw.y & 0xFFFFu),
-                                    (float)(cw.y >> 16));
-                }
+            if constexpr (pal2d) {
+                if (comp == nullptr) w.p[0] = make_float4(g.black, g.black, g.black, g.black);
+                else w.c = gload_u2(comp + (int64_t)line * g.pitch + g.lmargin + g.avs + c0);
             } else {
                 const int64_t o = (int64_t)line * g.cpitch + kColOrigin + c0;
-                v = *reinterpret_cast<const float4 *>(planes[0] + o);
-                if (nc > 1) v = f4add(v, *reinterpret_cast<const float4 *>(planes[1] + o));
-                if (nc > 2) v = f4add(v, *reinterpret_cast<const float4 *>(planes[2] + o));
-                if (nc > 3) v = f4add(v, *reinterpret_cast<const float4 *>(planes[3] + o));
-                if (c0 + 4 > g.aw) {
-                    if (c0 + 1 >= g.aw) v.y = 0.0f;
-                    if (c0 + 2 >= g.aw) v.z = 0.0f;
-                    v.w = 0.0f;
-                }
+#pragma unroll
+                for (int k = 0; k < NC; ++k) w.p[k] = gload_f4(planes[k] + o);
+            }
+        }
+        return w;
+    };
+    auto sum = [&](const Raw &w) -> float4 {
+        float4 v = w.p[0];
+        if constexpr (pal2d) {
+            v = f4add(v, make_float4((float)(w.c.x & 0xFFFFu), (float)(w.c.x >> 16), (float)(w.c.y & 0xFFFFu),
+                                     (float)(w.c.y >> 16)));
+        } else {
+#pragma unroll
+            for (int k = 1; k < NC; ++k) v = f4add(v, w.p[k]);
+            if (c0 + 4 > g.aw) {
+                if (c0 + 1 >= g.aw) v.y = 0.0f;
+                if (c0 + 2 >= g.aw) v.z = 0.0f;
+                v.w = 0.0f;
             }
         }
         return v;
@@ -1487,7 +1538,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
         const int col = g.avs + c0 + e;
         w[e] = (loads && col >= 0 && col < g.fw) ? sine[col] + cosine[col] : 0.0f;
     }
-    constexpr int NCF = PAL2D ? 6 : 4;   // chroma taps, plus the luma taps for PAL_2D
+    constexpr int NCF = pal2d ? 6 : 4;   // chroma taps, plus the luma taps for PAL_2D
     float cf[kQHalo + 1][NCF];
 #pragma unroll
     for (int b = 0; b <= kQHalo; ++b)
@@ -1501,12 +1552,17 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     // rows n0-3 .. n0+3 of this thread's columns
     float4 r[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) r[k] = load_row(n0 - 3 + k);
+    for (int k = 0; k < 7; ++k) r[k] = sum(issue(n0 - 3 + k));
 
     for (int li = 0; li < nl; ++li) {
         const int number = n0 + li;
         const int buf = li & 1;
-        const float4 nxt = load_row(number + 4);
+        // this line's composite, then the next row, in flight through the
+        // filter (loads complete in order: the composite first)
+        uint2 cw = make_uint2(0u, 0u);
+        if (!pal2d && outputs && comp != nullptr)
+            cw = gload_u2(comp + (int64_t)number * g.pitch + g.lmargin + g.avs + c0);
+        const Raw nxt = issue(number + 4);
         {
             float4 t[4];
             float *t0 = &t[0].x, *t1 = &t[1].x, *t2 = &t[2].x, *t3 = &t[3].x;
@@ -1547,7 +1603,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                         const float a = fmaf(sb, cf[b][2 * half + 1], sa * cf[b][2 * half]);
                         if (((e + b) & 1) == 0) ev += a;
                         else od += a;
-                        if constexpr (PAL2D) {
+                        if constexpr (pal2d) {
                             if (half == 0) {
                                 // Y filter: lines 0 and +-2 (t0, t1)
                                 const float ay = fmaf(sb, cf[b][5], sa * cf[b][4]);
@@ -1565,14 +1621,16 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 }
             }
             const float bp = L.bp[li], bq = L.bq[li], vsw = L.vsw[li];
+            // the composite of this line: PAL_2D's filter input row itself
             float compv[4];
-            if (comp != nullptr) {
-                const uint2 cw = *reinterpret_cast<const uint2 *>(comp + (int64_t)number * g.pitch + g.lmargin + g.avs + c0);
-                compv[0] = (float)(cw.x & 0xFFFFu); compv[1] = (float)(cw.x >> 16);
-                compv[2] = (float)(cw.y & 0xFFFFu); compv[3] = (float)(cw.y >> 16);
-            } else {
+            if (pal2d) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) compv[e] = g.black;
+                for (int e = 0; e < 4; ++e) compv[e] = f4get(r[3], e);
+            } else {
+                // cw is zero for an absent field, which reads as black
+                const float cb = comp != nullptr ? 0.0f : g.black;
+                compv[0] = cb + (float)(cw.x & 0xFFFFu); compv[1] = cb + (float)(cw.x >> 16);
+                compv[2] = cb + (float)(cw.y & 0xFFFFu); compv[3] = cb + (float)(cw.y >> 16);
             }
             const int frame_line = number * 2 + off;
             const int orow = frame_line - g.first + g.pad_top;
@@ -1586,7 +1644,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 // PAL_2D luma: composite minus (PY sine + QY cosine) * 2, and
                 // column e is itself a sine column when (e & 1) == sel
                 float Y;
-                if constexpr (PAL2D) Y = compv[e] - ((e & 1) == sel ? AY[ps][e] : AY[pc][e]) * w[e] * 2.0f;
+                if constexpr (pal2d) Y = compv[e] - ((e & 1) == sel ? AY[ps][e] : AY[pc][e]) * w[e] * 2.0f;
                 else Y = compv[e] - f4get(r[3], e);
                 const float U = -(PU * bp + QU * bq) * 2.0f;
                 const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
@@ -1595,7 +1653,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) r[k] = r[k + 1];
-        r[6] = nxt;
+        r[6] = sum(nxt);
     }
 }
 
@@ -1708,19 +1766,17 @@ void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_m
 }
 
 void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
-                  const float *cfilt, int n_lblocks, hipStream_t s, bool composite)
+                  const float *cfilt, int n_lblocks, hipStream_t s, int nchroma)
 {
+    const bool composite = nchroma == 0;
     const int n_segs = (g.aw + kDemodThreads - 1) / kDemodThreads;
     if (g.quad && g.aw <= kQ4MaxWidth) {
         const int nlb = (n_lblocks * kDemodLines + kQ4Lines - 1) / kQ4Lines;
         const int blocks = n_jobs * 2 * nlb;
         if (blocks > 0) {
-            if (composite)
-                hipLaunchKernelGGL(pal_demod_quad4<true>, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine,
-                                   cosine, cfilt, nlb);
-            else
-                hipLaunchKernelGGL(pal_demod_quad4<false>, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine,
-                                   cosine, cfilt, nlb);
+            auto kern = nchroma == 4 ? pal_demod_quad4<4> : nchroma == 2 ? pal_demod_quad4<2>
+                      : nchroma == 1 ? pal_demod_quad4<1> : pal_demod_quad4<0>;
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt, nlb);
         }
         return;
     }
