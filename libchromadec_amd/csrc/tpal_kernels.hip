@@ -1493,42 +1493,53 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     // partial planes summed in order (zero outside the active area), or the
     // composite (PAL_2D: black when the field is absent); zero on lines
     // outside the field's active lines.
+    // The loads are unconditional, so that all of them sit in one basic
+    // block and the compiler's wait counts stay exact: a lane with nothing
+    // to load reads the filter table instead and drops the value in sum().
     struct Raw {
         float4 p[NC > 0 ? NC : 1];
         uint2 c;
+        bool ok;
     };
+    const uint16_t *const dummy16 = reinterpret_cast<const uint16_t *>(cfilt);
+    // this thread's column in each input, at line 0 (the dummy where the
+    // thread loads nothing)
+    const float *pbase[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) pbase[k] = loads ? planes[k] + kColOrigin + c0 : cfilt;
+    const uint16_t *const cbase = (loads && comp != nullptr) ? comp + g.lmargin + g.avs + c0 : dummy16;
     auto issue = [&](int line) -> Raw {
         Raw w;
+        w.ok = loads && line >= firstLine && line < lastLine;
+        const int64_t row = w.ok ? line : 0;
+        if constexpr (pal2d) {
+            w.c = gload_u2(cbase + (cbase != dummy16 ? row * g.pitch : 0));
+        } else {
+            const int64_t o = (pbase[0] != cfilt) ? row * g.cpitch : 0;
 #pragma unroll
-        for (int k = 0; k < (NC > 0 ? NC : 1); ++k) w.p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        w.c = make_uint2(0u, 0u);
-        if (loads && line >= firstLine && line < lastLine) {
-            if constexpr (pal2d) {
-                if (comp == nullptr) w.p[0] = make_float4(g.black, g.black, g.black, g.black);
-                else w.c = gload_u2(comp + (int64_t)line * g.pitch + g.lmargin + g.avs + c0);
-            } else {
-                const int64_t o = (int64_t)line * g.cpitch + kColOrigin + c0;
-#pragma unroll
-                for (int k = 0; k < NC; ++k) w.p[k] = gload_f4(planes[k] + o);
-            }
+            for (int k = 0; k < NC; ++k) w.p[k] = gload_f4(pbase[k] + o);
         }
         return w;
     };
     auto sum = [&](const Raw &w) -> float4 {
-        float4 v = w.p[0];
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if constexpr (pal2d) {
-            v = f4add(v, make_float4((float)(w.c.x & 0xFFFFu), (float)(w.c.x >> 16), (float)(w.c.y & 0xFFFFu),
-                                     (float)(w.c.y >> 16)));
+            if (!w.ok) return zero;
+            if (comp == nullptr) return make_float4(g.black, g.black, g.black, g.black);
+            return make_float4((float)(w.c.x & 0xFFFFu), (float)(w.c.x >> 16), (float)(w.c.y & 0xFFFFu),
+                               (float)(w.c.y >> 16));
         } else {
+            float4 v = w.p[0];
 #pragma unroll
             for (int k = 1; k < NC; ++k) v = f4add(v, w.p[k]);
+            if (!w.ok) v = zero;
             if (c0 + 4 > g.aw) {
                 if (c0 + 1 >= g.aw) v.y = 0.0f;
                 if (c0 + 2 >= g.aw) v.z = 0.0f;
                 v.w = 0.0f;
             }
+            return v;
         }
-        return v;
     };
 
     // carrier (exactly one of sine/cosine is non-zero, +-1) of the 4 columns
@@ -1553,16 +1564,18 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     float4 r[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) r[k] = sum(issue(n0 - 3 + k));
-
-    for (int li = 0; li < nl; ++li) {
+    // Line li: rows r[0..6] are lines li-3 .. li+3; `use` holds the loads
+    // of line li+4, `refill` receives those of line li+5.  The loop is
+    // unrolled by two with the two buffers swapping roles, so that no load
+    // result is copied (a copy would wait for it).
+    auto line = [&](int li, const Raw &use, Raw &refill) {
         const int number = n0 + li;
         const int buf = li & 1;
-        // this line's composite, then the next row, in flight through the
-        // filter (loads complete in order: the composite first)
+        // this line's composite, then the row two lines ahead, in flight
+        // through the filter (loads complete in order: the composite first)
         uint2 cw = make_uint2(0u, 0u);
-        if (!pal2d && outputs && comp != nullptr)
-            cw = gload_u2(comp + (int64_t)number * g.pitch + g.lmargin + g.avs + c0);
-        const Raw nxt = issue(number + 4);
+        if constexpr (!pal2d) cw = gload_u2(cbase + (cbase != dummy16 ? (int64_t)number * g.pitch : 0));
+        refill = issue(number + 5);
         {
             float4 t[4];
             float *t0 = &t[0].x, *t1 = &t[1].x, *t2 = &t[2].x, *t3 = &t[3].x;
@@ -1627,7 +1640,8 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
 #pragma unroll
                 for (int e = 0; e < 4; ++e) compv[e] = f4get(r[3], e);
             } else {
-                // cw is zero for an absent field, which reads as black
+                // an absent field reads as black
+                if (comp == nullptr) cw = make_uint2(0u, 0u);
                 const float cb = comp != nullptr ? 0.0f : g.black;
                 compv[0] = cb + (float)(cw.x & 0xFFFFu); compv[1] = cb + (float)(cw.x >> 16);
                 compv[2] = cb + (float)(cw.y & 0xFFFFu); compv[3] = cb + (float)(cw.y >> 16);
@@ -1653,7 +1667,12 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) r[k] = r[k + 1];
-        r[6] = sum(nxt);
+        r[6] = sum(use);
+    };
+    Raw qa = issue(n0 + 4), qb;
+    for (int li = 0; li < nl; li += 2) {
+        line(li, qa, qb);
+        if (li + 1 < nl) line(li + 1, qb, qa);
     }
 }
 
