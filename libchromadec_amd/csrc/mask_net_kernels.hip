@@ -331,15 +331,17 @@ __global__ void __launch_bounds__(256) k_head(const uint16_t *__restrict__ act2,
 
 constexpr int kActStride = 40;  // bf16 per activation row in LDS (32 + 8: spreads gathered rows over banks)
 constexpr int kActRowB = 2 * kActStride;
-constexpr int kZeroRow = kP;    // an all-zero row: the neighbour of a visit across the x edge
-constexpr int kHeadTaps = 9;    // taps per head round (one dz plane)
+constexpr int kZeroRow = kP;    // an all-zero feature row: the neighbour of a visit across the x edge
+constexpr int kHeadSlots = 16;  // head taps per round: taps 0-15, then 16-26
+constexpr int kGSlotB = kP * 4; // bytes per head tap slot: G[t][q'] of every visit
 
 struct LdsInfer {
     union {
         uint16_t act[kP * kActStride];         // act1, then act2
-        float g[(kP + 1) * kHeadTaps];         // head partial sums G[q][t] of one round (+ zero row)
+        float g[kHeadSlots * kP];              // head partial sums of one round, [tap slot][q']
     } u;
     uint2 feat[kP + 1];                        // 4 bf16 features per visit (+ zero row)
+    alignas(16) float bias[2][32];             // layer-1 and layer-2 biases (read as float4)
 };
 
 // Row blocks: wave w owns the 6 blocks (z, x) with z in {2w, 2w+1} and x in
@@ -350,7 +352,6 @@ struct LdsInfer {
 constexpr int kInferWaves = 4;                   // 256 threads; two workgroups per CU
 constexpr int kInferBlocks = 24 / kInferWaves;   // (z, x) row blocks per wave
 constexpr int kInferVisits = 3;                  // feature/head visits per thread: x = 0..2
-constexpr int kGRowB = kHeadTaps * 4;            // bytes per G row
 
 // Byte offset of act row (zt, lane y term, x) for tap T of block B (compile
 // time); zt[k] = ((z0 + k - 1) & 7) * 96 * rowbytes for k = 0..3 covers
@@ -396,7 +397,7 @@ __device__ __forceinline__ void layer1_steps(const char *fb, const int (&zt)[4],
         af[4] = l1_operand<J, 4>(fb, zt, yt, zero, hi);
         af[5] = l1_operand<J, 5>(fb, zt, yt, zero, hi);
 #pragma unroll
-        for (int b = 0; b < kInferBlocks; ++b) acc[b] = mfma(af[b], bf, acc[b]);
+        for (int b = 0; b < kInferBlocks; ++b) acc[b] = mfma(bf, af[b], acc[b]);
         // keep each step's loads next to its matrix ops: hoisting later
         // steps' operands would exceed the register budget
         __builtin_amdgcn_sched_barrier(0);
@@ -449,30 +450,30 @@ __device__ __forceinline__ void layer2_groups(const char *ab, const int (&zt)[4]
         const uint4 wm = wq[(3 * G + 0) % kL2Ring], w0 = wq[(3 * G + 1) % kL2Ring], wp = wq[(3 * G + 2) % kL2Ring];
         constexpr bool more = G + 1 < kL2Groups;
         // x' = 0: block x = 0 (dx 0), x = 1 (dx -1)
-        acc[0] = mfma(op[0], w0, acc[0]);
-        acc[3] = mfma(op[3], w0, acc[3]);
-        acc[1] = mfma(op[0], wm, acc[1]);
-        acc[4] = mfma(op[3], wm, acc[4]);
+        acc[0] = mfma(w0, op[0], acc[0]);
+        acc[3] = mfma(w0, op[3], acc[3]);
+        acc[1] = mfma(wm, op[0], acc[1]);
+        acc[4] = mfma(wm, op[3], acc[4]);
         if constexpr (more) {
             op[0] = l2_operand<G + 1, 0, 0>(ab, zt, yt);
             op[3] = l2_operand<G + 1, 1, 0>(ab, zt, yt);
         }
         // x' = 1: x = 2 (dx -1), x = 0 (dx +1), x = 1 (dx 0)
-        acc[2] = mfma(op[1], wm, acc[2]);
-        acc[5] = mfma(op[4], wm, acc[5]);
-        acc[0] = mfma(op[1], wp, acc[0]);
-        acc[3] = mfma(op[4], wp, acc[3]);
-        acc[1] = mfma(op[1], w0, acc[1]);
-        acc[4] = mfma(op[4], w0, acc[4]);
+        acc[2] = mfma(wm, op[1], acc[2]);
+        acc[5] = mfma(wm, op[4], acc[5]);
+        acc[0] = mfma(wp, op[1], acc[0]);
+        acc[3] = mfma(wp, op[4], acc[3]);
+        acc[1] = mfma(w0, op[1], acc[1]);
+        acc[4] = mfma(w0, op[4], acc[4]);
         if constexpr (more) {
             op[1] = l2_operand<G + 1, 0, 1>(ab, zt, yt);
             op[4] = l2_operand<G + 1, 1, 1>(ab, zt, yt);
         }
         // x' = 2: x = 2 (dx 0), x = 1 (dx +1)
-        acc[2] = mfma(op[2], w0, acc[2]);
-        acc[5] = mfma(op[5], w0, acc[5]);
-        acc[1] = mfma(op[2], wp, acc[1]);
-        acc[4] = mfma(op[5], wp, acc[4]);
+        acc[2] = mfma(w0, op[2], acc[2]);
+        acc[5] = mfma(w0, op[5], acc[5]);
+        acc[1] = mfma(wp, op[2], acc[1]);
+        acc[4] = mfma(wp, op[5], acc[4]);
         if constexpr (more) {
             op[2] = l2_operand<G + 1, 0, 2>(ab, zt, yt);
             op[5] = l2_operand<G + 1, 1, 2>(ab, zt, yt);
@@ -484,21 +485,53 @@ __device__ __forceinline__ void layer2_groups(const char *ab, const int (&zt)[4]
     }
 }
 
-// Partial logit of visit (z, y, X) over the taps of head round R (taps
-// 9R .. 9R+8 in order, dz = R - 1): G rows hold this round's 9 columns.
-// gz = ((z + dz) & 7) * 96 * kGRowB, gy[k] = ((y + k - 1) & 31) * 3 * kGRowB.
-template <int X, int T>
-__device__ __forceinline__ float head_sum(const char *g, int gz, const int (&gy)[3], int zero, float s)
+// Partial logit of visit (z, y, X) over head taps T .. E-1 of the round whose
+// first tap is R0: G holds slot t - R0 of every visit q' = ((z*3 + x)*32 + y).
+// rb[dz+1][dy+1] = ((z+dz)&7)*384 + ((y+dy)&31)*4; taps whose x + dx leaves
+// 0..2 read the zero padding of the reference's filter: they are skipped.
+template <int X, int T, int E, int R0>
+__device__ __forceinline__ float head_sum(const char *g, const int (&rb)[3][3], float s)
 {
-    if constexpr (T < kHeadTaps) {
-        constexpr int dy = T / 3 - 1, dx = T % 3 - 1, x = X + dx;
-        int a;
-        if constexpr (x < 0 || x > 2) a = zero;
-        else a = gz + gy[dy + 1] + x * kGRowB;
-        return head_sum<X, T + 1>(g, gz, gy, zero, s + *reinterpret_cast<const float *>(g + a + 4 * T));
+    if constexpr (T < E) {
+        constexpr int dz = T / 9 - 1, dy = (T / 3) % 3 - 1, dx = T % 3 - 1, x = X + dx;
+        if constexpr (x >= 0 && x <= 2)
+            s += *reinterpret_cast<const float *>(g + rb[dz + 1][dy + 1] + (T - R0) * kGSlotB + x * 128);
+        return head_sum<X, T + 1, E, R0>(g, rb, s);
     } else {
         return s;
     }
+}
+
+// Accumulators of a transposed matrix op start at the bias: lane (y, hi)
+// holds output channels 8j + 4hi + k in acc[b][4j + k].
+__device__ __forceinline__ void acc_from_bias(const float *bias, int hi, f32x16 (&acc)[kInferBlocks])
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float4 bv = *reinterpret_cast<const float4 *>(bias + 8 * j + 4 * hi);
+#pragma unroll
+        for (int b = 0; b < kInferBlocks; ++b) {
+            acc[b][4 * j] = bv.x;
+            acc[b][4 * j + 1] = bv.y;
+            acc[b][4 * j + 2] = bv.z;
+            acc[b][4 * j + 3] = bv.w;
+        }
+    }
+}
+
+// ReLU, BF16, and the activation rows: channels 8j + 4hi .. +3 of visit
+// (z0 + b/3, y, b%3) are one 8-byte store (ebase: this lane's row of block
+// 0 plus its channel half).
+__device__ __forceinline__ void store_act(char *act, int ebase, const f32x16 (&acc)[kInferBlocks])
+{
+#pragma unroll
+    for (int b = 0; b < kInferBlocks; ++b)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint2 pk = make_uint2(pack2(fmaxf(acc[b][4 * j], 0.0f), fmaxf(acc[b][4 * j + 1], 0.0f)),
+                                        pack2(fmaxf(acc[b][4 * j + 2], 0.0f), fmaxf(acc[b][4 * j + 3], 0.0f)));
+            *reinterpret_cast<uint2 *>(act + ebase + ((b / 3) * 96 + b % 3) * kActRowB + 16 * j) = pk;
+        }
 }
 
 __global__ void __launch_bounds__(64 * kInferWaves, 2)
@@ -512,7 +545,8 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     const int i = lane & 31, hi = lane >> 5;
     const int z0 = 2 * wave;
     if (tid == 0) L.feat[kZeroRow] = make_uint2(0u, 0u);
-    const float b1 = theta[kOffB1 + i], b2 = theta[kOffB2 + i], b3 = theta[kOffB3];
+    if (tid < 64) L.bias[tid >> 5][tid & 31] = theta[(tid < 32 ? kOffB1 : kOffB2) + (tid & 31)];
+    const float b3 = theta[kOffB3];
     // address terms (bytes): act rows for layer 2, feature rows for layer 1
     int zt_act[4], zt_feat[4], yt_act[3], yt_feat[3];
 #pragma unroll
@@ -567,8 +601,8 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         for (int j = 0; j < NV; ++j) asm volatile("" : "+v"(ia[j]), "+v"(ib[j]));
         // per-lane byte bases of the epilogue stores and head loads; each
         // store adds a compile-time offset (block, value) to them
-        int ebase = ((z0 * 32 + 4 * hi) * 3) * kActRowB + 2 * i;       // act row of acc[0][0]
-        int gbase = ((z0 * 32 + 4 * hi) * 3) * kGRowB;                  // G row of acc[0][0]
+        int ebase = ((z0 * 32 + i) * 3) * kActRowB + 8 * hi;           // act row of block 0, channel half
+        int gbase = 4 * hi * kGSlotB + (z0 * 96 + i) * 4;                // G slot 4hi, visit q' of block 0
         int hbase = ((z0 * 32 + i) * 3) * kActRowB + 16 * hi;            // head A row of block 0
         int w2off = lane * 16;                                           // weight fragment word
         asm volatile("" : "+v"(ebase), "+v"(gbase), "+v"(hbase), "+v"(w2off));
@@ -595,22 +629,14 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         }
         __syncthreads();
 
+        // The matrix ops are transposed (weights as the A operand): lane
+        // (y, hi) of block b gets output channels 8j + 4hi + k of visit
+        // (z0 + b/3, y, b%3) in acc[b][4j + k].
         f32x16 acc[NB];
         // ---- layer 1: K = t*4 + ci (112) --------------------------------
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
+        acc_from_bias(L.bias[0], hi, acc);
         layer1_steps<0>(fb, zt_feat, yt_feat, zero_feat, hi, f.w1, lane, acc);
-        // act row of acc[b][v]: ((z0 + b/3) * 32 + y) * 3 + b%3, y = 8(v/4) + 4hi + v%4
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int off = (((b / 3) * 32 + 8 * (v >> 2) + (v & 3)) * 3 + b % 3) * kActRowB;
-                *reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(L.u.act) + ebase + off) =
-                    (uint16_t)f2bf(fmaxf(acc[b][v] + b1, 0.0f));
-            }
+        store_act(reinterpret_cast<char *>(L.u.act), ebase, acc);
         // the first two groups' layer-2 weight fragments, in flight across
         // the barrier
         const char *w2 = reinterpret_cast<const char *>(f.w2);
@@ -620,10 +646,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         __syncthreads();
 
         // ---- layer 2: 27 taps x 32 channels, in groups of 3 taps x 16 ----
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
+        acc_from_bias(L.bias[1], hi, acc);
         {
             uint4 op[NB];
             op[0] = l2_operand<0, 0, 0>(ab, zt_act, yt_act);
@@ -642,14 +665,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         const uint4 w3b = *reinterpret_cast<const uint4 *>(w3 + w2off + 1024);
         fetch(t + gridDim.x);   // next tile's bins, in flight through the head
         __syncthreads();  // every wave has read act1
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int off = (((b / 3) * 32 + 8 * (v >> 2) + (v & 3)) * 3 + b % 3) * kActRowB;
-                *reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(L.u.act) + ebase + off) =
-                    (uint16_t)f2bf(fmaxf(acc[b][v] + b2, 0.0f));
-            }
+        store_act(reinterpret_cast<char *>(L.u.act), ebase, acc);
         __syncthreads();
 
         // ---- head: G[q][t] = sum_c act2[q][c] W3[t][c] ------------------
@@ -665,39 +681,45 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
             for (int b = 0; b < NB; ++b) {
 #pragma unroll
                 for (int v = 0; v < 16; ++v) acc[b][v] = 0.0f;
-                acc[b] = mfma(a0[b], w3a, acc[b]);
-                acc[b] = mfma(a1[b], w3b, acc[b]);
+                acc[b] = mfma(w3a, a0[b], acc[b]);
+                acc[b] = mfma(w3b, a1[b], acc[b]);
             }
         }
+        // acc[b][4j + k] is G of tap 8j + 4hi + k (taps 27-31 are zero
+        // weights) at visit (z0 + b/3, y, b%3)
         float logit[NV];
 #pragma unroll
         for (int j = 0; j < NV; ++j) logit[j] = b3;
         const char *gb = reinterpret_cast<const char *>(L.u.g);
-        int gy[3];
+        int rb[3][3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) gy[k] = ((i + k - 1) & 31) * 3 * kGRowB;
-        const int zero_g = kZeroRow * kGRowB;
-        // three rounds of 9 taps (one dz plane each) through LDS
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) rb[a][c] = ((zv + a - 1) & 7) * 384 + ((i + c - 1) & 31) * 4;
+        // two rounds through LDS: taps 0-15 (j = 0, 1), then 16-26 (j = 2, 3)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
             __syncthreads();  // act2 (round 0) or the previous round's G is read
-            const int c = i - kHeadTaps * r;
-            if (c >= 0 && c < kHeadTaps) {
-                char *gw = reinterpret_cast<char *>(L.u.g) + gbase + 4 * c;
+            char *gw = reinterpret_cast<char *>(L.u.g) + gbase;
 #pragma unroll
-                for (int b = 0; b < NB; ++b)
+            for (int b = 0; b < NB; ++b)
 #pragma unroll
-                    for (int v = 0; v < 16; ++v) {
-                        const int off = (((b / 3) * 32 + 8 * (v >> 2) + (v & 3)) * 3 + b % 3) * kGRowB;
-                        *reinterpret_cast<float *>(gw + off) = acc[b][v];
+                for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int off = (8 * jj + k) * kGSlotB + ((b / 3) * 3 + b % 3) * 128;
+                        *reinterpret_cast<float *>(gw + off) = acc[b][4 * (2 * r + jj) + k];
                     }
-            }
-            if (tid < kHeadTaps) L.u.g[kZeroRow * kHeadTaps + tid] = 0.0f;
             __syncthreads();
-            const int gz = ((zv + r - 1) & 7) * 96 * kGRowB;
-            logit[0] = head_sum<0, 0>(gb, gz, gy, zero_g, logit[0]);
-            logit[1] = head_sum<1, 0>(gb, gz, gy, zero_g, logit[1]);
-            logit[2] = head_sum<2, 0>(gb, gz, gy, zero_g, logit[2]);
+            if (r == 0) {
+                logit[0] = head_sum<0, 0, 16, 0>(gb, rb, logit[0]);
+                logit[1] = head_sum<1, 0, 16, 0>(gb, rb, logit[1]);
+                logit[2] = head_sum<2, 0, 16, 0>(gb, rb, logit[2]);
+            } else {
+                logit[0] = head_sum<0, 16, 27, 16>(gb, rb, logit[0]);
+                logit[1] = head_sum<1, 16, 27, 16>(gb, rb, logit[1]);
+                logit[2] = head_sum<2, 16, 27, 16>(gb, rb, logit[2]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < NV; ++j) mask[t * kP + pv + j] = 1.0f / (1.0f + expf(-logit[j]));
