@@ -342,6 +342,7 @@ struct LdsInfer {
     } u;
     uint2 feat[kP + 1];                        // 4 bf16 features per visit (+ zero row)
     alignas(16) float bias[2][32];             // layer-1 and layer-2 biases (read as float4)
+    uint4 w1[kK1 * 64];                        // layer-1 weight fragments, [k-step][lane]
 };
 
 // Row blocks: wave w owns the 6 blocks (z, x) with z in {2w, 2w+1} and x in
@@ -383,25 +384,32 @@ __device__ __forceinline__ uint4 l1_operand(const char *fb, const int (&zt)[4], 
     return make_uint4(lo.x, lo.y, up.x, up.y);
 }
 
+// Layer-1 k-step J: the matrix op of each block, then that block's gather
+// for step J+1 into the same registers; step J+1's weight fragment (LDS,
+// w1l = this lane's word of step 0) is read while step J's ops run.
 template <int J>
 __device__ __forceinline__ void layer1_steps(const char *fb, const int (&zt)[4], const int (&yt)[3], int zero, int hi,
-                                             const uint4 *w1, int lane, f32x16 (&acc)[kInferBlocks])
+                                             const char *w1l, uint4 &bf, uint4 (&af)[kInferBlocks],
+                                             f32x16 (&acc)[kInferBlocks])
 {
     if constexpr (J < kK1) {
-        const uint4 bf = w1[J * 64 + lane];
-        uint4 af[kInferBlocks];
-        af[0] = l1_operand<J, 0>(fb, zt, yt, zero, hi);
-        af[1] = l1_operand<J, 1>(fb, zt, yt, zero, hi);
-        af[2] = l1_operand<J, 2>(fb, zt, yt, zero, hi);
-        af[3] = l1_operand<J, 3>(fb, zt, yt, zero, hi);
-        af[4] = l1_operand<J, 4>(fb, zt, yt, zero, hi);
-        af[5] = l1_operand<J, 5>(fb, zt, yt, zero, hi);
-#pragma unroll
-        for (int b = 0; b < kInferBlocks; ++b) acc[b] = mfma(bf, af[b], acc[b]);
-        // keep each step's loads next to its matrix ops: hoisting later
-        // steps' operands would exceed the register budget
+        constexpr bool more = J + 1 < kK1;
+        const uint4 bcur = bf;
+        if constexpr (more) bf = *reinterpret_cast<const uint4 *>(w1l + (J + 1) * 1024);
+        acc[0] = mfma(bcur, af[0], acc[0]);
+        if constexpr (more) af[0] = l1_operand<J + 1, 0>(fb, zt, yt, zero, hi);
+        acc[1] = mfma(bcur, af[1], acc[1]);
+        if constexpr (more) af[1] = l1_operand<J + 1, 1>(fb, zt, yt, zero, hi);
+        acc[2] = mfma(bcur, af[2], acc[2]);
+        if constexpr (more) af[2] = l1_operand<J + 1, 2>(fb, zt, yt, zero, hi);
+        acc[3] = mfma(bcur, af[3], acc[3]);
+        if constexpr (more) af[3] = l1_operand<J + 1, 3>(fb, zt, yt, zero, hi);
+        acc[4] = mfma(bcur, af[4], acc[4]);
+        if constexpr (more) af[4] = l1_operand<J + 1, 4>(fb, zt, yt, zero, hi);
+        acc[5] = mfma(bcur, af[5], acc[5]);
+        if constexpr (more) af[5] = l1_operand<J + 1, 5>(fb, zt, yt, zero, hi);
         __builtin_amdgcn_sched_barrier(0);
-        layer1_steps<J + 1>(fb, zt, yt, zero, hi, w1, lane, acc);
+        layer1_steps<J + 1>(fb, zt, yt, zero, hi, w1l, bf, af, acc);
     }
 }
 
@@ -546,6 +554,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     const int z0 = 2 * wave;
     if (tid == 0) L.feat[kZeroRow] = make_uint2(0u, 0u);
     if (tid < 64) L.bias[tid >> 5][tid & 31] = theta[(tid < 32 ? kOffB1 : kOffB2) + (tid & 31)];
+    for (int k = tid; k < kK1 * 64; k += 64 * kInferWaves) L.w1[k] = f.w1[k];
     const float b3 = theta[kOffB3];
     // address terms (bytes): act rows for layer 2, feature rows for layer 1
     int zt_act[4], zt_feat[4], yt_act[3], yt_feat[3];
@@ -635,7 +644,18 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         f32x16 acc[NB];
         // ---- layer 1: K = t*4 + ci (112) --------------------------------
         acc_from_bias(L.bias[0], hi, acc);
-        layer1_steps<0>(fb, zt_feat, yt_feat, zero_feat, hi, f.w1, lane, acc);
+        {
+            const char *w1l = reinterpret_cast<const char *>(L.w1) + 16 * lane;
+            uint4 bf = *reinterpret_cast<const uint4 *>(w1l);
+            uint4 af[NB];
+            af[0] = l1_operand<0, 0>(fb, zt_feat, yt_feat, zero_feat, hi);
+            af[1] = l1_operand<0, 1>(fb, zt_feat, yt_feat, zero_feat, hi);
+            af[2] = l1_operand<0, 2>(fb, zt_feat, yt_feat, zero_feat, hi);
+            af[3] = l1_operand<0, 3>(fb, zt_feat, yt_feat, zero_feat, hi);
+            af[4] = l1_operand<0, 4>(fb, zt_feat, yt_feat, zero_feat, hi);
+            af[5] = l1_operand<0, 5>(fb, zt_feat, yt_feat, zero_feat, hi);
+            layer1_steps<0>(fb, zt_feat, yt_feat, zero_feat, hi, w1l, bf, af, acc);
+        }
         store_act(reinterpret_cast<char *>(L.u.act), ebase, acc);
         // the first two groups' layer-2 weight fragments, in flight across
         // the barrier
