@@ -30,6 +30,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "tpal_internal.h"
 
@@ -187,6 +188,25 @@ __device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
     const bool unsure = (fabsf(g1) <= d1) | (fabsf(g2) <= d2);
     const bool discard = (g1 < 0.0f) | (g2 < 0.0f);
     return unsure ? -1 : (discard ? 0 : 1);
+}
+
+// cos / sin (2 pi j / 16) with the exact values at multiples of pi/2.
+constexpr double kC1 = 0.9238795325112867, kS1 = 0.3826834323650898, kR2 = 0.7071067811865476;
+constexpr double kCos16[16] = {1.0, kC1, kR2, kS1, 0.0, -kS1, -kR2, -kC1, -1.0, -kC1, -kR2, -kS1, 0.0, kS1, kR2, kC1};
+constexpr double kSin16[16] = {0.0, kS1, kR2, kC1, 1.0, kC1, kR2, kS1, 0.0, -kS1, -kR2, -kC1, -1.0, -kC1, -kR2, -kS1};
+
+// Four wave sums at once, so that the shuffles' latencies overlap.
+__device__ __forceinline__ void wave_sum4_d(double &a, double &b, double &c, double &d)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ta = __shfl_xor(a, o, 64), tb = __shfl_xor(b, o, 64);
+        const double tc = __shfl_xor(c, o, 64), td = __shfl_xor(d, o, 64);
+        a += ta;
+        b += tb;
+        c += tc;
+        d += td;
+    }
 }
 
 __device__ __forceinline__ int decide64(double ar, double ai, double br, double bi, double th)
@@ -587,43 +607,45 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
 
             // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
             // Each lane sums its two rows' FP64 row DFTs, turned by the (ky, kz)
-            // phase of the bin; a wave reduction gives the exact bin.
+            // phase of the bin; a wave reduction gives the exact bin.  The
+            // plane's x is a compile-time constant, so the row DFT's
+            // twiddles are immediates.
             if (__any(any_flag)) {
                 const double black = (double)g.black;
-#pragma unroll 1
-                for (int wi = 0; wi < 24; ++wi) {
+                auto plane = [&](auto xc, int z) {
+                    constexpr int X = decltype(xc)::value, XR = 8 - X;
+                    const int wi = (X - 2) * 8 + z;
                     uint32_t bits = L.flg[wi];
                     if (wi == 17 || wi == 21) bits &= ~kSelf;
-                    const int x = 2 + (wi >> 3), z = wi & 7;
-                    const int zr = (10 - z) & 7, xr = 8 - x;
+                    const int zr = (10 - z) & 7;
                     while (bits != 0u) {
                         const int y = __builtin_ctz(bits);
                         bits &= bits - 1u;
                         const int yr = (40 - y) & 31;
-                        const int v = (z * 32 + y) * 3 + (x - 2);
                         double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
 #pragma unroll 1
                         for (int h = 0; h < 2; ++h) {
-                            // FP64 row DFT at bins x and xr = 8 - x from the samples
-                            // (phases uniform across the wave).  With c, s the
-                            // cos/sin of 2 pi x n / 16, bin x is sum u (c - i s) and
-                            // bin 8-x is sum (-1)^n u (c + i s): split by n parity.
-                            // A row outside the tile's lines holds black: u = 0.
-                            if (rowp[h] == nullptr) continue;
+                            // FP64 row DFT at bins X and 8 - X from the samples.
+                            // With c, s the cos/sin of 2 pi X n / 16, bin X is
+                            // sum u (c - i s) and bin 8-X is sum (-1)^n u (c + i s):
+                            // split by n parity.  A row outside the tile's lines
+                            // holds black: u = 0.
                             const uint4 q0 = h ? r10 : r00, q1 = h ? r11 : r01;
                             const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-                            const double rw = c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]];
+                            double rw = rowp[h] != nullptr ? c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]] : 0.0;
+                            // opaque per bin: otherwise the 32 windowed samples
+                            // are hoisted out of the bin loop into registers
+                            asm volatile("" : "+v"(rw));
                             double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
 #pragma unroll
                             for (int n = 0; n < 16; n += 2) {
                                 const uint32_t w2 = raw[n >> 1];
                                 const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
                                 const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
-                                const int j0 = (2 * x * n) & 31, j1 = (2 * x * (n + 1)) & 31;
-                                ce = fma(u0, L.cs[j0], ce);
-                                se = fma(u0, L.sn[j0], se);
-                                co = fma(u1, L.cs[j1], co);
-                                so = fma(u1, L.sn[j1], so);
+                                ce = fma(u0, kCos16[(X * n) & 15], ce);
+                                se = fma(u0, kSin16[(X * n) & 15], se);
+                                co = fma(u1, kCos16[(X * (n + 1)) & 15], co);
+                                so = fma(u1, kSin16[(X * (n + 1)) & 15], so);
                             }
                             const double ra = ce + co, ia = -(se + so);
                             const double rb = ce - co, ib = se - so;
@@ -636,15 +658,18 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                             b0 += rb * cb + ib * sb;
                             b1 += ib * cb - rb * sb;
                         }
-                        a0 = wave_sum_d(a0);
-                        a1 = wave_sum_d(a1);
-                        b0 = wave_sum_d(b0);
-                        b1 = wave_sum_d(b1);
-                        if (lane == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
-                            L.keep[(x - 2) * 8 + z] |= 1u << y;
-                            L.keep[(xr - 2) * 8 + zr] |= 1u << yr;
+                        wave_sum4_d(a0, a1, b0, b1);
+                        if (lane == 0 && decide64(a0, a1, b0, b1, thr64[(z * 32 + y) * 3 + (X - 2)])) {
+                            L.keep[(X - 2) * 8 + z] |= 1u << y;
+                            L.keep[(XR - 2) * 8 + zr] |= 1u << yr;
                         }
                     }
+                };
+#pragma unroll 1
+                for (int z = 0; z < 8; ++z) {
+                    plane(std::integral_constant<int, 2>{}, z);
+                    plane(std::integral_constant<int, 3>{}, z);
+                    plane(std::integral_constant<int, 4>{}, z);
                 }
                 __syncthreads();
             }
