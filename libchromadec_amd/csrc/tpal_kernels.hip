@@ -457,29 +457,35 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
     };
     prefetch(((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx_begin);
 
+    // This lane's two field rows of a tile (16 samples each); a row outside
+    // the tile's lines reads as black (zero after black subtraction).  They
+    // are loaded during the previous tile's inverse, and serve the forward
+    // x transform and the FP64 re-decisions.
+    uint4 rows[2][2];
+    auto load_rows = [&](int txl) {
+        const int tX = g.avs - kXT / 2 + (kXT / 2) * txl;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            rows[h][0] = rows[h][1] = make_uint4(blk2, blk2, blk2, blk2);
+            if (rowp[h] != nullptr) {
+                rows[h][0] = gload_u4(rowp[h] + tX);
+                rows[h][1] = gload_u4(rowp[h] + tX + 8);
+            }
+        }
+    };
+    if constexpr (MODE != kTileApply) load_rows(tx_begin);
+
     for (int tx = tx_begin; tx < tx_end; ++tx) {
         const int tileX = g.avs - kXT / 2 + (kXT / 2) * tx;
 
         const int64_t tile = ((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx;
         if constexpr (MODE != kTileApply) {
             // ---- forward x: two rows per lane -> x bins 2..6 ----------------
-            // both rows' loads are issued before either is used; a row
-            // outside the tile's lines reads as black (zero after black
-            // subtraction)
-            uint4 raw[2][2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                raw[h][0] = raw[h][1] = make_uint4(blk2, blk2, blk2, blk2);
-                if (rowp[h] != nullptr) {
-                    raw[h][0] = gload_u4(rowp[h] + tileX);
-                    raw[h][1] = gload_u4(rowp[h] + tileX + 8);
-                }
-            }
             float l1 = 0.0f;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 float s[16];
-                unpack16(raw[h][0], raw[h][1], g.black, s);
+                unpack16(rows[h][0], rows[h][1], g.black, s);
                 c32 X[5];
                 l1 += xdft16(s, X) * roww[h];
                 const int m = (lane + 64 * h) & 15;
@@ -526,6 +532,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
             __syncthreads();
 
             if constexpr (MODE == kTileSpectra) {
+                if (tx + 1 < tx_end) load_rows(tx + 1);
                 for (int i = lane; i < 1280; i += 64) {
                     const c32 v = Sat(L, i >> 8, (i >> 3) & 31, i & 7);
                     spec[tile * 1280 + i] = make_float2(v.x, v.y);
@@ -539,16 +546,6 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
             // gathered by wave ballots into keep words (plane (x-2, z), bit y);
             // a kept visit also keeps its partner, whose word is the mirror of
             // the ballot.  Keep is the OR over visits, as in the reference.
-            // rows for a possible FP64 re-decision, fetched while the mask runs
-            uint4 r00 = make_uint4(0u, 0u, 0u, 0u), r01 = r00, r10 = r00, r11 = r00;
-            if (rowp[0] != nullptr) {
-                r00 = gload_u4(rowp[0] + tileX);
-                r01 = gload_u4(rowp[0] + tileX + 8);
-            }
-            if (rowp[1] != nullptr) {
-                r10 = gload_u4(rowp[1] + tileX);
-                r11 = gload_u4(rowp[1] + tileX + 8);
-            }
             {
                 // Step i (unrolled) decides plane (x, z) = (2 + i/4, i%4) on
                 // lanes 0-31 and plane (x, z + 4) on lanes 32-63, so the
@@ -630,7 +627,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                             // sum u (c - i s) and bin 8-X is sum (-1)^n u (c + i s):
                             // split by n parity.  A row outside the tile's lines
                             // holds black: u = 0.
-                            const uint4 q0 = h ? r10 : r00, q1 = h ? r11 : r01;
+                            const uint4 q0 = h ? rows[1][0] : rows[0][0], q1 = h ? rows[1][1] : rows[0][1];
                             const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
                             double rw = rowp[h] != nullptr ? c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]] : 0.0;
                             // opaque per bin: otherwise the 32 windowed samples
@@ -673,7 +670,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                 }
                 __syncthreads();
             }
-
+            // the next tile's rows, in flight through the inverse
+            if (tx + 1 < tx_end) load_rows(tx + 1);
         } else {
 #pragma unroll
             for (int j = 0; j < kSpecPerLane; ++j) {
