@@ -553,7 +553,10 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                 // immediate.  The mirror of plane z lies at z' = (10 - z) % 8,
                 // which is 4 above the lower half's for z < 3 and 4 below for
                 // z = 3.  The next step's bins are read while this one decides.
-                const bool uni = g.thr_uniform >= 0.0f;
+                // The uniform-threshold case (the default) is a loop of its
+                // own with no memory access but LDS: a load under a branch in
+                // every step would put a wait for all outstanding memory
+                // operations after each join.
                 const int yo = vy * kSpad + 4 * vhalf;
                 const int yrp = ((40 - vy) & 31) * kSpad + 4 * vhalf, yrm = yrp - 8 * vhalf;
                 const int tho = vy * 3 + 384 * vhalf;
@@ -562,28 +565,33 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                     const int zr = (10 - (i & 3)) & 7;
                     return (4 - (i >> 2)) * kSk + zr + ((i & 3) < 3 ? yrp : yrm);
                 };
-                c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
+                auto steps = [&](auto uc) {
+                    constexpr bool kUniform = decltype(uc)::value;
+                    c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
 #pragma unroll
-                for (int i = 0; i < 12; ++i) {
-                    c32 an = a, bn = b;
-                    if (i + 1 < 12) {
-                        an = L.u.S[off_a(i + 1)];
-                        bn = L.u.S[off_b(i + 1)];
+                    for (int i = 0; i < 12; ++i) {
+                        c32 an = a, bn = b;
+                        if (i + 1 < 12) {
+                            an = L.u.S[off_a(i + 1)];
+                            bn = L.u.S[off_b(i + 1)];
+                        }
+                        const int k = i >> 2, z = i & 3;
+                        const float th = kUniform ? g.thr_uniform : thr32[z * 96 + k + tho];
+                        const int d = decide32(a, b, err, th);
+                        const uint64_t kb = __ballot(d > 0);
+                        const uint64_t fb = __ballot(d < 0);
+                        if (vy == 0) {
+                            const int w = k * 8 + z + 4 * vhalf;
+                            L.bal[w] = vhalf ? (uint32_t)(kb >> 32) : (uint32_t)kb;
+                            L.flg[w] = vhalf ? (uint32_t)(fb >> 32) : (uint32_t)fb;
+                        }
+                        a = an;
+                        b = bn;
+                        __builtin_amdgcn_sched_barrier(0);
                     }
-                    const int k = i >> 2, z = i & 3;
-                    const float th = uni ? g.thr_uniform : thr32[z * 96 + k + tho];
-                    const int d = decide32(a, b, err, th);
-                    const uint64_t kb = __ballot(d > 0);
-                    const uint64_t fb = __ballot(d < 0);
-                    if (vy == 0) {
-                        const int w = k * 8 + z + 4 * vhalf;
-                        L.bal[w] = vhalf ? (uint32_t)(kb >> 32) : (uint32_t)kb;
-                        L.flg[w] = vhalf ? (uint32_t)(fb >> 32) : (uint32_t)fb;
-                    }
-                    a = an;
-                    b = bn;
-                    __builtin_amdgcn_sched_barrier(0);
-                }
+                };
+                if (g.thr_uniform >= 0.0f) steps(std::true_type{});
+                else steps(std::false_type{});
             }
             __syncthreads();
             // keep word (k, z) = own visits' ballot | ballot of the visit plane
