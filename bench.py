@@ -38,6 +38,7 @@ N_FRAMES = 64
 ALGO_IN = 2 * 1135 * 313 * 2            # composite bytes per frame
 ALGO_OUT_PS = 3 * 922 * 576 * 4         # yuv444ps planes per frame
 NOMINAL_GFLOP_PER_FRAME = 1.396         # 4329 tiles x 261.6 kFLOP + demod
+NOMINAL_GFLOP_PER_FRAME_2D = 0.372      # 4366 field tiles x 24.9 kFLOP + demod
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
@@ -128,7 +129,7 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true",
                     help="skip comparing the GPU frames with the oracle's in the CPU-baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the learned-mask and batched legs")
-    ap.add_argument("--extras", default="learned,batched,abi",
+    ap.add_argument("--extras", default="learned,batched,2d,abi",
                     help="comma-separated legs to run beside the headline: learned, batched, abi")
     ap.add_argument("--stress", type=int, default=0,
                     help="instead of the benchmark, run the dropout stress workload over this many frames "
@@ -242,6 +243,8 @@ def main() -> int:
         res["learned"]["per_frame_time_vs_plain"] = round(fps / res["learned"]["value"], 2)
     if "batched" in extras:
         res["batched"] = batched_leg(args.steps)
+    if "2d" in extras:
+        res["transform_2d"] = transform2d_leg(fields, args.steps)
     if "abi" in extras:
         res["abi_end_to_end"] = abi_leg(fields)
     if args.cpu_sample > 0:
@@ -284,6 +287,51 @@ def abi_leg(fields, threads: int = 8):
     return {"async_frames_per_s": round(ok / dt, 1), "threads": threads, "sync_frames_per_s": round(sync, 1),
             "note": "chd_decode_frames_async from a .tbc file, 6.4 MB of host planes per frame (PCIe-inclusive); "
                     "the sync figure is the chd_decode_frame loop, which also copies the planes into numpy"}
+
+
+def transform2d_leg(fields, steps):
+    """CHD_DEC_TRANSFORM_2D on the same 64 frames: per-field 16x32 tiles
+    (transform_pal_2d.cpp:107-287) and the same demodulator, fields resident
+    in HBM, timed like the headline leg."""
+    import torch
+    from libchromadec_amd import abi
+    dev = torch.device("cuda:0")
+    plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_2D, pixel_format=abi.CHD_PIXEL_YUV444PS,
+                                            max_batch_frames=N_FRAMES))
+    src = torch.from_numpy(fields.view(np.int16)).to(dev)
+    packed = torch.empty((fields.shape[0], plan.packed_field_elems), dtype=torch.int16, device=dev)
+    plan.pack(src.data_ptr(), fields.shape[0], packed.data_ptr())
+    out = torch.empty((N_FRAMES, plan.frame_bytes // 4), dtype=torch.float32, device=dev)
+    seqs = [(packed.data_ptr(), N_FRAMES, 0, N_FRAMES)]
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        plan.decode(seqs, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.decode(seqs, out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plan.set_timing(True)
+    tile_ms, demod_ms, n = 0.0, 0.0, 0
+    for _ in range(max(3, min(steps, 10))):
+        plan.decode(seqs, out.data_ptr(), stream)
+        a, b, nt, _ = plan.last_timing()
+        tile_ms += a
+        demod_ms += b
+        n += nt
+    plan.set_timing(False)
+    torch.cuda.synchronize()
+    tile_avg, demod_avg = tile_ms / n, demod_ms / n
+    algo = (ALGO_IN + ALGO_OUT_PS) * N_FRAMES
+    tflops = NOMINAL_GFLOP_PER_FRAME_2D * N_FRAMES / ((tile_avg + demod_avg) * 1e-3) / 1e3
+    return {"value": round(N_FRAMES * steps / dt, 1), "unit": "frames/s", "ms_per_step": round(1e3 * dt / steps, 4),
+            "tile_kernel": "tpal2d_tiles", "tile_avg_launch_ms": round(tile_avg, 4),
+            "demod_avg_launch_ms": round(demod_avg, 4),
+            "hbm_frac_tile_kernel": round(algo / (tile_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "valu": {"nominal_gflop_per_frame": NOMINAL_GFLOP_PER_FRAME_2D,
+                     "achieved_tflops_nominal": round(tflops, 2),
+                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}}
 
 
 def learned_leg(plain_plan, packed, fields, truth, tu, tv, out, seqs, stream, steps):

@@ -50,22 +50,6 @@ tpal::Tables make_tables()
         t.cos32_d[j] = std::cos(2.0 * M_PI * j / 32.0);
         t.sin32_d[j] = std::sin(2.0 * M_PI * j / 32.0);
     }
-    for (int k = 0; k < 5; ++k)
-        for (int n = 0; n < 16; ++n) {
-            const double a = 2.0 * M_PI * (k + 2) * n / 16.0;
-            t.dx_re[k][n] = (float)(t.wx_d[n] * std::cos(a));
-            t.dx_im[k][n] = (float)(-t.wx_d[n] * std::sin(a));
-            t.ix_re[k][n] = (float)(2.0 / 4096.0 * std::cos(a));
-            t.ix_im[k][n] = (float)(-2.0 / 4096.0 * std::sin(a));
-        }
-    for (int k = 0; k < 9; ++k)
-        for (int n = 0; n < 32; ++n) {
-            const double a = 2.0 * M_PI * (k + 4) * n / 32.0;
-            t.dx2_re[k][n] = (float)(t.wx2_d[n] * std::cos(a));
-            t.dx2_im[k][n] = (float)(-t.wx2_d[n] * std::sin(a));
-            t.ix2_re[k][n] = (float)(2.0 / 512.0 * std::cos(a));
-            t.ix2_im[k][n] = (float)(-2.0 / 512.0 * std::sin(a));
-        }
     return t;
 }
 

@@ -92,14 +92,9 @@ constexpr int kFiltStride = 8;
 
 // Constant tables of the tile transforms (device __constant__ copy c_tab).
 struct Tables {
-    // 3D: windowed forward DFT of 16 samples to x bins 2..6, and the inverse
-    // c2r kernel (2/4096) * exp(+i...) for the same bins.
-    float dx_re[5][16], dx_im[5][16];
-    float ix_re[5][16], ix_im[5][16];
+    // Window factors: 3D (z, y, x) and 2D (y, x).  The x DFTs use
+    // compile-time twiddles.
     float wz[8], wy[32], wx[16];
-    // 2D: bins 4..12 of a 32-sample window, inverse scaled by 2/512.
-    float dx2_re[9][32], dx2_im[9][32];
-    float ix2_re[9][32], ix2_im[9][32];
     float wy2[16], wx2[32];
     // FP64 recheck tables.
     double wz_d[8], wy_d[32], wx_d[16], wy2_d[16], wx2_d[32];

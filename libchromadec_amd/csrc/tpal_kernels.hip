@@ -817,11 +817,73 @@ __device__ __forceinline__ float group_sum16(float v)
     for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-__device__ __forceinline__ double group_sum16_d(double v)
+__device__ __forceinline__ void group_sum16_4d(double &a, double &b, double &c, double &d)
 {
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    for (int o = 8; o > 0; o >>= 1) {
+        const double ta = __shfl_xor(a, o, 64), tb = __shfl_xor(b, o, 64);
+        const double tc = __shfl_xor(c, o, 64), td = __shfl_xor(d, o, 64);
+        a += ta;
+        b += tb;
+        c += tc;
+        d += td;
+    }
+}
+
+// Windowed 32-sample real DFT to bins 4..12 (X[j] = bin j+4), folded by the
+// half-period symmetry: even bins see u[n] + u[n+16], odd bins u[n] - u[n+16]
+// (16 terms, compile-time twiddles).  Returns sum |u[n]|.
+__device__ __forceinline__ float xdft32(const float (&s)[32], c32 (&X)[9])
+{
+    float u[32], l1 = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 32; ++n) {
+        u[n] = s[n] * c_tab.wx2[n];
+        l1 += fabsf(u[n]);
+    }
+    float e[16], o[16];
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        e[n] = u[n] + u[n + 16];
+        o[n] = u[n] - u[n + 16];
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int k = j + 4;
+        float re = 0.0f, im = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const float v = (k & 1) ? o[n] : e[n];
+            re = cmac(re, v, cos32(k * n));
+            im = cmac(im, v, -sin32(k * n));
+        }
+        X[j] = c32{re, im};
+    }
+    return l1;
+}
+
+// c2r over bins 4..12 (C[j] = bin j+4, pre-scaled): out[n] = 2 Re sum C e^{+i..}.
+// Even bins repeat every 16 samples, odd bins change sign.
+__device__ __forceinline__ void xidft32(const c32 (&C)[9], float (&out)[32])
+{
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        float E = 0.0f, O = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int k = j + 4;
+            const float c = cos32(k * n), sn = sin32(k * n);
+            if (k & 1) {
+                O = cmac(O, C[j].x, c);
+                O = cmac(O, C[j].y, -sn);
+            } else {
+                E = cmac(E, C[j].x, c);
+                E = cmac(E, C[j].y, -sn);
+            }
+        }
+        out[n] = E + O;
+        out[n + 16] = E - O;
+    }
 }
 
 __global__ void __launch_bounds__(64)
@@ -881,20 +943,11 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
             for (int n = 0; n < 32; ++n) s[n] = 0.0f;
         }
         const float rw = c_tab.wy2[ln];
-        float l1 = 0.0f;
-#pragma unroll
-        for (int n = 0; n < 32; ++n) l1 += fabsf(s[n]) * c_tab.wx2[n];
+        c32 X[9];
+        const float l1 = xdft32(s, X);
         const float err = kErrK2 * kU * group_sum16(l1 * rw);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            float re = 0.0f, im = 0.0f;
-#pragma unroll
-            for (int n = 0; n < 32; ++n) {
-                re += s[n] * c_tab.dx2_re[k][n];
-                im += s[n] * c_tab.dx2_im[k][n];
-            }
-            L.A[q][k][ln] = c32{re * rw, im * rw};
-        }
+        for (int k = 0; k < 9; ++k) L.A[q][k][ln] = c32{X[k].x * rw, X[k].y * rw};
         if (ln == 0) L.nflag[q] = 0;
         __syncthreads();
 
@@ -941,20 +994,31 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
             const int yr = (24 - y) & 15, xr = 16 - x;
             double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
             if (mine && have) {
+                // this line's FP64 DFT at bins x and xr by Horner's rule in
+                // w = e^{-2 pi i x / 32}, then turned by the line's vertical
+                // phase e^{-2 pi i ky line / 16} (32ths: 2 ky line)
                 const double rwd = c_tab.wy2_d[ln];
-                for (int n = 0; n < 32; ++n) {
+                const double wa0 = c_tab.cos32_d[x], wa1 = -c_tab.sin32_d[x];
+                const double wb0 = c_tab.cos32_d[xr], wb1 = -c_tab.sin32_d[xr];
+                double pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0;
+#pragma unroll
+                for (int n = 31; n >= 0; --n) {
                     const double xv = (double)s[n] * rwd * c_tab.wx2_d[n];
-                    // phase index in 32ths: ky*line*2 (16-point) + kx*n (32-point)
-                    const int ja = (2 * y * ln + x * n) & 31;
-                    const int jb = (2 * yr * ln + xr * n) & 31;
-                    a0 += xv * c_tab.cos32_d[ja];
-                    a1 -= xv * c_tab.sin32_d[ja];
-                    b0 += xv * c_tab.cos32_d[jb];
-                    b1 -= xv * c_tab.sin32_d[jb];
+                    const double ta = pa0 * wa0 - pa1 * wa1 + xv, tb = pb0 * wb0 - pb1 * wb1 + xv;
+                    pa1 = pa0 * wa1 + pa1 * wa0;
+                    pb1 = pb0 * wb1 + pb1 * wb0;
+                    pa0 = ta;
+                    pb0 = tb;
                 }
+                const int ja = (2 * y * ln) & 31, jb = (2 * yr * ln) & 31;
+                const double ca = c_tab.cos32_d[ja], sa = c_tab.sin32_d[ja];
+                const double cb = c_tab.cos32_d[jb], sb = c_tab.sin32_d[jb];
+                a0 = pa0 * ca + pa1 * sa;
+                a1 = pa1 * ca - pa0 * sa;
+                b0 = pb0 * cb + pb1 * sb;
+                b1 = pb1 * cb - pb0 * sb;
             }
-            a0 = group_sum16_d(a0); a1 = group_sum16_d(a1);
-            b0 = group_sum16_d(b0); b1 = group_sum16_d(b1);
+            group_sum16_4d(a0, a1, b0, b1);
             if (mine && ln == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
                 L.keep[q][x - 4][y] = 1;
                 L.keep[q][xr - 4][yr] = 1;
@@ -975,16 +1039,14 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
 
         // ---- inverse x and overlap-add ----------------------------------------
         c32 C[9];
+        const float sc = 2.0f / 512.0f;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) C[k] = L.A[q][k][ln];
-        float out[32];
-#pragma unroll
-        for (int n = 0; n < 32; ++n) {
-            float a = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) a += C[k].x * c_tab.ix2_re[k][n] + C[k].y * c_tab.ix2_im[k][n];
-            out[n] = a;
+        for (int k = 0; k < 9; ++k) {
+            const c32 t = L.A[q][k][ln];
+            C[k] = c32{t.x * sc, t.y * sc};
         }
+        float out[32];
+        xidft32(C, out);
         if (live && tx >= tx_own0 && row_ok) {
             const int c0 = tileX - g.avs + kColOrigin;
             float *dst = orow + c0;
