@@ -169,9 +169,13 @@ __device__ __forceinline__ double wave_sum_d(double v)
 // (transform_pal_3d.cpp:313-327: discard when |a|^2 < |b|^2 t^2 or
 // |b|^2 < |a|^2 t^2).  Returns 1 keep, 0 discard, -1 when FP32 error `e` on
 // the bins could change the outcome.  |m_hat - m| <= 2|a|e + e^2, plus the
-// rounding of the products themselves.
+// rounding of the products themselves.  The callers' bound e is zero only
+// for a tile whose windowed samples are all zero (a 2D padding step past its
+// strip, or a tile of black): every bin is then exactly zero and the pair is
+// kept (0 < 0 is false), as an FP64 evaluation decides.
 __device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
 {
+    if (e == 0.0f) return 1;
     const float ma = a.x * a.x + a.y * a.y;
     const float mb = b.x * b.x + b.y * b.y;
     // |a| <= max(|a.x|,|a.y|) + (sqrt 2 - 1) min(|a.x|,|a.y|)
