@@ -42,6 +42,24 @@ def test_transform3d_full_frames(orc, patch_capture, fmt):
         _check(got, _oracle(orc, 8, fields, f, fmt), fmt)
 
 
+@pytest.mark.parametrize("kind", [7, 8])
+def test_black_capture_decodes_to_black(orc, tmp_path, kind):
+    """Every tile of an all-black capture has all-zero windowed samples: its
+    FP32 error bound is zero and each carrier-mirror pair is kept without an
+    FP64 re-decision (0 < 0 is false).  The planes are exactly black, as the
+    oracle's are."""
+    from libchromadec_amd import synth
+    fields = np.full((12, synth.FIELD_HEIGHT, synth.FIELD_WIDTH), synth.BLACK, dtype=np.uint16)
+    path = str(tmp_path / "black.tbc")
+    synth.write_tbc(path, fields)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444ps").commit()
+    for f in (0, 3, 5):
+        got = dec.decode(f)
+        assert not got.any(), f
+        _check(got, _oracle(orc, kind, fields, f, "yuv444ps"), "yuv444ps")
+
+
 @pytest.mark.parametrize("kind", [6, 8])
 def test_subcarrier_locked_second_field_shift(orc, patch_capture, tmp_path, kind):
     """Subcarrier-locked PAL: the second field of every frame is moved two
