@@ -145,7 +145,6 @@ struct Worker {
     uint16_t *h_stage = nullptr;        // pinned: 14 fields as read from the file
     uint16_t *d_stage = nullptr;        // their device copies, before packing into slots
     void *d_out = nullptr;
-    void *h_out = nullptr;              // pinned: one decoded frame on its way to the host
     int device = 0;
     uint64_t seen_revision = 0;         // dropout options the cached fields were corrected with
 
@@ -161,12 +160,45 @@ struct Worker {
         if (d_out) (void)hipFree(d_out);
         if (d_stage) (void)hipFree(d_stage);
         if (h_stage) (void)hipHostFree(h_stage);
-        if (h_out) (void)hipHostFree(h_out);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
 }  // namespace
+
+// Pinned frame buffers: at most kPoolKeep idle ones are kept.
+constexpr size_t kPoolKeep = 16;
+
+FramePool::~FramePool()
+{
+    for (void *p : free_list) (void)hipHostFree(p);
+}
+
+void *FramePool::get()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!free_list.empty()) {
+            void *p = free_list.back();
+            free_list.pop_back();
+            return p;
+        }
+    }
+    void *p = nullptr;
+    return hipHostMalloc(&p, bytes, 0) == hipSuccess ? p : nullptr;
+}
+
+void FramePool::put(void *p)
+{
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (free_list.size() < kPoolKeep) {
+            free_list.push_back(p);
+            return;
+        }
+    }
+    (void)hipHostFree(p);
+}
 
 struct chd_decoder {
     chd_video_t *video = nullptr;
@@ -176,6 +208,7 @@ struct chd_decoder {
     std::unordered_map<std::string, bool> boolean;
     std::unordered_map<std::string, std::string> str;
     bool committed = false;
+    std::shared_ptr<FramePool> framePool;   // pinned host frames (set at commit)
 
     // committed state
     chd_decoder_kind_t resolved = CHD_DEC_AUTO;
@@ -240,7 +273,6 @@ chd_status_t worker_init(chd_decoder_t *d, Worker &w, const char *who)
     const size_t stage_bytes = (size_t)14 * g.fw * g.fh * sizeof(uint16_t);
     if (e == hipSuccess) e = hipHostMalloc((void **)&w.h_stage, stage_bytes, 0);
     if (e == hipSuccess) e = hipMalloc((void **)&w.d_stage, stage_bytes);
-    if (e == hipSuccess) e = hipHostMalloc(&w.h_out, chd_tpal_frame_bytes(w.plan), 0);
     if (e == hipSuccess)
         e = hipMemsetD16Async((hipDeviceptr_t)w.d_slots, (unsigned short)d->vp.black16bIre,
                               (size_t)kCacheSlots * g.field_elems, w.stream);
@@ -485,17 +517,13 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
     const bool is_float = d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS ||
                           d->format == CHD_PIXEL_RGBS;
     const size_t bytes = n * (is_float ? sizeof(float) : sizeof(uint16_t));
-    // through the worker's pinned buffer: a single DMA at link rate
-    hipError_t e = hipMemcpyAsync(w.h_out, w.d_out, bytes, hipMemcpyDeviceToHost, w.stream);
+    // straight into a pinned frame buffer: a single DMA at link rate
+    frame->pool = d->framePool;
+    frame->data = d->framePool->get();
+    if (frame->data == nullptr) return fail(CHD_E_OOM, "chd_decode_frame: no pinned host memory for the frame");
+    hipError_t e = hipMemcpyAsync(frame->data, w.d_out, bytes, hipMemcpyDeviceToHost, w.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
     if (e != hipSuccess) return fail(CHD_E_INTERNAL, "chd_decode_frame: GPU decode failed: " + hip_err(e));
-    if (is_float) {
-        frame->f32.resize(n);
-        std::memcpy(frame->f32.data(), w.h_out, bytes);
-    } else {
-        frame->u16.resize(n);
-        std::memcpy(frame->u16.data(), w.h_out, bytes);
-    }
     frame->info.format = d->format;
     frame->info.width = d->outW;
     frame->info.height = d->outH;
@@ -795,6 +823,11 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
     d->lookAhead = (resolved == CHD_DEC_TRANSFORM_3D) ? 4 : 0;   // (ZTILE - 1 + 1) / 2
     d->threadCount = threads;
 
+    d->framePool = std::make_shared<FramePool>((size_t)d->planes * d->outW * d->outH *
+                                               (d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS ||
+                                                        d->format == CHD_PIXEL_RGBS
+                                                    ? sizeof(float)
+                                                    : sizeof(uint16_t)));
     // Build worker 0 now so a missing GPU fails here, not at first decode.
     Worker *w0 = get_worker(d, 0);
     chd_status_t st = worker_init(d, *w0, "chd_decoder_commit");
@@ -920,12 +953,12 @@ chd_status_t chd_frame_get_plane(const chd_frame_t *f, chd_plane_t p, const void
     case CHD_PIXEL_YUV444P16:
         if (p != CHD_PLANE_Y && p != CHD_PLANE_CB && p != CHD_PLANE_CR)
             return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: plane not valid for YUV444P16");
-        *out_data = f->u16.data() + ps * (size_t)p;
+        *out_data = f->u16() + ps * (size_t)p;
         *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(uint16_t);
         return CHD_OK;
     case CHD_PIXEL_GRAY16:
         if (p != CHD_PLANE_Y) return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: GRAY16 only has CHD_PLANE_Y");
-        *out_data = f->u16.data();
+        *out_data = f->u16();
         *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(uint16_t);
         return CHD_OK;
     case CHD_PIXEL_RGB48:
@@ -933,7 +966,7 @@ chd_status_t chd_frame_get_plane(const chd_frame_t *f, chd_plane_t p, const void
         // offsets 0/1/2 with the row pitch as stride (c_api_frame.cpp:161-179)
         if (p != CHD_PLANE_R && p != CHD_PLANE_G && p != CHD_PLANE_B)
             return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane: plane not valid for RGB48");
-        *out_data = f->u16.data() + (size_t)(p - CHD_PLANE_R);
+        *out_data = f->u16() + (size_t)(p - CHD_PLANE_R);
         *out_stride = (ptrdiff_t)f->width * 3 * (ptrdiff_t)sizeof(uint16_t);
         return CHD_OK;
     case CHD_PIXEL_YUV444PS:
@@ -964,7 +997,7 @@ chd_status_t chd_frame_get_plane_float(const chd_frame_t *f, chd_plane_t p, cons
             return fail(CHD_E_INVALID_ARG, "chd_frame_get_plane_float: Y'CbCr float frame plane must be Y, Cb, or Cr");
         idx = (size_t)p;
     }
-    *out_data = f->f32.data() + (size_t)f->width * f->height * idx;
+    *out_data = f->f32() + (size_t)f->width * f->height * idx;
     *out_stride = (ptrdiff_t)f->width * (ptrdiff_t)sizeof(float);
     return CHD_OK;
 }

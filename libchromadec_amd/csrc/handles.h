@@ -80,27 +80,33 @@ struct chd_cancel {
     std::atomic<bool> requested{false};
 };
 
-// Allocator whose resize() leaves new elements uninitialised: frame storage is
-// written in full by the download right after it is sized.
-template <typename T>
-struct UninitAlloc : std::allocator<T> {
-    template <typename U>
-    struct rebind {
-        using other = UninitAlloc<U>;
-    };
-    UninitAlloc() = default;
-    template <typename U>
-    UninitAlloc(const UninitAlloc<U> &) noexcept {}
-    template <typename U>
-    void construct(U *p) noexcept { ::new (static_cast<void *>(p)) U; }
-    template <typename U, typename... A>
-    void construct(U *p, A &&...a) { ::new (static_cast<void *>(p)) U(std::forward<A>(a)...); }
+// Pinned host buffers of one frame size, shared by a decoder and the frames
+// it returns (which may outlive it).  A decoded frame is downloaded straight
+// into one; chd_frame_free returns it.  Reuse avoids a fresh 6 MB mapping,
+// its page faults and a bounce copy per frame.
+struct FramePool {
+    explicit FramePool(size_t b) : bytes(b) {}
+    ~FramePool();
+    void *get();              // nullptr when pinned memory cannot be had
+    void put(void *p);
+    const size_t bytes;
+    std::mutex mu;
+    std::vector<void *> free_list;
 };
 
 struct chd_frame {
     chd_frame_info_t info{};
     chd_pixel_format_t format = CHD_PIXEL_YUV444P16;
     int32_t width = 0, height = 0, planes = 0;
-    std::vector<float, UninitAlloc<float>> f32;        // float formats: planes x height x width
-    std::vector<uint16_t, UninitAlloc<uint16_t>> u16;  // integer formats
+    std::shared_ptr<FramePool> pool;
+    void *data = nullptr;     // planes x height x width, float or uint16 by format
+    chd_frame() = default;
+    chd_frame(const chd_frame &) = delete;
+    chd_frame &operator=(const chd_frame &) = delete;
+    ~chd_frame()
+    {
+        if (data != nullptr) pool->put(data);
+    }
+    const float *f32() const { return static_cast<const float *>(data); }
+    const uint16_t *u16() const { return static_cast<const uint16_t *>(data); }
 };
