@@ -424,6 +424,14 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
     // vertical-stage roles: forward lanes (z, k) = lane / 5, inverse lanes
     // (zs, k) = lane / 5 with z = 0,1,4,5
     const bool fy_odd = par[min(lane / 5, 7)] != 0;
+    // Line parity alternating with z (consecutive fields of a normal
+    // capture): row ky+16 of plane z is (-1)^par[z] = sgn (-1)^z times row ky,
+    // so after the temporal DFT T[ky+16][kz] = sgn T[ky][(kz+4) & 7] and only
+    // rows 0..15 need the vertical stores and the temporal DFT.
+    bool alt = true;
+#pragma unroll
+    for (int z = 1; z < 8; ++z) alt = alt && par[z] == (par[0] ^ (z & 1));
+    const float sgn = par[0] ? -1.0f : 1.0f;
     const int izs = min(lane / 5, 3);
     const bool iy_odd = par[(izs < 2) ? izs : izs + 2] != 0;
     // mask-stage role: bins y = lane & 31; lanes 0-31 visit x = 2 (all z) and
@@ -517,21 +525,36 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                         c32 x0 = v[ky];
                         if (odd) x0 = twiddle<-1>(x0, ky);
                         Sat(L, k, ky, z) = x0;
-                        Sat(L, k, ky + 16, z) = odd ? c32{-x0.x, -x0.y} : x0;
+                        if (!alt) Sat(L, k, ky + 16, z) = odd ? c32{-x0.x, -x0.y} : x0;
                     }
                 }
             }
             __syncthreads();
 
             // ---- temporal: 8-point DFT per (k, ky) ----------------------------
-            for (int c = lane; c < 160; c += 64) {
-                const int k = c >> 5, ky = c & 31;
-                c32 v[8];
+            if (alt) {
+                for (int c = lane; c < 80; c += 64) {
+                    const int k = c >> 4, ky = c & 15;
+                    c32 v[8];
 #pragma unroll
-                for (int z = 0; z < 8; ++z) v[z] = Sat(L, k, ky, z);
-                fft<8, -1>(v);
+                    for (int z = 0; z < 8; ++z) v[z] = Sat(L, k, ky, z);
+                    fft<8, -1>(v);
 #pragma unroll
-                for (int z = 0; z < 8; ++z) Sat(L, k, ky, z) = v[z];
+                    for (int z = 0; z < 8; ++z) {
+                        Sat(L, k, ky, z) = v[z];
+                        Sat(L, k, ky + 16, z) = c32{sgn * v[(z + 4) & 7].x, sgn * v[(z + 4) & 7].y};
+                    }
+                }
+            } else {
+                for (int c = lane; c < 160; c += 64) {
+                    const int k = c >> 5, ky = c & 31;
+                    c32 v[8];
+#pragma unroll
+                    for (int z = 0; z < 8; ++z) v[z] = Sat(L, k, ky, z);
+                    fft<8, -1>(v);
+#pragma unroll
+                    for (int z = 0; z < 8; ++z) Sat(L, k, ky, z) = v[z];
+                }
             }
             __syncthreads();
 
