@@ -267,6 +267,24 @@ def test_out_of_range_and_reuse(patch_capture):
         dec.set_option("transform_threshold", 0.3)
 
 
+def test_frame_outlives_its_decoder(patch_capture):
+    """A frame's planes stay valid after chd_decoder_free (frames are
+    caller-owned; their pinned buffers return to the pool on chd_frame_free)."""
+    import ctypes as C
+    path, _ = patch_capture
+    v = abi.Video(path)
+    want = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps").commit().decode(2)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps").commit()
+    f = C.c_void_p()
+    abi.check(abi.lib().chd_decode_frame(dec.h, 2, C.byref(f)), "chd_decode_frame")
+    dec.close()
+    fr = abi.Frame(f)
+    try:
+        assert np.array_equal(fr.planes(), want)
+    finally:
+        fr.free()
+
+
 @pytest.mark.parametrize("kind", [6, 7, 8])
 def test_async_matches_sync(patch_capture, kind):
     path, _ = patch_capture
