@@ -107,10 +107,17 @@ def cpu_baseline(fields: np.ndarray, n_sample: int, gpu_frames=None):
     t0 = time.perf_counter()
     want = orc.decode_sequence(p, 8, fields, frames, orc.uniform_thresholds(8), 0, threads=threads)
     dt = time.perf_counter() - t0
+    # one core as well (the reference's thread_count=1 decode loop), on 2 frames
+    n1 = min(2, n_sample)
+    t1 = time.perf_counter()
+    orc.decode_sequence(p, 8, fields, list(range(n1)), orc.uniform_thresholds(8), 0, threads=1)
+    dt1 = time.perf_counter() - t1
     res = {"value": n_sample / dt, "unit": "frames/s", "cores": threads, "kind": "port",
            "sample": f"{n_sample} frames of the same 64-frame sequence, FP64 oracle "
                      f"(reference-equivalent TransformPal3D+PalColour+OutputWriter, radix-2 FFT), "
-                     f"one frame per OpenMP thread, {dt:.1f} s wall"}
+                     f"one frame per OpenMP thread, {dt:.1f} s wall",
+           "one_core": {"value": n1 / dt1, "unit": "frames/s", "cores": 1,
+                        "sample": f"frames 0-{n1 - 1} on one thread, {dt1:.1f} s wall"}}
     parity = None
     if gpu_frames is not None:
         d = gpu_frames.astype(np.float64) - want.astype(np.float64)
