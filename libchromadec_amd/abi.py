@@ -205,8 +205,7 @@ def lib():
         if _lib is None:
             _share_hip_runtime_with_torch()
             if not os.path.exists(LIB_PATH):
-                raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
-                                   "(or make -C libchromadec_amd)")
+                raise RuntimeError(f"{LIB_PATH} is not built; run make -C libchromadec_amd")
             L = C.CDLL(LIB_PATH)
             for name, (res, args) in SIGNATURES.items():
                 fn = getattr(L, name)

@@ -1,6 +1,6 @@
 """GPU parity: chd_decode_frame (HIP kernels) against the FP64 oracle.
 
-Tolerances (BASELINE north star): float planes within 1e-4 of full scale
+Tolerances: float planes within 1e-4 of full scale
 (E'Y 0..1, E'Cb/E'Cr +-0.5), yuv444p16 within 1 LSB.  Every call goes
 through the C ABI in libchromadec_amd/lib/libchromadec.so.
 """
