@@ -358,6 +358,10 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hi
     if (plan->timing) hipEventRecord(plan->ev[1], s);
     tpal::launch_demod(g, plan->d_frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
                        plan->n_lblocks, s, plan_nchroma(plan));
+    for (int i = 0; i < n_frames; ++i) {
+        const tpal::FrameJob &fj = plan->h_frame_jobs[i];
+        if (fj.offset[0] == fj.offset[1]) tpal::launch_fill_other_parity(g, fj.out, fj.offset[0] ^ 1, s);
+    }
     if (plan->timing) hipEventRecord(plan->ev[2], s);
     e = hipGetLastError();
     if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: kernel launch", e));

@@ -124,6 +124,9 @@ void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
                  hipStream_t s, int shift = 0);
 void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s);
+// the active lines of frame-line parity `parity` of one frame, as the zero
+// Y/U/V no field was decoded into (frames whose fields share one offset)
+void launch_fill_other_parity(const Geom &g, void *out, int parity, hipStream_t s);
 void launch_chroma_to_planes(const Geom &g, const float *src, float *dst, int64_t plane_stride, int n_fields,
                              hipStream_t s);
 

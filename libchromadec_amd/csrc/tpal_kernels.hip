@@ -1200,6 +1200,9 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
     const int job_id = rem >> 1;
     const FrameJob &job = jobs[job_id];
     const int off = job.offset[fsel];
+    // two fields on the same lines: the second one's decode replaces the
+    // first's (palcolour.cpp:295-296, 616)
+    if (fsel == 0 && job.offset[1] == off) return;
     const uint16_t *comp = job.field[fsel];
     const int firstLine = (g.first + 1 - off) / 2;
     const int lastLine = ((g.last - off) / 2) + 1;
@@ -1383,6 +1386,9 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
     const int job_id = rem >> 1;
     const FrameJob &job = jobs[job_id];
     const int off = job.offset[fsel];
+    // two fields on the same lines: the second one's decode replaces the
+    // first's (palcolour.cpp:295-296, 616)
+    if (fsel == 0 && job.offset[1] == off) return;
     const uint16_t *comp = job.field[fsel];
     const int firstLine = (g.first + 1 - off) / 2;
     const int lastLine = ((g.last - off) / 2) + 1;
@@ -1548,6 +1554,9 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     const int fsel = (blockIdx.x / n_lblocks) & 1;
     const FrameJob &job = jobs[blockIdx.x / (2 * n_lblocks)];
     const int off = job.offset[fsel];
+    // two fields on the same lines: the second one's decode replaces the
+    // first's (palcolour.cpp:295-296, 616)
+    if (fsel == 0 && job.offset[1] == off) return;
     const uint16_t *comp = job.field[fsel];
     const int firstLine = (g.first + 1 - off) / 2;
     const int lastLine = ((g.last - off) / 2) + 1;
@@ -1858,6 +1867,22 @@ __global__ void fill_border(const Geom g, void *out, int64_t n_frames)
     }
 }
 
+// A frame whose two fields share one line offset leaves the active lines of
+// the other parity undecoded: they hold the zero Y/U/V the component frame
+// starts with (component_frame.cpp:47-52), converted like any pixel.
+__global__ void fill_other_parity(const Geom g, void *out, int parity)
+{
+    const OutSpec os = out_spec(g);
+    const int rows = (g.last - g.first + 2) / 2;  // >= the lines of one parity
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)rows * g.aw;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / g.aw), c = (int)(i - (int64_t)r * g.aw);
+        const int line = g.first + ((g.first ^ parity) & 1) + 2 * r;
+        if (line > g.last) continue;
+        store_pixel(g, os, out, (int64_t)(line - g.first + g.pad_top) * g.out_w + g.pad_left + c, 0.0f, 0.0f, 0.0f);
+    }
+}
+
 }  // namespace tpal
 
 // Launch wrappers (C++ linkage, used by tpal_device.cpp).
@@ -1949,6 +1974,11 @@ void launch_chroma_to_planes(const Geom &g, const float *src, float *dst, int64_
 void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s)
 {
     if (n_frames > 0) hipLaunchKernelGGL(fill_border, dim3(1024), dim3(256), 0, s, g, out, n_frames);
+}
+
+void launch_fill_other_parity(const Geom &g, void *out, int parity, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_other_parity, dim3(512), dim3(256), 0, s, g, out, parity & 1);
 }
 
 }  // namespace tpal

@@ -89,6 +89,41 @@ def test_subcarrier_locked_second_field_shift(orc, patch_capture, tmp_path, kind
 
 
 @pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
+@pytest.mark.parametrize("kind", [6, 7, 8])
+def test_second_field_marked_first(orc, patch_capture, tmp_path, kind, fmt):
+    """A second field whose metadata says isFirstField sits on the first
+    field's lines (SourceField::getOffset, source_field.h:77-79), so the
+    windows holding it no longer alternate line parity field by field and
+    the 3D tiles take the full vertical and temporal transform.  Its frame
+    carries the second field's lines only; the other lines are the zero
+    Y/U/V the component frame starts with (component_frame.cpp:47-52)."""
+    import json
+    from libchromadec_amd import synth
+    _, fields = patch_capture
+    path = str(tmp_path / "order.tbc")
+    side = synth.write_tbc(path, fields)
+    with open(side) as fh:
+        meta = json.load(fh)
+    meta["fields"][3]["isFirstField"] = True   # second field of frame 1
+    with open(side, "w") as fh:
+        json.dump(meta, fh)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format=fmt).commit()
+    p = orc.pal_params()
+    for f in (0, 1, 3):
+        win, offs, s, e = orc.frame_window(fields, f, kind, p.black16b)
+        offs = [0 if 2 * (f - (2 if kind == 8 else 0)) + w == 3 else o for w, o in enumerate(offs)]
+        assert (offs[s:e] == [0, 0]) == (f == 1)
+        want = orc.decode_window(p, kind, win, offs, s, e, orc.uniform_thresholds(kind), 0 if fmt == "yuv444ps" else 1)
+        got = dec.decode(f)
+        _check(got, want, fmt)
+        if f == 1:
+            # the offset matters: the regular order decodes differently
+            reg = _oracle(orc, kind, fields, f, fmt).astype(np.float64)
+            assert np.abs(got.astype(np.float64) - reg).max() > (1e-2 if fmt == "yuv444ps" else 100)
+
+
+@pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
 @pytest.mark.parametrize("capture", ["bars_capture", "patch_capture"])
 def test_pal2d_full_frames(orc, request, capture, fmt):
     """CHD_DEC_PAL_2D: the PALcolour 2D filter on the raw composite
