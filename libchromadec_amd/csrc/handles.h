@@ -53,7 +53,7 @@ struct Metadata {
     int32_t numberOfFields() const { return (int32_t)fields.size(); }
     const FieldMeta &field(int32_t number) const;          // 1-based; blank record if out of range
     int32_t numberOfFrames(bool firstFieldFirst) const;     // core.cpp:1092-1107
-    int32_t fieldNumber(int32_t frame, bool second, bool firstFieldFirst) const;  // core.cpp:1117-1170
+    int32_t fieldNumber(int32_t frame, bool second, bool firstFieldFirst) const;  // core.cpp:1117-1179
 };
 
 // Reads fields of a headerless little-endian uint16 .tbc file.

@@ -61,12 +61,21 @@ int32_t Metadata::fieldNumber(int32_t frame, bool second, bool firstFieldFirst) 
         firstNo++;
         secondNo++;
         if (firstNo > numberOfFields() || secondNo > numberOfFields()) {
+            log_message(CHD_LOG_ERROR, "frame " + std::to_string(frame) + ": no field marked isFirstField before "
+                                       "the end of the metadata");
             firstNo = secondNo = -1;
             break;
         }
     }
-    if (firstNo > numberOfFields() || secondNo > numberOfFields()) firstNo = secondNo = -1;
+    if (firstNo > numberOfFields() || secondNo > numberOfFields()) {
+        log_message(CHD_LOG_ERROR, "frame " + std::to_string(frame) + ": field number beyond the last field");
+        firstNo = secondNo = -1;
+    }
     if (firstNo == -1) return -1;
+    // both fields of the pair marked first: a broken capture, decoded as is
+    if (field(secondNo).isFirstField)
+        log_message(CHD_LOG_WARN, "frame " + std::to_string(frame) + ": fields " + std::to_string(firstNo) + " and " +
+                                      std::to_string(secondNo) + " are both marked isFirstField");
     return second ? secondNo : firstNo;
 }
 

@@ -242,3 +242,36 @@ def test_device_plan_without_gpu_fails_loudly():
     with pytest.raises(abi.ChdError) as e:
         abi.TpalPlan(bad)
     assert e.value.name == "CHD_E_DECODER_UNKNOWN"
+
+
+def test_both_fields_marked_first_warns(tmp_path, bars_capture):
+    """A frame whose second field is also marked isFirstField is paired as
+    is, with a warning (LdDecodeMetaData::getFieldNumber, core.cpp:1173-1176);
+    the frame count is unchanged (core.cpp:1092-1106)."""
+    import json
+    from libchromadec_amd import synth
+    _, fields = bars_capture
+    path = str(tmp_path / "order.tbc")
+    side = synth.write_tbc(path, fields)
+    with open(side) as fh:
+        meta = json.load(fh)
+    meta["fields"][3]["isFirstField"] = True
+    with open(side, "w") as fh:
+        json.dump(meta, fh)
+    L = abi.lib()
+    got = []
+    CB = C.CFUNCTYPE(None, C.c_int32, C.c_uint, C.c_char_p, C.c_void_p)
+    cb = CB(lambda lvl, flags, msg, user: got.append((lvl, msg.decode())))
+    L.chd_set_log_callback(C.cast(cb, C.c_void_p), None)
+    try:
+        v = abi.Video(path)
+        assert v.info().num_frames == fields.shape[0] // 2
+        a, b = fields[0].copy(), fields[1].copy()
+        v.correct_dropouts(0, a, b)
+        assert not any("isFirstField" in m for _, m in got)
+        a, b = fields[2].copy(), fields[3].copy()
+        v.correct_dropouts(1, a, b)
+    finally:
+        L.chd_set_log_callback(None, None)
+    warn = [m for lvl, m in got if lvl == 2 and "isFirstField" in m]
+    assert warn and "fields 3 and 4" in warn[0], got
