@@ -49,6 +49,8 @@ typedef struct chd_tpal_params {
     int32_t device;                      /* HIP device ordinal */
     double  kr, kb;                      /* luma coefficients of the colour-difference
                                             precision (R'G'B' output); 0 = 0.299, 0.114 */
+    double  luma_nr_level;               /* luma coring in IRE (PalColour yNRLevel,
+                                            palcolour.cpp:411-448); 0 = off */
 } chd_tpal_params_t;
 
 /* Build a plan: constant tables, thresholds and a workspace sized for

@@ -80,7 +80,7 @@ class TpalParams(C.Structure):
                 ("out_width", C.c_int32), ("out_height", C.c_int32),
                 ("pad_left", C.c_int32), ("pad_top", C.c_int32),
                 ("max_batch_frames", C.c_int32), ("device", C.c_int32),
-                ("kr", C.c_double), ("kb", C.c_double)]
+                ("kr", C.c_double), ("kb", C.c_double), ("luma_nr_level", C.c_double)]
 
 
 class TpalSequence(C.Structure):
@@ -408,12 +408,13 @@ def color_scales(color_difference="modern", broadcast_scaling="scientific"):
 
 def pal_tpal_params(kind=CHD_DEC_TRANSFORM_3D, pixel_format=CHD_PIXEL_YUV444PS, threshold=0.4,
                     max_batch_frames=64, first_line=44, last_line=619, avs=185, ave=1107, device=0,
-                    black=0x4000, white=0xD300, clamp=0):
+                    black=0x4000, white=0xD300, clamp=0, luma_nr_level=0.0):
     """Device-plan parameters for the line-locked 4fSC PAL geometry."""
     ecb, ecr = color_scales()
     return TpalParams(kind, 1135, 313, avs, ave, first_line, last_line, 98, 138, black, white,
                       4433618.75, 17734475.0, threshold, None, 1.0, 0.0, ecb, ecr, pixel_format, clamp,
-                      ave - avs, last_line - first_line + 1, 0, 0, max_batch_frames, device, 0.299, 0.114)
+                      ave - avs, last_line - first_line + 1, 0, 0, max_batch_frames, device, 0.299, 0.114,
+                      luma_nr_level)
 
 
 class TpalPlan:

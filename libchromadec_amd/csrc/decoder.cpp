@@ -739,7 +739,6 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
 
     double luma_nr = 0.0;
     if (auto it = d->f64.find("luma_nr_level"); it != d->f64.end()) luma_nr = it->second;
-    if (luma_nr > 0.0) return fail(CHD_E_UNSUPPORTED, fn + ": luma_nr_level > 0 is not supported by this build");
     std::string mask = "threshold";
     if (auto it = d->str.find("transform_mask"); it != d->str.end()) mask = it->second;
     if (mask != "threshold" && mask != "learned")
@@ -817,6 +816,7 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
     p.device = 0;
     p.kr = kr;
     p.kb = kb;
+    p.luma_nr_level = luma_nr;
 
     d->resolved = resolved;
     d->lookBehind = (resolved == CHD_DEC_TRANSFORM_3D) ? 2 : 0;  // (HALFZTILE + 1) / 2

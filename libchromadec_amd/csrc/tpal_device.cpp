@@ -154,6 +154,12 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     const double theta = (p->chroma_phase_deg * M_PI) / 180.0;
     g.chroma_gain_cos = (float)(std::cos(theta) * p->chroma_gain);
     g.chroma_gain_sin = (float)(std::sin(theta) * p->chroma_gain);
+    // PalColour::doYNR: nr_y = yNRLevel * irescale, irescale the int32
+    // quotient (white - black) / 100 (palcolour.cpp:414-416); off unless > 0
+    g.ynr = 0.0f;
+    if (p->luma_nr_level > 0.0)
+        g.ynr = (float)(p->luma_nr_level * (double)((p->white_16b_ire - p->black_16b_ire) / 100));
+    if (!(g.ynr > 0.0f)) g.ynr = 0.0f;
     {
         // Y'UV -> R'G'B' (color_conversion.cpp:33-67): the composite U/V
         // reductions follow from ecbFromU = 1 / (2 (1 - kb) uReduction)
@@ -291,6 +297,8 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     ok = ok && dmalloc((void **)&plan->d_slab_jobs, plan->max_units * sizeof(tpal::SlabJob));
     ok = ok && dmalloc((void **)&plan->d_field_jobs, plan->max_units * sizeof(tpal::FieldJob2D));
     ok = ok && dmalloc((void **)&plan->d_frame_jobs, plan->max_frames * sizeof(tpal::FrameJob));
+    if (g.ynr > 0.0f)
+        ok = ok && dmalloc((void **)&plan->d_raw, plan->max_frames * tpal::raw_frame_floats_host(g) * sizeof(float));
     ok = ok && hipHostMalloc((void **)&plan->h_slab_jobs, plan->max_units * sizeof(tpal::SlabJob), 0) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&plan->h_field_jobs, plan->max_units * sizeof(tpal::FieldJob2D), 0) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&plan->h_frame_jobs, plan->max_frames * sizeof(tpal::FrameJob), 0) == hipSuccess;
@@ -356,8 +364,9 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hi
     else if (plan->kind == CHD_DEC_TRANSFORM_2D)
         tpal::launch_tpal2d(g, plan->d_field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
     if (plan->timing) hipEventRecord(plan->ev[1], s);
-    tpal::launch_demod(g, plan->d_frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
-                       plan->n_lblocks, s, plan_nchroma(plan));
+    e = tpal::launch_demod(g, plan->d_frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
+                           plan->n_lblocks, s, plan_nchroma(plan), plan->d_raw);
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: demod launch", e));
     for (int i = 0; i < n_frames; ++i) {
         const tpal::FrameJob &fj = plan->h_frame_jobs[i];
         if (fj.offset[0] == fj.offset[1]) tpal::launch_fill_other_parity(g, fj.out, fj.offset[0] ^ 1, s);
@@ -484,6 +493,7 @@ void chd_tpal_plan_free(chd_tpal_plan_t *plan)
     hipFree(plan->d_slab_jobs);
     hipFree(plan->d_field_jobs);
     hipFree(plan->d_frame_jobs);
+    if (plan->d_raw) hipFree(plan->d_raw);
     if (plan->d_spec) hipFree(plan->d_spec);
     if (plan->d_mask) hipFree(plan->d_mask);
     if (plan->h_slab_jobs) hipHostFree(plan->h_slab_jobs);
@@ -732,9 +742,9 @@ chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_
         if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: job upload", e));
         plan->staging_pending = true;
-        tpal::launch_demod(g, plan->d_frame_jobs, cnt, plan->d_sine, plan->d_cosine, plan->d_cfilt, plan->n_lblocks,
-                           s, 1);
-        e = hipGetLastError();
+        e = tpal::launch_demod(g, plan->d_frame_jobs, cnt, plan->d_sine, plan->d_cosine, plan->d_cfilt,
+                               plan->n_lblocks, s, 1, plan->d_raw);
+        if (e == hipSuccess) e = hipGetLastError();
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: launch", e));
     }
     return CHD_OK;
@@ -752,6 +762,7 @@ struct chd_tpal_batch {
     tpal::FieldJob2D *d_field = nullptr;
     tpal::FrameJob *d_frame = nullptr;
     float *d_partials = nullptr;
+    float *d_raw = nullptr;        // uncored Y/U/V of every frame, with luma coring on
     void *d_out = nullptr;
     bool border = false;
     int strip = tpal::kStrip3, groups = 0;
@@ -773,7 +784,7 @@ void batch_release(chd_tpal_batch *b)
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     if (b->cap) hipStreamDestroy(b->cap);
-    void *ptrs[] = {b->d_slab, b->d_field, b->d_frame, b->d_partials, b->d_spec, b->d_mask};
+    void *ptrs[] = {b->d_slab, b->d_field, b->d_frame, b->d_partials, b->d_spec, b->d_mask, b->d_raw};
     for (void *q : ptrs)
         if (q) hipFree(q);
 }
@@ -810,8 +821,9 @@ void batch_all(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
     chd_tpal_plan *plan = b->plan;
     if (b->border) tpal::launch_fill_border(plan->g, b->d_out, b->n_frames, s);
     batch_tiles(b, s, strip, groups);
-    tpal::launch_demod(plan->g, b->d_frame, b->n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
-                       plan->n_lblocks, s, plan_nchroma(plan));
+    // d_raw is allocated whenever coring is on, so this launch cannot refuse
+    (void)tpal::launch_demod(plan->g, b->d_frame, b->n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
+                             plan->n_lblocks, s, plan_nchroma(plan), b->d_raw);
 }
 
 hipError_t batch_capture(chd_tpal_batch *b)
@@ -926,6 +938,8 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
                 for (int k = 0; k < fj.nchroma; ++k) fj.chroma[fsel][k] = rebase(fj.chroma[fsel][k]);
         }
     }
+    if (e == hipSuccess && g.ynr > 0.0f)
+        e = hipMalloc((void **)&b->d_raw, (size_t)b->n_frames * tpal::raw_frame_floats_host(g) * sizeof(float));
     if (e == hipSuccess && is3d) e = hipMalloc((void **)&b->d_slab, slabs.size() * sizeof(tpal::SlabJob));
     if (e == hipSuccess && !fjobs.empty())
         e = hipMalloc((void **)&b->d_field, fjobs.size() * sizeof(tpal::FieldJob2D));

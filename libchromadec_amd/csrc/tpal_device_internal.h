@@ -24,6 +24,7 @@ struct chd_tpal_plan {
     tpal::SlabJob *d_slab_jobs = nullptr;
     tpal::FieldJob2D *d_field_jobs = nullptr;
     tpal::FrameJob *d_frame_jobs = nullptr;
+    float *d_raw = nullptr;     // uncored Y/U/V of a chunk, with luma coring on
     tpal::SlabJob *h_slab_jobs = nullptr;       // pinned staging
     tpal::FieldJob2D *h_field_jobs = nullptr;
     tpal::FrameJob *h_frame_jobs = nullptr;

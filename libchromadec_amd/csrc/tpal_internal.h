@@ -63,6 +63,8 @@ struct Geom {
     // box of the clamp mode
     float rgb[4];
     float rgb_lo[3], rgb_hi[3];
+    // luma coring level in composite units (PalColour::doYNR nr_y), 0 = off
+    float ynr;
 };
 
 struct SlabJob {
@@ -118,8 +120,11 @@ void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_m
                    const double *thr64, hipStream_t s);
 // nchroma: the FrameJob.nchroma every job of the launch has (4, 2, 1, or 0
 // for CHD_DEC_PAL_2D, which filters the composite)
-void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
-                  const float *cfilt, int n_lblocks, hipStream_t s, int nchroma);
+// raw: with luma coring on (g.ynr > 0), raw_frame_floats(g) floats per job
+// for the uncored planes (hipErrorInvalidValue without them); ignored otherwise.
+hipError_t launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
+                        const float *cfilt, int n_lblocks, hipStream_t s, int nchroma, float *raw);
+inline int64_t raw_frame_floats_host(const Geom &g) { return 3 * (int64_t)(g.last - g.first + 1) * g.aw; }
 // shift: samples dropped from the start of each field (black appended)
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
                  hipStream_t s, int shift = 0);

@@ -1185,11 +1185,34 @@ __device__ __forceinline__ void store_pixel(const Geom &g, const OutSpec &os, vo
     }
 }
 
+// Luma coring (PalColour::doYNR) needs the whole decoded E'Y line, so with
+// it on the demodulators leave raw Y/U/V (composite units) in `raw`: per
+// frame of a launch 3 planes of (last - first + 1) frame lines x aw samples.
+__device__ __forceinline__ int64_t raw_frame_floats(const Geom &g) { return 3 * (int64_t)(g.last - g.first + 1) * g.aw; }
+
+// Output pixel (frame_line, column c from active_video_start) of frame job
+// `job_id`: converted, or (RAW) left uncored for luma coring.
+template <bool RAW>
+__device__ __forceinline__ void emit_pixel(const Geom &g, const OutSpec &os, const FrameJob &job, float *raw,
+                                           int job_id, int frame_line, int c, float Y, float U, float V)
+{
+    if constexpr (RAW) {
+        const int64_t plane = (int64_t)(g.last - g.first + 1) * g.aw;
+        gfloat *r = (gfloat *)(raw + job_id * raw_frame_floats(g)) + (int64_t)(frame_line - g.first) * g.aw + c;
+        r[0] = Y;
+        r[plane] = U;
+        r[2 * plane] = V;
+    } else {
+        const int64_t o = (int64_t)(frame_line - g.first + g.pad_top) * g.out_w + g.pad_left + c;
+        store_pixel(g, os, job.out, o, Y, U, V);
+    }
+}
+
 // Grid: jobs x 2 fields x line blocks x column segments.
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
           const float *__restrict__ cosine, const float *__restrict__ cfilt /* [kMaxFilter+1][4] */,
-          int n_lblocks, int n_segs)
+          int n_lblocks, int n_segs, float *raw)
 {
     __shared__ LdsD L;
     const int tid = threadIdx.x;
@@ -1342,10 +1365,8 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
             const float Y = pal2d ? compv - (PY * sine[i] + QY * cosine[i]) * 2.0f : compv - L.chroma[r0][i - s0];
             const float U = -(PU * bp + QU * bq) * 2.0f;
             const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
-            const int frame_line = number * 2 + off;
-            const int orow = frame_line - g.first + g.pad_top;
-            const int64_t o = (int64_t)orow * g.out_w + g.pad_left + (i - g.avs);
-            store_pixel(g, os, job.out, o, Y, U, V);
+            if (raw != nullptr) emit_pixel<true>(g, os, job, raw, job_id, number * 2 + off, i - g.avs, Y, U, V);
+            else emit_pixel<false>(g, os, job, raw, job_id, number * 2 + off, i - g.avs, Y, U, V);
         }
         __syncthreads();
     }
@@ -1375,7 +1396,8 @@ struct LdsQ {
 
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
-               const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, int n_segs)
+               const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, int n_segs,
+               float *raw)
 {
     __shared__ LdsQ L;
     const int tid = threadIdx.x;
@@ -1505,10 +1527,8 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
             const float Y = compv - L.chroma[r0][ci];
             const float U = -(PU * bp + QU * bq) * 2.0f;
             const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
-            const int frame_line = number * 2 + off;
-            const int orow = frame_line - g.first + g.pad_top;
-            const int64_t o = (int64_t)orow * g.out_w + g.pad_left + (i - g.avs);
-            store_pixel(g, os, job.out, o, Y, U, V);
+            if (raw != nullptr) emit_pixel<true>(g, os, job, raw, job_id, number * 2 + off, i - g.avs, Y, U, V);
+            else emit_pixel<false>(g, os, job, raw, job_id, number * 2 + off, i - g.avs, Y, U, V);
         }
         __syncthreads();
     }
@@ -1543,16 +1563,17 @@ struct LdsQ4 {
 __device__ __forceinline__ float4 f4add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 __device__ __forceinline__ float f4get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
 
-template <int NC>
+template <int NC, bool RAW>
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
-                const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks)
+                const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, float *raw)
 {
     __shared__ LdsQ4 L;
     const int tid = threadIdx.x;
     const int lb = blockIdx.x % n_lblocks;
     const int fsel = (blockIdx.x / n_lblocks) & 1;
-    const FrameJob &job = jobs[blockIdx.x / (2 * n_lblocks)];
+    const int job_id = blockIdx.x / (2 * n_lblocks);
+    const FrameJob &job = jobs[job_id];
     const int off = job.offset[fsel];
     // two fields on the same lines: the second one's decode replaces the
     // first's (palcolour.cpp:295-296, 616)
@@ -1776,8 +1797,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 compv[2] = cb + (float)(cw.y & 0xFFFFu); compv[3] = cb + (float)(cw.y >> 16);
             }
             const int frame_line = number * 2 + off;
-            const int orow = frame_line - g.first + g.pad_top;
-            const int64_t obase = (int64_t)orow * g.out_w + g.pad_left + c0;
+            const int64_t obase = (int64_t)(frame_line - g.first + g.pad_top) * g.out_w + g.pad_left + c0;
             // taps with e + b of parity sel sit on sine columns (m terms: P01, Q23)
             const int ps = sel, pc = sel ^ 1;
 #pragma unroll
@@ -1791,7 +1811,10 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 else Y = compv[e] - f4get(r[3], e);
                 const float U = -(PU * bp + QU * bq) * 2.0f;
                 const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
-                if (c0 + e < g.aw) store_pixel(g, os, job.out, obase + e, Y, U, V);
+                if (c0 + e < g.aw) {
+                    if constexpr (RAW) emit_pixel<true>(g, os, job, raw, job_id, frame_line, c0 + e, Y, U, V);
+                    else store_pixel(g, os, job.out, obase + e, Y, U, V);
+                }
             }
         }
 #pragma unroll
@@ -1867,6 +1890,44 @@ __global__ void fill_border(const Geom g, void *out, int64_t n_frames)
     }
 }
 
+// Luma coring, PalColour::doYNR (palcolour.cpp:411-448): each decoded frame
+// line's E'Y minus its high-pass clipped to +-g.ynr.  The high-pass is the
+// 25-tap symmetric FIR f_nrpal (deemp.h:620-633) centred on the sample, over
+// the active samples with zeros outside them.  kNrTaps[j] is tap 12 +- j.
+constexpr float kNrTaps[13] = {
+    7.479077367478024e-01f, -2.223459379380252e-01f, -1.492080690537196e-01f, -6.398130386469833e-02f,
+    1.098965697652802e-03f, 3.029931283148555e-02f, 2.861211356202848e-02f, 1.309163063177155e-02f,
+    -4.423694440267179e-04f, -5.577680979937061e-03f, -4.499636864042073e-03f, -1.857019211291029e-03f,
+    1.141291975113614e-04f};
+
+// Grid: frame jobs x (last - first + 1) frame lines; raw as the demodulators
+// left it.  Lines of a parity no field was decoded into are skipped (the
+// two-fields-on-one-parity fill writes them).
+__global__ void __launch_bounds__(256) ynr_output(const Geom g, const FrameJob *__restrict__ jobs,
+                                                  const float *__restrict__ raw)
+{
+    const int nlines = g.last - g.first + 1;
+    const int r = blockIdx.x % nlines, job_id = blockIdx.x / nlines;
+    const FrameJob &job = jobs[job_id];
+    const int par = (g.first + r) & 1;
+    if (par != job.offset[0] && par != job.offset[1]) return;
+    const OutSpec os = out_spec(g);
+    const int64_t plane = (int64_t)nlines * g.aw;
+    const float *yl = raw + job_id * raw_frame_floats(g) + (int64_t)r * g.aw;
+    for (int c = threadIdx.x; c < g.aw; c += blockDim.x) {
+        float a = kNrTaps[0] * yl[c];
+#pragma unroll
+        for (int j = 1; j <= 12; ++j) {
+            const float lv = c - j >= 0 ? yl[c - j] : 0.0f;
+            const float rv = c + j < g.aw ? yl[c + j] : 0.0f;
+            a = fmaf(kNrTaps[j], lv + rv, a);
+        }
+        a = fminf(fmaxf(a, -g.ynr), g.ynr);
+        store_pixel(g, os, job.out, (int64_t)(r + g.pad_top) * g.out_w + g.pad_left + c, yl[c] - a, yl[plane + c],
+                    yl[2 * plane + c]);
+    }
+}
+
 // A frame whose two fields share one line offset leaves the active lines of
 // the other parity undecoded: they hold the zero Y/U/V the component frame
 // starts with (component_frame.cpp:47-52), converted like any pixel.
@@ -1929,8 +1990,8 @@ void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_m
         hipLaunchKernelGGL(tpal2d_tiles, dim3(blocks), dim3(64), 0, s, g, jobs, n_ty_max, thr32, thr64);
 }
 
-void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
-                  const float *cfilt, int n_lblocks, hipStream_t s, int nchroma)
+static void demod_kernels(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
+                          const float *cfilt, int n_lblocks, hipStream_t s, int nchroma, float *raw)
 {
     const bool composite = nchroma == 0;
     const int n_segs = (g.aw + kDemodThreads - 1) / kDemodThreads;
@@ -1938,9 +1999,12 @@ void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *
         const int nlb = (n_lblocks * kDemodLines + kQ4Lines - 1) / kQ4Lines;
         const int blocks = n_jobs * 2 * nlb;
         if (blocks > 0) {
-            auto kern = nchroma == 4 ? pal_demod_quad4<4> : nchroma == 2 ? pal_demod_quad4<2>
-                      : nchroma == 1 ? pal_demod_quad4<1> : pal_demod_quad4<0>;
-            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt, nlb);
+            auto kern = raw != nullptr
+                            ? (nchroma == 4 ? pal_demod_quad4<4, true> : nchroma == 2 ? pal_demod_quad4<2, true>
+                               : nchroma == 1 ? pal_demod_quad4<1, true> : pal_demod_quad4<0, true>)
+                            : (nchroma == 4 ? pal_demod_quad4<4, false> : nchroma == 2 ? pal_demod_quad4<2, false>
+                               : nchroma == 1 ? pal_demod_quad4<1, false> : pal_demod_quad4<0, false>);
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt, nlb, raw);
         }
         return;
     }
@@ -1949,13 +2013,24 @@ void launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *
         const int blocks = n_jobs * 2 * nlb * n_segs;
         if (blocks > 0)
             hipLaunchKernelGGL(pal_demod_quad, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt,
-                               nlb, n_segs);
+                               nlb, n_segs, raw);
         return;
     }
     const int blocks = n_jobs * 2 * n_lblocks * n_segs;
     if (blocks > 0)
         hipLaunchKernelGGL(pal_demod, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt,
-                           n_lblocks, n_segs);
+                           n_lblocks, n_segs, raw);
+}
+
+hipError_t launch_demod(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
+                        const float *cfilt, int n_lblocks, hipStream_t s, int nchroma, float *raw)
+{
+    if (g.ynr > 0.0f && raw == nullptr) return hipErrorInvalidValue;  // coring reads the raw planes
+    if (g.ynr <= 0.0f) raw = nullptr;
+    demod_kernels(g, jobs, n_jobs, sine, cosine, cfilt, n_lblocks, s, nchroma, raw);
+    if (raw != nullptr && n_jobs > 0)
+        hipLaunchKernelGGL(ynr_output, dim3(n_jobs * (g.last - g.first + 1)), dim3(256), 0, s, g, jobs, raw);
+    return hipSuccess;
 }
 
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,

@@ -19,6 +19,8 @@
  *   src/decoders/palcolour/transform_pal_3d.cpp:64-333  3D window/lattice/filter/OLA
  *   src/decoders/palcolour/transform_pal_2d.cpp:63-287  2D window/lattice/filter/OLA
  *   src/decoders/palcolour/palcolour.cpp:150-409,456-646 LUTs, burst, demod
+ *   src/decoders/palcolour/palcolour.cpp:411-448        luma noise coring (doYNR)
+ *   src/decoders/filter/deemp.h:620-633, iirfilter.h:39-98  its high-pass FIR
  *   src/decoders/source_field.h:77-91                   field offset / active lines
  *   src/output/component_frame.cpp:35-61                component frame layout
  *   src/output/output_writer.cpp:46-76,124-143,371-486,670-727  output conversion
@@ -50,6 +52,7 @@ typedef struct orc_params {
     int32_t clamp_mode; /* 0 none, 1 legal_rgb_sdr, 2 legal_rgb_hdr, 3 legal_ycbcr_bt601 */
     int32_t pad_left, pad_top, out_width, out_height;
     double kr, kb;      /* luma coefficients of the colour-difference precision */
+    double luma_nr_level; /* PalColour yNRLevel in IRE; 0 = off */
 } orc_params;
 
 /* ------------------------------------------------------------------------ */
@@ -558,6 +561,55 @@ void orc_detect_burst(const orc_params *p, int32_t number, const uint16_t *field
  * the line is decoded as decodeLine<uint16_t, false> (the PALcolour 2D
  * filter on the raw composite, palcolour.cpp:327-328, 525-609, 622-630):
  * luma subtracts the chroma re-synthesised from the Y filter's P/Q. */
+/* PalColour::doYNR (palcolour.cpp:411-448): analog-style luma coring of one
+ * frame line.  f_nrpal is an IIRFilter<25, 1> (deemp.h:620-633), a 25-tap
+ * FIR with a[0] = 1; it is fed zeros for the 12 samples before the active
+ * line, the active samples, then 12 zeros, and its output 12 samples later
+ * is the high-pass of sample h.  That value, clipped to +-nr_y, is
+ * subtracted.  nr_y = level * ((white - black) / 100), the division being
+ * that of the int32 IRE fields. */
+static const double c_nrpal_b[25] = {
+    1.141291975113614e-04, -1.857019211291029e-03, -4.499636864042073e-03, -5.577680979937061e-03,
+    -4.423694440267179e-04, 1.309163063177155e-02, 2.861211356202848e-02, 3.029931283148555e-02,
+    1.098965697652802e-03, -6.398130386469833e-02, -1.492080690537196e-01, -2.223459379380252e-01,
+    7.479077367478024e-01, -2.223459379380252e-01, -1.492080690537196e-01, -6.398130386469833e-02,
+    1.098965697652803e-03, 3.029931283148557e-02, 2.861211356202848e-02, 1.309163063177156e-02,
+    -4.423694440267185e-04, -5.577680979937061e-03, -4.499636864042074e-03, -1.857019211291030e-03,
+    1.141291975113614e-04};
+
+void orc_ynr_line(const orc_params *p, double *Yline)
+{
+    const double irescale = (double)((p->white16b - p->black16b) / 100);
+    const double nr_y = p->luma_nr_level * irescale;
+    const int delay = 25 / 2;
+    const int avs = p->active_video_start, ave = p->active_video_end;
+    double x[25] = {0};
+    double *hp = calloc((size_t)ave + delay, sizeof(double));
+    /* IIRFilter::feed with bOrder 25, aOrder 1 */
+#define ORC_FEED(v, dst)                                  \
+    do {                                                  \
+        double y0 = c_nrpal_b[0] * (v);                   \
+        for (int k = 24; k >= 1; k--) {                   \
+            x[k] = x[k - 1];                              \
+            y0 += c_nrpal_b[k] * x[k];                    \
+        }                                                 \
+        x[0] = (v);                                       \
+        dst = y0;                                         \
+    } while (0)
+    double sink;
+    for (int h = avs - delay; h < avs; h++) ORC_FEED(0.0, sink);
+    for (int h = avs; h < ave; h++) ORC_FEED(Yline[h], hp[h]);
+    for (int h = ave; h < ave + delay; h++) ORC_FEED(0.0, hp[h]);
+#undef ORC_FEED
+    (void)sink;
+    for (int h = avs; h < ave; h++) {
+        double a = hp[h + delay];
+        if (fabs(a) > nr_y) a = (a > 0) ? nr_y : -nr_y;
+        Yline[h] -= a;
+    }
+    free(hp);
+}
+
 static void decode_field(const orc_params *p, const pal_luts *L, const uint16_t *comp, int off,
                          const double *chroma, int pal2d, double *Yp, double *Up, double *Vp)
 {
@@ -621,6 +673,7 @@ static void decode_field(const orc_params *p, const pal_luts *L, const uint16_t 
             oU[i] = -(PU * bp + QU * bq) * 2.0;
             oV[i] = Vsw * -(QV * bp - PV * bq) * 2.0;
         }
+        if (p->luma_nr_level > 0.0) orc_ynr_line(p, oY);
     }
     free(m);
     free(n);

@@ -28,7 +28,7 @@ class OrcParams(C.Structure):
                 ("clamp_mode", C.c_int32),
                 ("pad_left", C.c_int32), ("pad_top", C.c_int32),
                 ("out_width", C.c_int32), ("out_height", C.c_int32),
-                ("kr", C.c_double), ("kb", C.c_double)]
+                ("kr", C.c_double), ("kb", C.c_double), ("luma_nr_level", C.c_double)]
 
 
 # output formats of the oracle's OutputWriter: float Y'CbCr planes, u16
@@ -84,7 +84,16 @@ def lib():
         _lib.orc_decode_sequence.argtypes = [P(OrcParams), i32, P(d), P(C.c_uint16), i32, P(i32), i32, i32,
                                              C.c_void_p, i32]
         _lib.orc_decode_sequence.restype = C.c_int
+        _lib.orc_ynr_line.argtypes = [P(OrcParams), P(d)]
     return _lib
+
+
+def ynr_line(p: OrcParams, yline) -> np.ndarray:
+    """PalColour::doYNR on one frame line of field_width Y samples (a copy)."""
+    y = np.array(yline, dtype=np.float64)
+    assert y.shape == (p.field_width,)
+    lib().orc_ynr_line(C.byref(p), _dp(y))
+    return y
 
 
 def _dp(a):
@@ -115,13 +124,13 @@ def pal_params(first_line=44, last_line=619, avs=185, ave=1107, black=0x4000, wh
                clamp_mode=0, pad_left=0, pad_top=0, out_width=None, out_height=None,
                chroma_gain=1.0, chroma_phase_deg=0.0, color_difference="modern",
                broadcast_scaling="scientific", field_width=1135, field_height=313,
-               burst=(98, 138), fsc=4433618.75, sample_rate=17734475.0) -> OrcParams:
+               burst=(98, 138), fsc=4433618.75, sample_rate=17734475.0, luma_nr_level=0.0) -> OrcParams:
     ecb, ecr = color_conversion(color_difference, broadcast_scaling)
     kr, kb = (0.30, 0.11) if color_difference == "classic" else (0.299, 0.114)
     aw, ah = ave - avs, last_line - first_line + 1
     return OrcParams(field_width, field_height, avs, ave, first_line, last_line, burst[0], burst[1],
                      black, white, fsc, sample_rate, chroma_gain, chroma_phase_deg, ecb, ecr,
-                     clamp_mode, pad_left, pad_top, out_width or aw, out_height or ah, kr, kb)
+                     clamp_mode, pad_left, pad_top, out_width or aw, out_height or ah, kr, kb, luma_nr_level)
 
 
 def uniform_thresholds(kind: int, threshold: float = 0.4) -> np.ndarray:

@@ -176,7 +176,6 @@ def test_option_registry(bars_capture):
     (dict(output_format="yuv440p16"), "CHD_E_INVALID_ARG"),
     (dict(color_difference_precision="new"), "CHD_E_INVALID_ARG"),
     (dict(output_format="rgb48"), "CHD_E_UNSUPPORTED"),
-    (dict(luma_nr_level=1.0), "CHD_E_UNSUPPORTED"),
     (dict(), "CHD_E_UNSUPPORTED"),  # valid options; no HIP device here
 ])
 def test_commit_validation(bars_capture, opts, status):

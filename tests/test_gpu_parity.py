@@ -24,6 +24,22 @@ def _oracle(orc, kind, fields, frame, fmt, **geo):
     return orc.decode_frame(p, kind, fields, frame, t, 0 if fmt == "yuv444ps" else 1)
 
 
+def _pads(aw, ah, mult=16):
+    """Left/top padding and padded size (OutputWriter::updateConfiguration,
+    output_writer.cpp:214-238)."""
+    pl = pt = 0
+    w, h, pr, pb = aw, ah, 0, 0
+    while w % mult:
+        if w % 2 == 0: pr += 1
+        else: pl += 1
+        w = pl + aw + pr
+    while h % mult:
+        if h % 2 == 0: pb += 1
+        else: pt += 1
+        h = pt + ah + pb
+    return pl, pt, w, h
+
+
 def _check(got, want, fmt):
     d = np.abs(got.astype(np.float64) - want.astype(np.float64))
     tol = F32_TOL if fmt in ("yuv444ps", "grays") else U16_TOL
@@ -123,6 +139,68 @@ def test_second_field_marked_first(orc, patch_capture, tmp_path, kind, fmt):
             assert np.abs(got.astype(np.float64) - reg).max() > (1e-2 if fmt == "yuv444ps" else 100)
 
 
+@pytest.mark.parametrize("kind", [6, 7, 8])
+def test_second_field_first_pairing(orc, patch_capture, kind):
+    """Second-field-first pairing (LdDecodeMetaData::getFieldNumber,
+    core.cpp:1129-1150): frame k is the first field 2k+3 with the second
+    field 2k+2 (1-based), one frame fewer than first-field-first
+    (core.cpp:1092-1106)."""
+    path, fields = patch_capture
+    v = abi.Video(path, second_field_first=True)
+    n = fields.shape[0] // 2 - 1
+    assert v.info().num_frames == n
+    paired = np.stack([fields[i] for k in range(n) for i in (2 * k + 2, 2 * k + 1)])
+    dec = abi.Decoder(v, kind, output_format="yuv444ps").commit()
+    for f in (0, 2, n - 1):
+        got = dec.decode(f)
+        _check(got, _oracle(orc, kind, paired, f, "yuv444ps"), "yuv444ps")
+    with pytest.raises(abi.ChdError):
+        dec.decode(n)
+
+
+@pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
+@pytest.mark.parametrize("kind", [6, 7, 8])
+def test_luma_nr(orc, patch_capture, kind, fmt):
+    """luma_nr_level cores E'Y after demodulation (PalColour::doYNR,
+    palcolour.cpp:411-448, 643-645); U and V are those of the uncored
+    decode, bit for bit."""
+    path, fields = patch_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format=fmt, luma_nr_level=2.0).commit()
+    for f in (0, 3):
+        got = dec.decode(f)
+        _check(got, _oracle(orc, kind, fields, f, fmt, luma_nr_level=2.0), fmt)
+    plain = abi.Decoder(v, kind, output_format=fmt).commit().decode(3)
+    assert np.array_equal(got[1:], plain[1:])
+    d = np.abs(got[0].astype(np.float64) - plain[0].astype(np.float64))
+    assert d.max() > (1e-3 if fmt == "yuv444ps" else 50)
+
+
+@pytest.mark.parametrize("fmt", ["rgb48", "rgbs"])
+def test_luma_nr_wide_line_rgb(orc, patch_capture, fmt):
+    """Coring on a 1112-sample active line (the column-segmented 4fSC
+    demodulator) with padding and R'G'B' output."""
+    path, fields = patch_capture
+    crop = dict(first_active_frame_line=40, last_active_frame_line=600, first_active_sample=9,
+                last_active_sample=1120)
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format=fmt, padding_multiple=16, luma_nr_level=3.5,
+                      **crop).commit()
+    oi = dec.output_info()
+    got = dec.decode(2)
+    fl, ll = crop["first_active_frame_line"], crop["last_active_frame_line"]
+    fs, ls = crop["first_active_sample"], crop["last_active_sample"]
+    aw, ah = ls - fs + 1, ll - fl + 1
+    pl, pt, w, h = _pads(aw, ah)
+    assert (w, h) == (oi.width, oi.height)
+    out_fmt = orc.OUT_RGB48 if fmt == "rgb48" else orc.OUT_RGBS
+    want = orc.decode_frame(orc.pal_params(first_line=fl, last_line=ll, avs=fs, ave=ls + 1, pad_left=pl, pad_top=pt,
+                                           out_width=oi.width, out_height=oi.height, luma_nr_level=3.5),
+                            8, fields, 2, orc.uniform_thresholds(8), out_fmt)
+    d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+    assert d.max() <= (1 if fmt == "rgb48" else F32_TOL), d.max()
+
+
 @pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
 @pytest.mark.parametrize("capture", ["bars_capture", "patch_capture"])
 def test_pal2d_full_frames(orc, request, capture, fmt):
@@ -166,17 +244,7 @@ def test_crop_padding_clamp(orc, patch_capture, kind, crop):
     fs, ls = crop["first_active_sample"], crop["last_active_sample"]
     aw, ah = ls - fs + 1, ll - fl + 1
     assert oi.width % 16 == 0 and oi.height % 16 == 0 and oi.width >= aw and oi.height >= ah
-    # OutputWriter::updateConfiguration padding order (output_writer.cpp:214-238)
-    pl = pt = 0
-    w, h, pr, pb = aw, ah, 0, 0
-    while w % 16:
-        if w % 2 == 0: pr += 1
-        else: pl += 1
-        w = pl + aw + pr
-    while h % 16:
-        if h % 2 == 0: pb += 1
-        else: pt += 1
-        h = pt + ah + pb
+    pl, pt, w, h = _pads(aw, ah)
     want = _oracle(orc, kind, fields, 3, "yuv444ps", first_line=fl, last_line=ll, avs=fs, ave=ls + 1,
                    clamp_mode=1, pad_left=pl, pad_top=pt, out_width=w, out_height=h)
     _check(got, want, "yuv444ps")
