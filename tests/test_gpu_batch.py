@@ -78,3 +78,36 @@ def test_every_launch_shape_gives_the_same_frames(sequences):
             assert torch.equal(out, ref), (strip, groups)
     with pytest.raises(abi.ChdError):
         b.set_shape(5, 0)
+
+
+@pytest.mark.parametrize("kind", [abi.CHD_DEC_TRANSFORM_3D, abi.CHD_DEC_PAL_2D])
+def test_luma_nr_plan_and_batch(orc, sequences, kind):
+    """Luma coring (luma_nr_level, PalColour::doYNR) through chd_tpal_decode in
+    chunks of 4 frames and through a captured batch with its own raw planes:
+    the plan's frames match the oracle's cored decode, and every replay gives
+    them bit for bit."""
+    import torch
+    plan = abi.TpalPlan(abi.pal_tpal_params(kind=kind, max_batch_frames=4, luma_nr_level=1.5))
+    keep, runs = _upload(plan, sequences)
+    total = sum(r[3] for r in runs)
+    ref = torch.zeros((total, plan.frame_bytes // 4), dtype=torch.float32, device="cuda:0")
+    plan.decode(runs, ref.data_ptr())
+    torch.cuda.synchronize()
+    got = ref.cpu().numpy().reshape(total, 3, 576, 922)
+    p = orc.pal_params(luma_nr_level=1.5)
+    base = 0
+    for f, r in zip(sequences, runs):
+        n = r[3]
+        for i in sorted({0, n - 1}):
+            want = orc.decode_frame(p, kind, f, i, orc.uniform_thresholds(kind), 0)
+            d = np.abs(got[base + i].astype(np.float64) - want)
+            assert d.max() <= 1e-4, (base + i, d.max())
+        base += n
+    out = torch.zeros_like(ref)
+    batch = abi.TpalBatch(plan, runs, out.data_ptr(), autotune=False)
+    for rep in range(2):
+        out.zero_()
+        batch.run(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+    batch.close()
