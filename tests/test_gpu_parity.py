@@ -158,6 +158,27 @@ def test_second_field_first_pairing(orc, patch_capture, kind):
         dec.decode(n)
 
 
+@pytest.mark.parametrize("kind", [6, 8])
+def test_reverse_field_order(patch_capture, kind):
+    """reverse_field_order flips the pairing order the metadata gives
+    (isFirstFieldFirst != reverseFieldOrder, c_api_decoder.cpp:859-874): on a
+    first-field-first capture it decodes the second-field-first frames, and
+    on a video opened second-field-first it restores the plain ones."""
+    path, fields = patch_capture
+    n = fields.shape[0] // 2 - 1
+    rev = abi.Decoder(abi.Video(path), kind, output_format="yuv444ps", reverse_field_order=True).commit()
+    sff = abi.Decoder(abi.Video(path, second_field_first=True), kind, output_format="yuv444ps").commit()
+    for f in (0, n - 1):
+        assert np.array_equal(rev.decode(f), sff.decode(f)), f
+    with pytest.raises(abi.ChdError):
+        rev.decode(n)
+    both = abi.Decoder(abi.Video(path, second_field_first=True), kind, output_format="yuv444ps",
+                       reverse_field_order=True).commit()
+    plain = abi.Decoder(abi.Video(path), kind, output_format="yuv444ps").commit()
+    for f in (0, n):
+        assert np.array_equal(both.decode(f), plain.decode(f)), f
+
+
 @pytest.mark.parametrize("fmt", ["yuv444ps", "yuv444p16"])
 @pytest.mark.parametrize("kind", [6, 7, 8])
 def test_luma_nr(orc, patch_capture, kind, fmt):
