@@ -67,6 +67,13 @@ struct TbcFile {
 };
 
 chd_status_t read_ld_json(const std::string &path, Metadata &meta, std::string &err);
+// `.tbc.db` (ld-decode SQLite) sidecar, through the system libsqlite3 loaded
+// at run time (sqlite_sidecar.cpp)
+chd_status_t read_ld_sqlite(const std::string &path, Metadata &meta, std::string &err);
+// videoParameters.system names (parseVideoSystemName, core.cpp:116) and the per-system
+// subcarrier and active-line defaults (core.cpp:40-84)
+bool parse_system(const std::string &name, System &out);
+void apply_system_defaults(VideoParams &vp);
 
 }  // namespace chd
 

@@ -89,6 +89,8 @@ std::string normalise_system(std::string n)
     return n;
 }
 
+}  // namespace
+
 bool parse_system(const std::string &name, System &out)
 {
     const std::string n = normalise_system(name);
@@ -128,6 +130,8 @@ void apply_system_defaults(VideoParams &vp)
         break;
     }
 }
+
+namespace {
 
 bool read_int_array(const json::Value *v, std::vector<int32_t> &out)
 {
@@ -318,20 +322,25 @@ chd_status_t chd_video_open_composite(const char *path, const char *metadata_pat
     if (metadata_path_or_null != nullptr) {
         side = metadata_path_or_null;
         if (!file_exists(side)) return chd::fail(CHD_E_FILE_NOT_FOUND, fn + ": sidecar file does not exist: " + side);
+    } else if (file_exists(p + ".db")) {
+        // resolveSidecarFlavour (c_api_video.cpp:304-321): `.db` before `.json`
+        side = p + ".db";
     } else if (file_exists(p + ".json")) {
         side = p + ".json";
-    } else if (file_exists(p + ".db")) {
-        side = p + ".db";
     } else {
-        return chd::fail(CHD_E_METADATA_MISSING, fn + ": no `.json` sidecar next to " + p);
+        return chd::fail(CHD_E_METADATA_MISSING, fn + ": no `.db` or `.json` sidecar next to " + p);
     }
-    if (ends_with(side, ".db") || ends_with(side, ".meta"))
-        return chd::fail(CHD_E_FORMAT_UNSUPPORTED,
-                         fn + ": SQLite sidecars are not supported by this build; use the `.tbc.json` form");
+    if (ends_with(side, ".meta"))
+        return chd::fail(CHD_E_FORMAT_UNSUPPORTED, fn + ": CVBS `.meta` sidecars are not supported by this build");
     auto v = std::make_unique<chd_video>();
     v->path = p;
     std::string err;
-    const chd_status_t st = chd::read_ld_json(side, v->meta, err);
+    // LdDecodeMetaData::read (core.cpp:335-350): `.json` is JSON, anything
+    // else SQLite
+    std::string lower = side;
+    for (char &c : lower) c = (char)std::tolower((unsigned char)c);
+    const chd_status_t st = ends_with(lower, ".json") ? chd::read_ld_json(side, v->meta, err)
+                                                      : chd::read_ld_sqlite(side, v->meta, err);
     if (st != CHD_OK) return chd::fail(st, fn + ": " + err);
     chd::VideoParams &vp = v->meta.vp;
     if (override_or_null != nullptr) {

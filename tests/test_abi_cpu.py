@@ -142,7 +142,8 @@ def test_open_errors(tmp_path, bars_capture):
     (tmp_path / "x.tbc").write_bytes(b"\0" * 16)
     with pytest.raises(abi.ChdError) as e:
         abi.Video(str(tmp_path / "x.tbc"))
-    assert e.value.name == "CHD_E_FORMAT_UNSUPPORTED"
+    # a truncated SQLite sidecar fails to read (c_api_video.cpp:526-530)
+    assert e.value.name == "CHD_E_METADATA_CORRUPT"
 
 
 def test_option_registry(bars_capture):

@@ -465,3 +465,30 @@ def test_truncated_capture_decodes_black_neighbours(orc, patch_capture, tmp_path
     f2[10:] = 0x4000
     f2 = f2[:12]
     _check(got, _oracle(orc, 8, f2, 3, "yuv444ps"), "yuv444ps")
+
+
+@pytest.mark.parametrize("kind", [7, 8])
+def test_sqlite_sidecar_decodes_like_json(patch_capture, tmp_path, kind):
+    """A capture whose sidecar is the `.tbc.db` form decodes bit for bit as
+    with the `.tbc.json` form of the same metadata, dropout concealment
+    included."""
+    import json
+    import shutil
+    from libchromadec_amd import synth
+    from test_sqlite_sidecar import _write_db
+    _, fields = patch_capture
+    drops = [[] for _ in range(fields.shape[0])]
+    drops[4] = [(300, 340, 100)]
+    jpath, dpath = str(tmp_path / "j.tbc"), str(tmp_path / "d.tbc")
+    side = synth.write_tbc(jpath, fields, dropouts=drops)
+    with open(side) as fh:
+        meta = json.load(fh)
+    shutil.copy(jpath, dpath)
+    _write_db(dpath + ".db", meta)
+    decs = []
+    for p in (jpath, dpath):
+        d = abi.Decoder(abi.Video(p), kind, output_format="yuv444ps")
+        d.set_dropout(True)
+        decs.append(d.commit())
+    for f in (1, 2, 4):
+        assert np.array_equal(decs[0].decode(f), decs[1].decode(f)), f
