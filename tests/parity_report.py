@@ -18,6 +18,7 @@ float planes (full scale: E'Y 0..1, E'Cb/E'Cr +-0.5) and in yuv444p16 LSB:
         compared bit for bit with the oracle's), plain and learned
   PAL 2D   the 64-frame sequence with CHD_DEC_PAL_2D (the PALcolour 2D
         filter on the raw composite, no transform)
+  luma coring  the 64-frame 3D decode with luma_nr_level 2.0
 
 The oracle is the checker here; nothing it computes feeds the GPU path.
 """
@@ -58,9 +59,9 @@ def _merge(rows):
             "uv_rms": float(np.sqrt(np.mean([r["uv_rms"] ** 2 for r in rows]))), "u16": rows[0]["u16"]}
 
 
-def _plan(kind, fmt, frames):
+def _plan(kind, fmt, frames, **kw):
     pf = abi.CHD_PIXEL_YUV444PS if fmt == "ps" else abi.CHD_PIXEL_YUV444P16
-    return abi.TpalPlan(abi.pal_tpal_params(kind=kind, pixel_format=pf, max_batch_frames=frames))
+    return abi.TpalPlan(abi.pal_tpal_params(kind=kind, pixel_format=pf, max_batch_frames=frames, **kw))
 
 
 def _pack(plan, fields):
@@ -82,8 +83,8 @@ def _decode(plan, packed, n_frames, fmt):
     return a if fmt == "ps" else a.view(np.uint16)
 
 
-def _oracle_plain(kind, fields, frames, fmt):
-    return orc.decode_sequence(orc.pal_params(), kind, fields, frames, orc.uniform_thresholds(kind),
+def _oracle_plain(kind, fields, frames, fmt, **kw):
+    return orc.decode_sequence(orc.pal_params(**kw), kind, fields, frames, orc.uniform_thresholds(kind),
                                0 if fmt == "ps" else 1, threads=min(16, os.cpu_count() or 1))
 
 
@@ -141,6 +142,16 @@ def config_3d(fields):
         got = _decode(plan, _pack(plan, fields), 64, fmt)
         res[fmt] = _stats(got[frames], _oracle_plain(8, fields, frames, fmt), fmt == "p16")
     return "3D, 64-frame sequence", frames, res
+
+
+def config_ynr(fields):
+    frames = [0, 1, 31, 62, 63]
+    res = {}
+    for fmt in ("ps", "p16"):
+        plan = _plan(abi.CHD_DEC_TRANSFORM_3D, fmt, 64, luma_nr_level=2.0)
+        got = _decode(plan, _pack(plan, fields), 64, fmt)
+        res[fmt] = _stats(got[frames], _oracle_plain(8, fields, frames, fmt, luma_nr_level=2.0), fmt == "p16")
+    return "3D with luma coring (luma_nr_level 2.0), 64-frame sequence", frames, res
 
 
 def config_batch():
@@ -214,7 +225,7 @@ def main():
     fields = bench.make_workload(64)[0]
     lines = ["| Config | Frames checked | Output | E'Y max | U/V max | U/V RMS |", "|---|---|---|---|---|---|"]
     for fn in (config_2d, lambda: config_3d(fields), config_batch, lambda: config_learned(fields), config_stress,
-               lambda: config_pal2d(fields)):
+               lambda: config_pal2d(fields), lambda: config_ynr(fields)):
         name, frames, res = fn()
         for fmt, r in res.items():
             if r["u16"]:
