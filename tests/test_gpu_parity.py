@@ -409,11 +409,14 @@ def test_frame_outlives_its_decoder(patch_capture):
         fr.free()
 
 
+@pytest.mark.parametrize("nr", [0.0, 1.5])
 @pytest.mark.parametrize("kind", [6, 7, 8])
-def test_async_matches_sync(patch_capture, kind):
+def test_async_matches_sync(patch_capture, kind, nr):
+    """Async workers (each with its own plan, stream and, with luma coring,
+    raw planes) give the synchronous frames bit for bit."""
     path, _ = patch_capture
     v = abi.Video(path)
-    dec = abi.Decoder(v, kind, output_format="yuv444p16", thread_count=3).commit()
+    dec = abi.Decoder(v, kind, output_format="yuv444p16", thread_count=3, luma_nr_level=nr).commit()
     res = dec.decode_async([5, 0, 3, 1, 4, 2, 3])
     assert sorted(res) == [0, 1, 2, 3, 4, 5]
     for i in range(6):
