@@ -303,6 +303,7 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     ok = ok && hipHostMalloc((void **)&plan->h_field_jobs, plan->max_units * sizeof(tpal::FieldJob2D), 0) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&plan->h_frame_jobs, plan->max_frames * sizeof(tpal::FrameJob), 0) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&plan->staging_done, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&plan->call_done, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&plan->ev[i]) == hipSuccess;
     if (!ok) {
         chd_tpal_plan_free(plan);
@@ -328,22 +329,23 @@ static int plan_nchroma(const chd_tpal_plan *plan)
     return plan->kind == CHD_DEC_TRANSFORM_3D ? 4 : plan->kind == CHD_DEC_TRANSFORM_2D ? 2 : 0;
 }
 
-static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hipStream_t s)
+static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int n_units, int n_frames, hipStream_t s,
+                              const std::string &who)
 {
     const Geom &g = plan->g;
     hipError_t e = hipSuccess;
     if (plan->kind == CHD_DEC_TRANSFORM_3D) {
-        e = hipMemcpyAsync(plan->d_slab_jobs, plan->h_slab_jobs, n_units * sizeof(tpal::SlabJob),
+        e = hipMemcpyAsync(ws.slab_jobs, plan->h_slab_jobs, n_units * sizeof(tpal::SlabJob),
                            hipMemcpyHostToDevice, s);
     } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
-        e = hipMemcpyAsync(plan->d_field_jobs, plan->h_field_jobs, n_units * sizeof(tpal::FieldJob2D),
+        e = hipMemcpyAsync(ws.field_jobs, plan->h_field_jobs, n_units * sizeof(tpal::FieldJob2D),
                            hipMemcpyHostToDevice, s);
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(plan->d_frame_jobs, plan->h_frame_jobs, n_frames * sizeof(tpal::FrameJob),
+        e = hipMemcpyAsync(ws.frame_jobs, plan->h_frame_jobs, n_frames * sizeof(tpal::FrameJob),
                            hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
-    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: job upload", e));
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg((who + ": job upload").c_str(), e));
     const bool border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
     if (border) {
         for (int i = 0; i < n_frames; ++i) tpal::launch_fill_border(g, plan->h_frame_jobs[i].out, 1, s);
@@ -353,27 +355,27 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, int n_units, int n_frames, hi
         // learned mask: tile spectra -> network -> masked inverse
         chd_mask_net *net = plan->net;
         const int64_t tiles = (int64_t)n_units * g.n_ty3 * g.n_tx3;
-        tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra,
+        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra,
                             plan->d_spec, nullptr);
         mnet::launch_infer(reinterpret_cast<const float2 *>(plan->d_spec), tiles, net->theta, net->f, plan->d_mask,
                            net->cus, s);
-        tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply,
+        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply,
                             plan->d_spec, plan->d_mask);
     } else if (plan->kind == CHD_DEC_TRANSFORM_3D)
-        tpal::launch_tpal3d(g, plan->d_slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s);
+        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s);
     else if (plan->kind == CHD_DEC_TRANSFORM_2D)
-        tpal::launch_tpal2d(g, plan->d_field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
+        tpal::launch_tpal2d(g, ws.field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
     if (plan->timing) hipEventRecord(plan->ev[1], s);
-    e = tpal::launch_demod(g, plan->d_frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
-                           plan->n_lblocks, s, plan_nchroma(plan), plan->d_raw);
-    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: demod launch", e));
+    e = tpal::launch_demod(g, ws.frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
+                           plan->n_lblocks, s, plan_nchroma(plan), ws.raw);
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg((who + ": demod launch").c_str(), e));
     for (int i = 0; i < n_frames; ++i) {
         const tpal::FrameJob &fj = plan->h_frame_jobs[i];
         if (fj.offset[0] == fj.offset[1]) tpal::launch_fill_other_parity(g, fj.out, fj.offset[0] ^ 1, s);
     }
     if (plan->timing) hipEventRecord(plan->ev[2], s);
     e = hipGetLastError();
-    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode: kernel launch", e));
+    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg((who + ": kernel launch").c_str(), e));
     if (plan->timing) {
         // accumulate kernel time of this chunk (waits for it; timing mode only)
         hipEventSynchronize(plan->ev[2]);
@@ -396,6 +398,36 @@ static void wait_staging(chd_tpal_plan *plan)
     }
 }
 
+// The workspace a plan owns.
+static TpalWorkspace plan_ws(const chd_tpal_plan *plan)
+{
+    TpalWorkspace ws;
+    ws.partials = plan->d_partials;
+    ws.max_units = plan->max_units;
+    ws.max_frames = plan->max_frames;
+    ws.slab_jobs = plan->d_slab_jobs;
+    ws.field_jobs = plan->d_field_jobs;
+    ws.frame_jobs = plan->d_frame_jobs;
+    ws.raw = plan->d_raw;
+    return ws;
+}
+
+// A call enqueued on a different stream than the previous one must not
+// overwrite partials or raw planes that the previous call's kernels still
+// read: order it after that call's last kernel.
+static void call_begin(chd_tpal_plan *plan, hipStream_t s)
+{
+    if (plan->call_pending && plan->last_stream != s) hipStreamWaitEvent(s, plan->call_done, 0);
+}
+
+static void call_end(chd_tpal_plan *plan, hipStream_t s)
+{
+    if (hipEventRecord(plan->call_done, s) == hipSuccess) {
+        plan->call_pending = true;
+        plan->last_stream = s;
+    }
+}
+
 static int64_t frame_bytes(const chd_tpal_plan *plan)
 {
     const Geom &g = plan->g;
@@ -412,6 +444,8 @@ chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n
     hipSetDevice(plan->device);
     plan->last_tile_ms = plan->last_demod_ms = 0;
     plan->n_tile_launches = plan->n_demod_launches = 0;
+    const TpalWorkspace ws = plan_ws(plan);
+    call_begin(plan, s);
     const int per_chunk = std::max(1, std::min(plan->max_frames, plan->max_units / 2));
     for (int base = 0; base < n; base += per_chunk) {
         const int cnt = std::min(per_chunk, n - base);
@@ -462,10 +496,11 @@ chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n
                 fj.nchroma = 0;
             }
         }
-        chd_status_t st = run_chunk(plan, 2 * cnt, cnt, s);
+        chd_status_t st = run_chunk(plan, ws, 2 * cnt, cnt, s, "chd_decode_frame");
         plan->staging_pending = true;
         if (st != CHD_OK) return st;
     }
+    call_end(plan, s);
     return CHD_OK;
 }
 
@@ -500,6 +535,7 @@ void chd_tpal_plan_free(chd_tpal_plan_t *plan)
     if (plan->h_field_jobs) hipHostFree(plan->h_field_jobs);
     if (plan->h_frame_jobs) hipHostFree(plan->h_frame_jobs);
     if (plan->staging_done) hipEventDestroy(plan->staging_done);
+    if (plan->call_done) hipEventDestroy(plan->call_done);
     for (int i = 0; i < 4; ++i)
         if (plan->ev[i]) hipEventDestroy(plan->ev[i]);
     delete plan;
@@ -525,19 +561,27 @@ chd_status_t chd_tpal_pack_fields(chd_tpal_plan_t *plan, const uint16_t *d_src, 
     return CHD_OK;
 }
 
-chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs, void *d_out,
-                             void *stream)
+}  // extern "C"
+
+chd_status_t tpal_check_runs(const chd_tpal_plan *plan, const chd_tpal_sequence_t *seqs, int n_seqs, void *d_out,
+                             const char *who)
 {
+    const std::string W = who;
     if (plan == nullptr || (n_seqs > 0 && (seqs == nullptr || d_out == nullptr)) || n_seqs < 0)
-        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_decode: null argument");
+        return chd::fail(CHD_E_INVALID_ARG, W + ": null argument");
     for (int i = 0; i < n_seqs; ++i) {
         const chd_tpal_sequence_t &q = seqs[i];
         if (q.count < 0 || q.first_frame < 0 || q.first_frame + q.count > q.n_frames ||
             (q.count > 0 && q.d_fields == nullptr))
-            return chd::fail(CHD_E_OUT_OF_RANGE, "chd_tpal_decode: frame run outside its sequence");
+            return chd::fail(CHD_E_OUT_OF_RANGE, W + ": frame run outside its sequence");
     }
+    return CHD_OK;
+}
+
+chd_status_t tpal_decode_runs(chd_tpal_plan *plan, const TpalWorkspace &ws, const chd_tpal_sequence_t *seqs,
+                              int n_seqs, void *d_out, hipStream_t s, const char *who)
+{
     hipSetDevice(plan->device);
-    const hipStream_t s = (hipStream_t)stream;
     const Geom &g = plan->g;
     const int64_t fb = frame_bytes(plan);
     const bool is3d = plan->kind == CHD_DEC_TRANSFORM_3D;
@@ -548,7 +592,7 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
     int64_t out_index = 0;
     auto flush = [&]() -> chd_status_t {
         if (frames == 0) return CHD_OK;
-        chd_status_t st = run_chunk(plan, units, frames, s);
+        chd_status_t st = run_chunk(plan, ws, units, frames, s, who);
         plan->staging_pending = true;
         units = frames = runs = 0;
         return st;
@@ -557,6 +601,7 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
         return (fi >= 0 && fi < 2 * q.n_frames) ? q.d_fields + fi * g.field_elems : nullptr;
     };
     wait_staging(plan);
+    call_begin(plan, s);
     for (int si = 0; si < n_seqs; ++si) {
         const chd_tpal_sequence_t &q = seqs[si];
         int64_t f = q.first_frame;
@@ -565,10 +610,10 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
             // frames this run can take in the current chunk
             const int extra = is3d ? 2 : 0;
             const int unit_per_frame = is3d ? 1 : 2;
-            int room_units = plan->max_units - units - extra;
-            int room = std::min(plan->max_frames - frames, room_units / unit_per_frame);
+            int room_units = ws.max_units - units - extra;
+            int room = std::min(ws.max_frames - frames, room_units / unit_per_frame);
             if (runs >= kMaxRunsPerChunk || room <= 0) {
-                if (frames == 0) return chd::fail(CHD_E_INTERNAL, "chd_tpal_decode: workspace too small");
+                if (frames == 0) return chd::fail(CHD_E_INTERNAL, std::string(who) + ": workspace too small");
                 chd_status_t st = flush();
                 if (st != CHD_OK) return st;
                 wait_staging(plan);
@@ -585,7 +630,7 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
                         sj.field[z] = field_ptr(q, 2 * slab + z);
                         sj.offset[z] = z & 1;
                     }
-                    sj.partial = plan->d_partials + (size_t)(units + k) * plan->unit_floats;
+                    sj.partial = ws.partials + (size_t)(units + k) * plan->unit_floats;
                 }
                 units += cnt + 2;
             } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
@@ -593,7 +638,7 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
                     tpal::FieldJob2D &jj = plan->h_field_jobs[units + k];
                     jj.field = field_ptr(q, 2 * f + k);
                     jj.offset = k & 1;
-                    jj.partial = plan->d_partials + (size_t)(units + k) * plan->unit_floats;
+                    jj.partial = ws.partials + (size_t)(units + k) * plan->unit_floats;
                 }
                 units += 2 * cnt;
             } else {
@@ -628,8 +673,20 @@ chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *s
             f += cnt;
         }
     }
-    return flush();
+    const chd_status_t st = flush();
+    if (st == CHD_OK) call_end(plan, s);
+    return st;
 }
+
+chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs, void *d_out,
+                             void *stream)
+{
+    const chd_status_t st = tpal_check_runs(plan, seqs, n_seqs, d_out, "chd_tpal_decode");
+    if (st != CHD_OK) return st;
+    return tpal_decode_runs(plan, plan_ws(plan), seqs, n_seqs, d_out, (hipStream_t)stream, "chd_tpal_decode");
+}
+
+extern "C" {
 
 int64_t chd_tpal_tiles_per_slab(const chd_tpal_plan_t *plan)
 {
@@ -681,6 +738,7 @@ chd_status_t chd_tpal_slab_spectra(chd_tpal_plan_t *plan, const uint16_t *d_fiel
     const hipStream_t s = (hipStream_t)stream;
     const Geom &g = plan->g;
     const int64_t tps = (int64_t)g.n_ty3 * g.n_tx3;
+    call_begin(plan, s);
     for (int base = 0; base < n_slabs; base += plan->max_units) {
         const int cnt = std::min(plan->max_units, n_slabs - base);
         wait_staging(plan);
@@ -705,6 +763,7 @@ chd_status_t chd_tpal_slab_spectra(chd_tpal_plan_t *plan, const uint16_t *d_fiel
         e = hipGetLastError();
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_slab_spectra: launch", e));
     }
+    call_end(plan, s);
     return CHD_OK;
 }
 
@@ -720,6 +779,7 @@ chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_
     const int64_t fb = frame_bytes(plan);
     const int per_chunk = std::max(1, std::min(plan->max_frames, plan->max_units / 2));
     const bool border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
+    call_begin(plan, s);
     for (int64_t base = 0; base < n_frames; base += per_chunk) {
         const int cnt = (int)std::min<int64_t>(per_chunk, n_frames - base);
         wait_staging(plan);
@@ -747,6 +807,7 @@ chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_
         if (e == hipSuccess) e = hipGetLastError();
         if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg("chd_tpal_decode_separated: launch", e));
     }
+    call_end(plan, s);
     return CHD_OK;
 }
 

@@ -30,6 +30,12 @@ struct chd_tpal_plan {
     tpal::FrameJob *h_frame_jobs = nullptr;
     hipEvent_t staging_done = nullptr;
     bool staging_pending = false;
+    // The partials, job lists and uncored planes are shared by every call on
+    // the plan: a call on another stream first waits for the previous call's
+    // last kernel (recorded in call_done on last_stream).
+    hipEvent_t call_done = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool call_pending = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timing = false;
     double last_tile_ms = 0, last_demod_ms = 0;
@@ -49,5 +55,26 @@ struct TpalWindow {
     void *out;
 };
 
+// Device memory one decode call works in: overlap-add partials, job lists
+// and, with luma coring, the uncored planes.  A plan owns one sized for
+// max_batch_frames; chd_tpal_decode_static carves one out of a static
+// device array.  max_units and max_frames never exceed the plan's (its
+// pinned job staging is sized for those).
+struct TpalWorkspace {
+    float *partials = nullptr;          // max_units * plan->unit_floats
+    int max_units = 0;                  // slabs (3D) or fields (2D) per chunk
+    int max_frames = 0;                 // frames per chunk
+    tpal::SlabJob *slab_jobs = nullptr;
+    tpal::FieldJob2D *field_jobs = nullptr;
+    tpal::FrameJob *frame_jobs = nullptr;
+    float *raw = nullptr;               // max_frames raw planes, with luma coring on
+};
+
 chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **out, const char *who);
 chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n, hipStream_t s);
+// chd_tpal_decode over an explicit workspace (arguments already checked).
+chd_status_t tpal_decode_runs(chd_tpal_plan *plan, const TpalWorkspace &ws, const chd_tpal_sequence_t *seqs,
+                              int n_seqs, void *d_out, hipStream_t s, const char *who);
+// Validates the frame runs of a chd_tpal_decode-style call.
+chd_status_t tpal_check_runs(const chd_tpal_plan *plan, const chd_tpal_sequence_t *seqs, int n_seqs, void *d_out,
+                             const char *who);
