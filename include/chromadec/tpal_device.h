@@ -85,6 +85,23 @@ typedef struct chd_tpal_sequence {
 chd_status_t chd_tpal_decode(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs,
                              void *d_out, void *stream);
 
+/* chd_tpal_decode without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory at decode
+ * time (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The overlap-add partials, job lists and (with luma
+ * coring) uncored planes live in one static device array of
+ * CHD_TPAL_STATIC_SCRATCH_MIB MiB per GPU, held until the process ends, so
+ * frames are decoded in chunks of as many frames as fit (4 for 3D PAL,
+ * 12 for 2D).  Output equals chd_tpal_decode's bit for bit.  The plan still
+ * comes from chd_tpal_plan_create (which allocates) and must have no learned
+ * mask (CHD_E_UNSUPPORTED); a geometry whose single frame needs more than the
+ * array is rejected with CHD_E_INVALID_ARG.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (any plan, any stream)
+ * race, so serialise them. */
+#define CHD_TPAL_STATIC_SCRATCH_MIB 64
+chd_status_t chd_tpal_decode_static(chd_tpal_plan_t *plan, const chd_tpal_sequence_t *seqs, int32_t n_seqs,
+                                    void *d_out, void *stream);
+
 /* Demodulate frames whose chroma was separated elsewhere: the second half
  * of the reference's decode, PalColour::decodeFrames with prefiltered
  * chroma (palcolour.cpp:279-335) and OutputWriter, without the transform.

@@ -151,6 +151,7 @@ SIGNATURES = {
     "chd_tpal_frame_bytes": (C.c_size_t, [_P]),
     "chd_tpal_pack_fields": (C.c_int32, [_P, _P, C.c_int64, _P, _P]),
     "chd_tpal_decode": (C.c_int32, [_P, C.POINTER(TpalSequence), C.c_int32, _P, _P]),
+    "chd_tpal_decode_static": (C.c_int32, [_P, C.POINTER(TpalSequence), C.c_int32, _P, _P]),
     "chd_tpal_set_timing": (C.c_int32, [_P, C.c_int]),
     "chd_tpal_decode_separated": (C.c_int32, [_P, _P, _P, C.c_int64, _P, _P]),
     "chd_tpal_batch_create": (C.c_int32, [_P, C.POINTER(TpalSequence), C.c_int32, _P, C.c_int32, C.POINTER(_P)]),
@@ -441,6 +442,12 @@ class TpalPlan:
         """seqs: list of (d_fields, n_frames, first_frame, count)."""
         arr = (TpalSequence * len(seqs))(*[TpalSequence(*s) for s in seqs])
         check(lib().chd_tpal_decode(self.h, arr, len(seqs), d_out, stream), "chd_tpal_decode")
+
+    def decode_static(self, seqs, d_out: int, stream: int = 0):
+        """chd_tpal_decode_static: the same decode with no device allocation
+        (workspace in the library's static per-GPU scratch array)."""
+        arr = (TpalSequence * len(seqs))(*[TpalSequence(*s) for s in seqs])
+        check(lib().chd_tpal_decode_static(self.h, arr, len(seqs), d_out, stream), "chd_tpal_decode_static")
 
     def decode_separated(self, d_fields: int, d_chroma: int, n_frames: int, d_out: int, stream: int = 0):
         check(lib().chd_tpal_decode_separated(self.h, d_fields, d_chroma, n_frames, d_out, stream),
