@@ -121,8 +121,10 @@ chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_
  * batch live in device memory; the sequences' fields and d_out must stay at
  * the same addresses for the life of the batch (their contents may change).
  * With autotune != 0 (3D plans) the strip length (tiles per work item: 4, 8,
- * 16) and the grid (one workgroup per item, or 2-8 persistent workgroups
- * per CU) are chosen by timing each on this batch.  A plan with a learned
+ * 12, 16, 24) and the grid (one workgroup per item, or 2-12 persistent
+ * workgroups per CU) are chosen by timing each on this batch; without it the
+ * strip length comes from the launch size and the grid is one workgroup per
+ * item.  A plan with a learned
  * mask network (chd_tpal_set_mask_net) is captured with it.  The plan must
  * outlive the batch; use one batch per stream at a time. */
 typedef struct chd_tpal_batch chd_tpal_batch_t;

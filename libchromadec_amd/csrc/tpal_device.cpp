@@ -267,6 +267,7 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
         uniform = uniform && thr32[i] == thr32[0];
     }
     g.thr_uniform = uniform ? thr32[0] : -1.0f;
+    g.thr_uniform_d = uniform ? thr64[0] : -1.0;
 
     chd_status_t st = ensure_tables(p->device);
     if (st != CHD_OK) { delete plan; return st; }
@@ -826,7 +827,7 @@ struct chd_tpal_batch {
     float *d_raw = nullptr;        // uncored Y/U/V of every frame, with luma coring on
     void *d_out = nullptr;
     bool border = false;
-    int strip = tpal::kStrip3, groups = 0;
+    int strip = tpal::kStrip3, groups = 0;   // strip: set at creation
     double tile_ms = 0;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
@@ -986,6 +987,7 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
     b->n_frames = (int)frames.size();
     b->d_out = d_out;
     b->border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
+    if (is3d) b->strip = tpal::choose_strip3(g, b->n_units);
     hipError_t e = hipSuccess;
     if (b->n_units > 0)
         e = hipMalloc((void **)&b->d_partials, (size_t)b->n_units * plan->unit_floats * sizeof(float));
@@ -1029,13 +1031,13 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
         int cus = 0;
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device);
         if (cus <= 0) cus = 256;
-        const int strips[3] = {4, 8, 16};
+        const int strips[5] = {4, 8, 12, 16, 24};
         const int per_cu[6] = {0, 2, 3, 4, 6, 8};
         hipEvent_t t0 = nullptr, t1 = nullptr;
         e = hipEventCreate(&t0);
         if (e == hipSuccess) e = hipEventCreate(&t1);
         double best = 1e30;
-        for (int si = 0; si < 3 && e == hipSuccess; ++si) {
+        for (int si = 0; si < 5 && e == hipSuccess; ++si) {
             for (int pi = 0; pi < 6 && e == hipSuccess; ++pi) {
                 const int groups = per_cu[pi] * cus;
                 batch_tiles(b, b->cap, strips[si], groups);   // warm-up
@@ -1097,8 +1099,9 @@ chd_status_t chd_tpal_batch_config(const chd_tpal_batch_t *batch, int32_t *strip
 
 chd_status_t chd_tpal_batch_set_shape(chd_tpal_batch_t *batch, int32_t strip, int32_t groups)
 {
-    if (batch == nullptr || (strip != 4 && strip != 8 && strip != 16) || groups < 0)
-        return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_batch_set_shape: strip must be 4, 8 or 16 and groups >= 0");
+    if (batch == nullptr || strip < 4 || strip > 64 || strip % 4 != 0 || groups < 0)
+        return chd::fail(CHD_E_INVALID_ARG,
+                         "chd_tpal_batch_set_shape: strip must be a multiple of 4 from 4 to 64 and groups >= 0");
     hipSetDevice(batch->plan->device);
     hipStreamSynchronize(batch->cap);
     batch->strip = strip;

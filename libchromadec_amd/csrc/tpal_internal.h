@@ -38,6 +38,7 @@ struct Geom {
     int32_t cbs, cbe;        // colour burst [cbs, cbe)
     float black;             // black16bIre
     float thr_uniform;       // squared threshold when all bins share it, else -1
+    double thr_uniform_d;    // the same in FP64 (the re-decision's threshold)
     int32_t pitch;           // packed row pitch (uint16 elements)
     int32_t lmargin;         // packed column of sample 0
     int64_t field_elems;     // packed field size (elements)
@@ -111,11 +112,13 @@ hipError_t upload_tables(const Tables &t);
 // to stored spectra.
 enum { kTilePlain = 0, kTileSpectra = 1, kTileApply = 2 };
 constexpr int kTileBins = 5 * 32 * 8;
-// strip: tiles per work item (4, 8 or 16); max_groups > 0 caps the grid
-// (persistent workgroups), 0 launches one workgroup per item.
+// strip: tiles per work item (a multiple of 4; 0 picks one for the launch
+// size with choose_strip3); max_groups > 0 caps the grid (persistent
+// workgroups), 0 launches one workgroup per item.
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
                    hipStream_t s, int mode = kTilePlain, void *spec = nullptr, const float *mask = nullptr,
-                   int strip = kStrip3, int max_groups = 0);
+                   int strip = 0, int max_groups = 0);
+int choose_strip3(const Geom &g, int n_jobs);
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
                    const double *thr64, hipStream_t s);
 // nchroma: the FrameJob.nchroma every job of the launch has (4, 2, 1, or 0

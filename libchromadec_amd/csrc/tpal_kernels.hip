@@ -220,7 +220,7 @@ __device__ __forceinline__ int decide64(double ar, double ai, double br, double 
 }
 
 // ---------------------------------------------------------------------------
-// 3D tile kernel.  One 64-lane workgroup = one strip of kStrip3 (+1 leading)
+// 3D tile kernel.  One 64-lane workgroup = one strip of 4-24 (+1 leading)
 // tiles of one tile row of one slab.
 // ---------------------------------------------------------------------------
 
@@ -361,12 +361,12 @@ __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
 // ([tile][k][ky][kz], tile = (job * n_ty3 + ty) * n_tx3 + tx).
 // kTileApply: read those bins back, weight each by the soft mask of its
 // visit (`mask`, [tile][768]) and run the inverse and overlap-add.
-// Work item = one strip of STRIP tiles of one tile row of one slab.  The
+// Work item = one strip of `strip` tiles of one tile row of one slab.  The
 // grid is either one workgroup per item (PERSIST false) or persistent (a
 // multiple of 8 workgroups, each looping over items).
-template <int MODE, int STRIP, bool PERSIST>
-__global__ void __launch_bounds__(64)
-tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const float *__restrict__ thr32,
+template <int MODE, bool PERSIST>
+__global__ void __launch_bounds__(64, 3)   // 3 waves per SIMD: at most 168 VGPRs
+tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int strip_len, const float *__restrict__ thr32,
              const double *__restrict__ thr64, float2 *__restrict__ spec, const float *__restrict__ mask)
 {
     __shared__ typename LdsFor<MODE>::T L;
@@ -382,7 +382,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
     // share 6 of their 8 fields) and neighbouring tile rows (which share 8
     // of their 16 lines), so each field line is fetched into that L2 about
     // once instead of once per tile.
-    const int n_strips = (g.n_tx3 + STRIP - 1) / STRIP;
+    const int n_strips = (g.n_tx3 + strip_len - 1) / strip_len;
     const int total = n_jobs * g.n_ty3 * n_strips;
     const int G = gridDim.x;
     const int xcd = blockIdx.x & 7, q = total >> 3, r = total & 7;
@@ -402,8 +402,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
 #pragma unroll
     for (int z = 0; z < 8; ++z) par[z] = (job.offset[z] - tileY) & 1;
 
-    const int tx_own0 = strip * STRIP;
-    const int tx_end = min(tx_own0 + STRIP, g.n_tx3);
+    const int tx_own0 = strip * strip_len;
+    const int tx_end = min(tx_own0 + strip_len, g.n_tx3);
     // the leading tile completes the first owned columns; spectra need none
     const int tx_begin = MODE == kTileSpectra ? tx_own0 : max(tx_own0 - 1, 0);
 
@@ -594,13 +594,22 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                 };
                 auto steps = [&](auto uc) {
                     constexpr bool kUniform = decltype(uc)::value;
+                    // With one threshold for every visit, plane (x, z) = (4, 2)
+                    // holds the pairs of plane (4, 0) with a and b swapped
+                    // (z' = (10 - z) % 8, rows mirrored), and (4, 6) those of
+                    // (4, 4).  decide32 is symmetric in a and b, so step 10 is
+                    // step 8's ballot moved to the mirror rows, and its pairs
+                    // are re-decided (if at all) through plane (4, 0) / (4, 4).
+                    constexpr int kSteps = kUniform ? 11 : 12;
+                    auto step_of = [](int j) { return (kUniform && j >= 10) ? j + 1 : j; };
                     c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
 #pragma unroll
-                    for (int i = 0; i < 12; ++i) {
+                    for (int j = 0; j < kSteps; ++j) {
+                        const int i = step_of(j);
                         c32 an = a, bn = b;
-                        if (i + 1 < 12) {
-                            an = L.u.S[off_a(i + 1)];
-                            bn = L.u.S[off_b(i + 1)];
+                        if (j + 1 < kSteps) {
+                            an = L.u.S[off_a(step_of(j + 1))];
+                            bn = L.u.S[off_b(step_of(j + 1))];
                         }
                         const int k = i >> 2, z = i & 3;
                         const float th = kUniform ? g.thr_uniform : thr32[z * 96 + k + tho];
@@ -609,8 +618,13 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                         const uint64_t fb = __ballot(d < 0);
                         if (vy == 0) {
                             const int w = k * 8 + z + 4 * vhalf;
-                            L.bal[w] = vhalf ? (uint32_t)(kb >> 32) : (uint32_t)kb;
+                            const uint32_t kw = vhalf ? (uint32_t)(kb >> 32) : (uint32_t)kb;
+                            L.bal[w] = kw;
                             L.flg[w] = vhalf ? (uint32_t)(fb >> 32) : (uint32_t)fb;
+                            if (kUniform && i == 8) {
+                                L.bal[w + 2] = mirror_rows(kw);
+                                L.flg[w + 2] = 0u;
+                            }
                         }
                         a = an;
                         b = bn;
@@ -633,76 +647,92 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, const f
                 if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
                 L.keep[lane] = w;
             }
-            bool any_flag = false;
-            if (lane < 24) any_flag = (L.flg[lane] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) != 0u;
+            // flag words (x - 2) * 8 + z on lanes 0-23, self-mirror bins removed
+            const uint32_t myflag =
+                lane < 24 ? (L.flg[lane] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
+            const uint64_t fwords = __ballot(myflag != 0u);
             __syncthreads();
 
             // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
-            // Each lane sums its two rows' FP64 row DFTs, turned by the (ky, kz)
-            // phase of the bin; a wave reduction gives the exact bin.  The
-            // plane's x is a compile-time constant, so the row DFT's
-            // twiddles are immediates.
-            if (__any(any_flag)) {
+            // Each lane forms the FP64 DFT of its two rows at bins x and 8 - x
+            // once per flagged x; a flagged visit turns them by the (ky, kz)
+            // phase of its bins and a wave reduction gives the exact bins.  x is
+            // a compile-time constant, so the row DFT's twiddles are immediates.
+            if (fwords != 0u) {
                 const double black = (double)g.black;
-                auto plane = [&](auto xc, int z) {
+                auto bins = [&](auto xc) {
                     constexpr int X = decltype(xc)::value, XR = 8 - X;
-                    const int wi = (X - 2) * 8 + z;
-                    uint32_t bits = L.flg[wi];
-                    if (wi == 17 || wi == 21) bits &= ~kSelf;
-                    const int zr = (10 - z) & 7;
-                    while (bits != 0u) {
-                        const int y = __builtin_ctz(bits);
-                        bits &= bits - 1u;
-                        const int yr = (40 - y) & 31;
-                        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+                    uint32_t zmask = (uint32_t)(fwords >> (8 * (X - 2))) & 0xFFu;
+                    if (zmask == 0u) return;
+                    // With c, s the cos/sin of 2 pi X n / 16, bin X of a row is
+                    // sum u (c - i s) and bin 8-X is sum (-1)^n u (c + i s): split
+                    // by n parity.  A row outside the tile's lines holds black:
+                    // u = 0.
+                    // one row at a time: both rows' FP64 samples at once would
+                    // not fit the tile kernel's register budget
+                    double ra[2], ia[2], rb[2], ib[2];
 #pragma unroll 1
-                        for (int h = 0; h < 2; ++h) {
-                            // FP64 row DFT at bins X and 8 - X from the samples.
-                            // With c, s the cos/sin of 2 pi X n / 16, bin X is
-                            // sum u (c - i s) and bin 8-X is sum (-1)^n u (c + i s):
-                            // split by n parity.  A row outside the tile's lines
-                            // holds black: u = 0.
-                            const uint4 q0 = h ? rows[1][0] : rows[0][0], q1 = h ? rows[1][1] : rows[0][1];
-                            const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-                            double rw = rowp[h] != nullptr ? c_tab.wz_d[rz[h]] * c_tab.wy_d[ry[h]] : 0.0;
-                            // opaque per bin: otherwise the 32 windowed samples
-                            // are hoisted out of the bin loop into registers
-                            asm volatile("" : "+v"(rw));
-                            double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
+                    for (int h = 0; h < 2; ++h) {
+                        const uint4 q0 = h ? rows[1][0] : rows[0][0], q1 = h ? rows[1][1] : rows[0][1];
+                        const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+                        double rw = (h ? rowp[1] : rowp[0]) != nullptr
+                                        ? c_tab.wz_d[h ? rz[1] : rz[0]] * c_tab.wy_d[h ? ry[1] : ry[0]]
+                                        : 0.0;
+                        // opaque per bin pair: otherwise the 32 windowed samples
+                        // are shared by the three pairs and held in registers
+                        asm volatile("" : "+v"(rw));
+                        double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
 #pragma unroll
-                            for (int n = 0; n < 16; n += 2) {
-                                const uint32_t w2 = raw[n >> 1];
-                                const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
-                                const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
-                                ce = fma(u0, kCos16[(X * n) & 15], ce);
-                                se = fma(u0, kSin16[(X * n) & 15], se);
-                                co = fma(u1, kCos16[(X * (n + 1)) & 15], co);
-                                so = fma(u1, kSin16[(X * (n + 1)) & 15], so);
-                            }
-                            const double ra = ce + co, ia = -(se + so);
-                            const double rb = ce - co, ib = se - so;
-                            // turn by the (ky, kz) phase of the row: 4 kz z + ky y (32ths)
-                            const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
-                            const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
-                            const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
-                            a0 += ra * ca + ia * sa;
-                            a1 += ia * ca - ra * sa;
-                            b0 += rb * cb + ib * sb;
-                            b1 += ib * cb - rb * sb;
+                        for (int n = 0; n < 16; n += 2) {
+                            const uint32_t w2 = raw[n >> 1];
+                            const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
+                            const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
+                            ce = fma(u0, kCos16[(X * n) & 15], ce);
+                            se = fma(u0, kSin16[(X * n) & 15], se);
+                            co = fma(u1, kCos16[(X * (n + 1)) & 15], co);
+                            so = fma(u1, kSin16[(X * (n + 1)) & 15], so);
                         }
-                        wave_sum4_d(a0, a1, b0, b1);
-                        if (lane == 0 && decide64(a0, a1, b0, b1, thr64[(z * 32 + y) * 3 + (X - 2)])) {
-                            L.keep[(X - 2) * 8 + z] |= 1u << y;
-                            L.keep[(XR - 2) * 8 + zr] |= 1u << yr;
+                        const double r0 = ce + co, i0 = -(se + so), r1 = ce - co, i1 = se - so;
+                        if (h == 0) {
+                            ra[0] = r0; ia[0] = i0; rb[0] = r1; ib[0] = i1;
+                        } else {
+                            ra[1] = r0; ia[1] = i0; rb[1] = r1; ib[1] = i1;
+                        }
+                    }
+                    while (zmask != 0u) {
+                        const int z = __builtin_ctz(zmask);
+                        zmask &= zmask - 1u;
+                        uint32_t bits = (uint32_t)__builtin_amdgcn_readlane((int)myflag, (X - 2) * 8 + z);
+                        const int zr = (10 - z) & 7;
+                        while (bits != 0u) {
+                            const int y = __builtin_ctz(bits);
+                            bits &= bits - 1u;
+                            const int yr = (40 - y) & 31;
+                            double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                // turn by the (ky, kz) phase of the row: 4 kz z + ky y (32ths)
+                                const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
+                                const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
+                                const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
+                                a0 += ra[h] * ca + ia[h] * sa;
+                                a1 += ia[h] * ca - ra[h] * sa;
+                                b0 += rb[h] * cb + ib[h] * sb;
+                                b1 += ib[h] * cb - rb[h] * sb;
+                            }
+                            wave_sum4_d(a0, a1, b0, b1);
+                            const double th =
+                                g.thr_uniform >= 0.0f ? g.thr_uniform_d : thr64[(z * 32 + y) * 3 + (X - 2)];
+                            if (lane == 0 && decide64(a0, a1, b0, b1, th)) {
+                                L.keep[(X - 2) * 8 + z] |= 1u << y;
+                                L.keep[(XR - 2) * 8 + zr] |= 1u << yr;
+                            }
                         }
                     }
                 };
-#pragma unroll 1
-                for (int z = 0; z < 8; ++z) {
-                    plane(std::integral_constant<int, 2>{}, z);
-                    plane(std::integral_constant<int, 3>{}, z);
-                    plane(std::integral_constant<int, 4>{}, z);
-                }
+                bins(std::integral_constant<int, 2>{});
+                bins(std::integral_constant<int, 3>{});
+                bins(std::integral_constant<int, 4>{});
                 __syncthreads();
             }
             // the next tile's rows, in flight through the inverse
@@ -1951,17 +1981,48 @@ namespace tpal {
 
 hipError_t upload_tables(const Tables &t) { return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof(Tables)); }
 
-template <int STRIP>
-static void launch3d_strip(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
-                           hipStream_t s, int mode, float2 *sp, const float *mask, int max_groups)
+// Tiles per work item when the caller leaves it open: a workgroup per item
+// (12 resident per CU at the tile kernel's footprint), each item costing its
+// owned tiles plus the leading one, so a launch takes about
+// ceil(items / resident) * (strip + 1) tile times.  Longer strips waste less
+// on leading tiles; shorter ones fill the chip when there are few slabs.
+// Strips are multiples of 4 tiles, so every strip's partial columns start on
+// a 128-byte line.
+int choose_strip3(const Geom &g, int n_jobs)
 {
-    const int items = n_jobs * g.n_ty3 * ((g.n_tx3 + STRIP - 1) / STRIP);
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            return n;
+        return 256;
+    }();
+    const int64_t resident = 12 * (int64_t)cus;
+    int best = kStrip3;
+    int64_t best_cost = INT64_MAX;
+    for (int strip : {4, 8, 12, 16, 24}) {
+        const int64_t items = (int64_t)n_jobs * g.n_ty3 * ((g.n_tx3 + strip - 1) / strip);
+        const int64_t cost = (items + resident - 1) / resident * (strip + 1);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = strip;
+        }
+    }
+    return best;
+}
+
+void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
+                   hipStream_t s, int mode, void *spec, const float *mask, int strip, int max_groups)
+{
+    float2 *sp = reinterpret_cast<float2 *>(spec);
+    if (strip <= 0) strip = choose_strip3(g, n_jobs);
+    const int items = n_jobs * g.n_ty3 * ((g.n_tx3 + strip - 1) / strip);
     if (items <= 0) return;
     // persistent grids are whole multiples of 8 (one share per XCD)
     const bool persist = max_groups > 0 && max_groups < items;
     const dim3 grid(persist ? std::max(8, max_groups & ~7) : items), blk(64);
 #define CHD_LAUNCH3D(M, P)                                                                                   \
-    hipLaunchKernelGGL((tpal3d_tiles<M, STRIP, P>), grid, blk, 0, s, g, jobs, n_jobs, thr32, thr64, sp, mask)
+    hipLaunchKernelGGL((tpal3d_tiles<M, P>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask)
     if (mode == kTileSpectra) {
         if (persist) CHD_LAUNCH3D(kTileSpectra, true); else CHD_LAUNCH3D(kTileSpectra, false);
     } else if (mode == kTileApply) {
@@ -1970,15 +2031,6 @@ static void launch3d_strip(const Geom &g, const SlabJob *jobs, int n_jobs, const
         if (persist) CHD_LAUNCH3D(kTilePlain, true); else CHD_LAUNCH3D(kTilePlain, false);
     }
 #undef CHD_LAUNCH3D
-}
-
-void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
-                   hipStream_t s, int mode, void *spec, const float *mask, int strip, int max_groups)
-{
-    float2 *sp = reinterpret_cast<float2 *>(spec);
-    if (strip == 4) launch3d_strip<4>(g, jobs, n_jobs, thr32, thr64, s, mode, sp, mask, max_groups);
-    else if (strip == 16) launch3d_strip<16>(g, jobs, n_jobs, thr32, thr64, s, mode, sp, mask, max_groups);
-    else launch3d_strip<kStrip3>(g, jobs, n_jobs, thr32, thr64, s, mode, sp, mask, max_groups);
 }
 
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
