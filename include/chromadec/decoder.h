@@ -1,9 +1,11 @@
 /* libchromadec-mi355x public ABI: decoders.
  *
  * Same kinds, option names, signatures and semantics as the reference's
- * include/chromadec/decoder.h:13-104 for the Transform PAL kinds.  Kinds
- * CHD_DEC_TRANSFORM_2D (7) and CHD_DEC_TRANSFORM_3D (8) decode on the GPU;
- * every other kind is rejected at commit with CHD_E_DECODER_UNKNOWN. */
+ * include/chromadec/decoder.h:13-104 for the PAL kinds.  CHD_DEC_PAL_2D (6),
+ * CHD_DEC_TRANSFORM_2D (7) and CHD_DEC_TRANSFORM_3D (8) decode on the GPU, and
+ * CHD_DEC_AUTO resolves to CHD_DEC_PAL_2D for a PAL source; CHD_DEC_NONE
+ * commits for geometry and dropout queries only.  Every other kind is
+ * rejected at commit with CHD_E_DECODER_UNKNOWN. */
 #ifndef CHROMADEC_DECODER_H
 #define CHROMADEC_DECODER_H
 
@@ -57,7 +59,7 @@ chd_status_t chd_decoder_get_output_info(const chd_decoder_t *d, chd_output_info
 /* Option names (decoder.h:58-88 of the reference). */
 #define CHD_OPT_CHROMA_GAIN                 "chroma_gain"               /* f64 */
 #define CHD_OPT_CHROMA_PHASE_DEG            "chroma_phase_deg"          /* f64 */
-#define CHD_OPT_LUMA_NR_LEVEL               "luma_nr_level"             /* f64; only 0 is supported */
+#define CHD_OPT_LUMA_NR_LEVEL               "luma_nr_level"             /* f64, IRE; 0 = off (luma coring) */
 #define CHD_OPT_PADDING_MULTIPLE            "padding_multiple"          /* i32, 1..1024 */
 #define CHD_OPT_REVERSE_FIELD_ORDER         "reverse_field_order"       /* bool */
 #define CHD_OPT_TRANSFORM_THRESHOLD         "transform_threshold"       /* f64, default 0.4 */
@@ -66,7 +68,7 @@ chd_status_t chd_decoder_get_output_info(const chd_decoder_t *d, chd_output_info
 #define CHD_OPT_LAST_ACTIVE_SAMPLE          "last_active_sample"        /* i32, inclusive */
 #define CHD_OPT_FIRST_ACTIVE_FRAME_LINE     "first_active_frame_line"   /* i32, inclusive */
 #define CHD_OPT_LAST_ACTIVE_FRAME_LINE      "last_active_frame_line"    /* i32, inclusive */
-#define CHD_OPT_OUTPUT_FORMAT               "output_format"             /* str: "yuv444p16"|"yuv444ps"|"gray16"|"grays" */
+#define CHD_OPT_OUTPUT_FORMAT               "output_format"             /* str: "yuv444p16"|"yuv444ps"|"gray16"|"grays"|"rgb48"|"rgbs" */
 #define CHD_OPT_OUTPUT_CLAMP                "output_clamp"              /* str: "none"|"legal_rgb_sdr"|"legal_rgb_hdr"|"legal_ycbcr_bt601" */
 #define CHD_OPT_COLOR_DIFFERENCE_PRECISION  "color_difference_precision"  /* str: "classic"|"modern" */
 #define CHD_OPT_BROADCAST_SCALING_PRECISION "broadcast_scaling_precision" /* str: "classic"|"modern"|"scientific" */

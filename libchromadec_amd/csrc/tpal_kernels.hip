@@ -173,9 +173,8 @@ __device__ __forceinline__ double wave_sum_d(double v)
 // for a tile whose windowed samples are all zero (a 2D padding step past its
 // strip, or a tile of black): every bin is then exactly zero and the pair is
 // kept (0 < 0 is false), as an FP64 evaluation decides.
-__device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
+__device__ __forceinline__ void decide32_nz(c32 a, c32 b, float e, float th, bool &keep, bool &unsure)
 {
-    if (e == 0.0f) return 1;
     const float ma = a.x * a.x + a.y * a.y;
     const float mb = b.x * b.x + b.y * b.y;
     // |a| <= max(|a.x|,|a.y|) + (sqrt 2 - 1) min(|a.x|,|a.y|)
@@ -189,9 +188,65 @@ __device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
     const float d1 = fmaf(db, th, da) + r1;
     const float d2 = fmaf(da, th, db) + r2;
     // bitwise ors: no short-circuit branches in the unrolled mask loop
-    const bool unsure = (fabsf(g1) <= d1) | (fabsf(g2) <= d2);
+    unsure = (fabsf(g1) <= d1) | (fabsf(g2) <= d2);
     const bool discard = (g1 < 0.0f) | (g2 < 0.0f);
-    return unsure ? -1 : (discard ? 0 : 1);
+    keep = !unsure & !discard;
+}
+
+// max / min of |x| and |y| in one instruction each.  The compiler's fmaxf
+// first canonicalises each input (IEEE mode), which costs two more; the bins
+// here are never NaN, so the canonicalisation changes nothing.
+__device__ __forceinline__ float absmax(float x, float y)
+{
+    float r;
+    asm("v_max_f32 %0, |%1|, |%2|" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+}
+__device__ __forceinline__ float absmin(float x, float y)
+{
+    float r;
+    asm("v_min_f32 %0, |%1|, |%2|" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+}
+
+// The terms of decide32_nz's bounds that depend only on the tile's e and the
+// threshold th: d1 = 2e na + 2e th nb + e^2 (1 + th) + 4u ma + 4u th mb, and
+// d2 the same with a and b swapped.
+struct DecideConst {
+    float th, c2e, c2eth, e2th1, u4th;
+};
+__device__ __forceinline__ DecideConst decide_const(float e, float th)
+{
+    return DecideConst{th, 2.0f * e, 2.0f * e * th, e * e * (1.0f + th), 4.0f * kU * th};
+}
+
+// decide32_nz as wave ballots (taken on the comparisons themselves, so that no
+// per-lane bool is materialised): keep and unsure lane masks.
+__device__ __forceinline__ void decide32_ballots(c32 a, c32 b, const DecideConst &q, uint64_t &keep,
+                                                 uint64_t &unsure)
+{
+    const float ma = fmaf(a.x, a.x, a.y * a.y);
+    const float mb = fmaf(b.x, b.x, b.y * b.y);
+    // |a| <= max(|a.x|,|a.y|) + (sqrt 2 - 1) min(|a.x|,|a.y|)
+    const float na = fmaf(0.41421357f, absmin(a.x, a.y), absmax(a.x, a.y));
+    const float nb = fmaf(0.41421357f, absmin(b.x, b.y), absmax(b.x, b.y));
+    const float g1 = fmaf(-q.th, mb, ma), g2 = fmaf(-q.th, ma, mb);
+    const float d1 = fmaf(q.c2e, na, fmaf(q.c2eth, nb, fmaf(4.0f * kU, ma, fmaf(q.u4th, mb, q.e2th1))));
+    const float d2 = fmaf(q.c2e, nb, fmaf(q.c2eth, na, fmaf(4.0f * kU, mb, fmaf(q.u4th, ma, q.e2th1))));
+    // one ballot per comparison, combined as scalar masks
+    unsure = __builtin_amdgcn_ballot_w64(fabsf(g1) <= d1) | __builtin_amdgcn_ballot_w64(fabsf(g2) <= d2);
+    const uint64_t discard = __builtin_amdgcn_ballot_w64(fminf(g1, g2) < 0.0f);
+    keep = __builtin_amdgcn_read_exec() & ~(unsure | discard);
+}
+
+// decide32_nz as one value: 1 keep, 0 discard, -1 unsure; a tile with e = 0
+// keeps every pair.
+__device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
+{
+    if (e == 0.0f) return 1;
+    bool keep, unsure;
+    decide32_nz(a, b, e, th, keep, unsure);
+    return unsure ? -1 : (keep ? 1 : 0);
 }
 
 // cos / sin (2 pi j / 16) with the exact values at multiples of pi/2.
@@ -211,6 +266,31 @@ __device__ __forceinline__ void wave_sum4_d(double &a, double &b, double &c, dou
         c += tc;
         d += td;
     }
+}
+
+// One DPP move of a double (its two halves); rows or banks outside the masks
+// read 0.
+template <int CTRL, int ROWMASK = 0xF, int BANKMASK = 0xF>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, BANKMASK, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, BANKMASK, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Four wave sums into lane 63 with DPP moves, no LDS round trips: pairs
+// (quad_perm 1,0,3,2), quads (2,3,0,1), half rows (row_half_mirror), rows
+// (row_mirror), then row 1 += row 0 and row 3 += row 2 (row_bcast:15), and
+// row 3 += rows 0-1 (row_bcast:31).  Other lanes hold partial sums.
+__device__ __forceinline__ void wave_sum4_to63(double &a, double &b, double &c, double &d)
+{
+    a += dpp_d<0xB1>(a); b += dpp_d<0xB1>(b); c += dpp_d<0xB1>(c); d += dpp_d<0xB1>(d);
+    a += dpp_d<0x4E>(a); b += dpp_d<0x4E>(b); c += dpp_d<0x4E>(c); d += dpp_d<0x4E>(d);
+    a += dpp_d<0x141>(a); b += dpp_d<0x141>(b); c += dpp_d<0x141>(c); d += dpp_d<0x141>(d);
+    a += dpp_d<0x140>(a); b += dpp_d<0x140>(b); c += dpp_d<0x140>(c); d += dpp_d<0x140>(d);
+    a += dpp_d<0x142, 0xA>(a); b += dpp_d<0x142, 0xA>(b); c += dpp_d<0x142, 0xA>(c); d += dpp_d<0x142, 0xA>(d);
+    a += dpp_d<0x143, 0xC>(a); b += dpp_d<0x143, 0xC>(b); c += dpp_d<0x143, 0xC>(c); d += dpp_d<0x143, 0xC>(d);
 }
 
 __device__ __forceinline__ int decide64(double ar, double ai, double br, double bi, double th)
@@ -240,9 +320,12 @@ struct Lds3 {
         c32 S[5 * kSk];       // spectrum S[k*kSk + ky*kSpad + kz]
     } u;
     uint32_t keep[40];        // word k*8+kz, bit ky: bin (x = k+2, ky, kz) kept
-    uint32_t bal[24];         // word (x-2)*8+z, bit y: visit (z, y, x) keeps its pair
-    uint32_t flg[24];         // word (x-2)*8+z, bit y: visit undecidable in FP32
+    // mask step words, one 64-bit ballot per step (x, z < 4): word pw(x-2, z)
+    // (below), bit y: visit (z, y, x) keeps its pair / is undecidable in FP32
+    alignas(8) uint32_t bal[24];
+    alignas(8) uint32_t flg[24];
     double cs[32], sn[32];    // cos/sin(2 pi j / 32) for the FP64 re-check
+    double wz[8], wy[32];     // its z and y window factors
 };
 
 // Apply mode holds the tile's soft mask (one value per visit) as well.
@@ -262,6 +345,10 @@ __device__ __forceinline__ int mask_visit(int k, int ky, int kz)
     if (k <= 2) return (kz * 32 + ky) * 3 + k;
     return (((10 - kz) & 7) * 32 + ((40 - ky) & 31)) * 3 + (4 - k);
 }
+
+// Word of plane (k = x - 2, z) in the mask step arrays: the two planes a step
+// decides, (k, z) and (k, z + 4), are one 64-bit word.
+__device__ __forceinline__ constexpr int pw(int k, int z) { return (k * 4 + (z & 3)) * 2 + (z >> 2); }
 
 // Bit y of w moved to bit (8 - y) mod 32: the mirror row yr = (40 - y) % 32
 // of transform_pal_3d.cpp:291, applied to a word of 32 bins along ky.
@@ -317,18 +404,19 @@ __device__ __forceinline__ float xdft16(const float (&s)[16], c32 (&X)[5])
     const float a = (g1 - g3) * r, b = (g1 + g3) * r;
     X[0] = c32{g0 + a, -(g2 + b)};
     X[4] = c32{g0 - a, g2 - b};
-    // bins 3 and 5
+    // bins 3 and 5: with t = 2 pi 3 n / 16, bin 5's twiddle is (-1)^n times
+    // the conjugate of bin 3's, so both come from the even-n and odd-n sums
+    // of o[n] cos t and -o[n] sin t
+    float ce = 0.0f, co = 0.0f, se = 0.0f, so = 0.0f;
 #pragma unroll
-    for (int j = 1; j <= 3; j += 2) {
-        const int k = j + 2;
-        float re = 0.0f, im = 0.0f;
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            re = cmac(re, o[n], cos32(2 * k * n));
-            im = cmac(im, o[n], -sin32(2 * k * n));
-        }
-        X[j] = c32{re, im};
+    for (int n = 0; n < 8; n += 2) {
+        ce = cmac(ce, o[n], cos32(6 * n));
+        se = cmac(se, o[n], -sin32(6 * n));
+        co = cmac(co, o[n + 1], cos32(6 * (n + 1)));
+        so = cmac(so, o[n + 1], -sin32(6 * (n + 1)));
     }
+    X[1] = c32{ce + co, se + so};
+    X[3] = c32{ce - co, so - se};
     return l1;
 }
 
@@ -374,6 +462,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     if (lane < 32) {
         L.cs[lane] = c_tab.cos32_d[lane];
         L.sn[lane] = c_tab.sin32_d[lane];
+        L.wy[lane] = c_tab.wy_d[lane];
+        if (lane < 8) L.wz[lane] = c_tab.wz_d[lane];
     }
     // Workgroups are dispatched round-robin over the 8 XCDs, each with its
     // own L2.  Give each XCD a contiguous share of the items, ordered slab
@@ -401,6 +491,12 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     int par[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) par[z] = (job.offset[z] - tileY) & 1;
+    // the same as a scalar bit mask (bit z: plane z holds odd lines), for
+    // uniform branches
+    int pbits = 0;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) pbits |= par[z] << z;
+    pbits = __builtin_amdgcn_readfirstlane(pbits);
 
     const int tx_own0 = strip * strip_len;
     const int tx_end = min(tx_own0 + strip_len, g.n_tx3);
@@ -428,12 +524,13 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     // capture): row ky+16 of plane z is (-1)^par[z] = sgn (-1)^z times row ky,
     // so after the temporal DFT T[ky+16][kz] = sgn T[ky][(kz+4) & 7] and only
     // rows 0..15 need the vertical stores and the temporal DFT.
-    bool alt = true;
-#pragma unroll
-    for (int z = 1; z < 8; ++z) alt = alt && par[z] == (par[0] ^ (z & 1));
+    const bool alt = pbits == 0xAA || pbits == 0x55;
     const float sgn = par[0] ? -1.0f : 1.0f;
-    const int izs = min(lane / 5, 3);
-    const bool iy_odd = par[(izs < 2) ? izs : izs + 2] != 0;
+    // exp(-2 pi i ky / 32) for this lane's temporal-stage ky (lane & 15; exact
+    // at the quarter turns)
+    const c32 ptw = {cos32(lane & 15), -sin32(lane & 15)};
+    // exp(+2 pi i ky / 32) for this lane's inverse-z ky (lane & 31)
+    const c32 itw = {cos32(lane & 31), sin32(lane & 31)};
     // mask-stage role: bins y = lane & 31; lanes 0-31 visit x = 2 (all z) and
     // x = 3 (z < 4), lanes 32-63 visit x = 3 (z >= 4) and x = 4 (all z)
     const int vy = lane & 31, vhalf = lane >> 5;
@@ -505,9 +602,15 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 for (int j = 0; j < 5; ++j) L.u.A[rz[h] * 5 + j][m] = c32{X[j].x * roww[h], X[j].y * roww[h]};
             }
             const float err = kErrK * kU * wave_sum(l1);
+            // the same in every lane: a scalar, so that the mask steps branch
+            // on it uniformly
+            const float erru = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(err)));
             __syncthreads();
 
             // ---- vertical: 16-point DFT of the data lines + parity twiddle ---
+            // (with alternating line parity the twiddle of the odd-parity
+            // planes is applied in the temporal stage instead: there a lane
+            // holds one ky, and only 4 of its 8 values need it)
             {
                 c32 v[16];
                 const bool act = lane < 40;
@@ -519,13 +622,18 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 if (act) {
                     const int z = lane / 5, k = lane - 5 * (lane / 5);
                     fft<16, -1>(v);
-                    const bool odd = fy_odd;
+                    if (alt) {
 #pragma unroll
-                    for (int ky = 0; ky < 16; ++ky) {
-                        c32 x0 = v[ky];
-                        if (odd) x0 = twiddle<-1>(x0, ky);
-                        Sat(L, k, ky, z) = x0;
-                        if (!alt) Sat(L, k, ky + 16, z) = odd ? c32{-x0.x, -x0.y} : x0;
+                        for (int ky = 0; ky < 16; ++ky) Sat(L, k, ky, z) = v[ky];
+                    } else {
+                        const bool odd = fy_odd;
+#pragma unroll
+                        for (int ky = 0; ky < 16; ++ky) {
+                            c32 x0 = v[ky];
+                            if (odd) x0 = twiddle<-1>(x0, ky);
+                            Sat(L, k, ky, z) = x0;
+                            Sat(L, k, ky + 16, z) = odd ? c32{-x0.x, -x0.y} : x0;
+                        }
                     }
                 }
             }
@@ -538,6 +646,20 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                     c32 v[8];
 #pragma unroll
                     for (int z = 0; z < 8; ++z) v[z] = Sat(L, k, ky, z);
+                    // parity twiddle exp(-2 pi i ky / 32) of the odd-parity planes
+                    // (odd z when plane 0 has even parity, even z otherwise)
+                    auto turn = [&](c32 b) {
+                        return c32{b.x * ptw.x - b.y * ptw.y, b.x * ptw.y + b.y * ptw.x};
+                    };
+                    if (pbits & 1) {
+                        asm volatile("");   // a scalar branch, not selects
+#pragma unroll
+                        for (int z = 0; z < 8; z += 2) v[z] = turn(v[z]);
+                    } else {
+                        asm volatile("");
+#pragma unroll
+                        for (int z = 1; z < 8; z += 2) v[z] = turn(v[z]);
+                    }
                     fft<8, -1>(v);
 #pragma unroll
                     for (int z = 0; z < 8; ++z) {
@@ -601,6 +723,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                     // step 8's ballot moved to the mirror rows, and its pairs
                     // are re-decided (if at all) through plane (4, 0) / (4, 4).
                     constexpr int kSteps = kUniform ? 11 : 12;
+                    const DecideConst qu = decide_const(erru, g.thr_uniform);
                     auto step_of = [](int j) { return (kUniform && j >= 10) ? j + 1 : j; };
                     c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
 #pragma unroll
@@ -612,18 +735,18 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                             bn = L.u.S[off_b(step_of(j + 1))];
                         }
                         const int k = i >> 2, z = i & 3;
-                        const float th = kUniform ? g.thr_uniform : thr32[z * 96 + k + tho];
-                        const int d = decide32(a, b, err, th);
-                        const uint64_t kb = __ballot(d > 0);
-                        const uint64_t fb = __ballot(d < 0);
-                        if (vy == 0) {
-                            const int w = k * 8 + z + 4 * vhalf;
-                            const uint32_t kw = vhalf ? (uint32_t)(kb >> 32) : (uint32_t)kb;
-                            L.bal[w] = kw;
-                            L.flg[w] = vhalf ? (uint32_t)(fb >> 32) : (uint32_t)fb;
+                        const DecideConst q = kUniform ? qu : decide_const(erru, thr32[z * 96 + k + tho]);
+                        uint64_t kb, fb;
+                        decide32_ballots(a, b, q, kb, fb);
+                        if (lane == 0) {
+                            // planes (k, z) on lanes 0-31 and (k, z + 4) on lanes 32-63
+                            *reinterpret_cast<uint64_t *>(&L.bal[pw(k, z)]) = kb;
+                            *reinterpret_cast<uint64_t *>(&L.flg[pw(k, z)]) = fb;
                             if (kUniform && i == 8) {
-                                L.bal[w + 2] = mirror_rows(kw);
-                                L.flg[w + 2] = 0u;
+                                const uint64_t m = mirror_rows((uint32_t)kb) |
+                                                   ((uint64_t)mirror_rows((uint32_t)(kb >> 32)) << 32);
+                                *reinterpret_cast<uint64_t *>(&L.bal[pw(k, 2)]) = m;
+                                *reinterpret_cast<uint64_t *>(&L.flg[pw(k, 2)]) = 0u;
                             }
                         }
                         a = an;
@@ -631,8 +754,18 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                         __builtin_amdgcn_sched_barrier(0);
                     }
                 };
-                if (g.thr_uniform >= 0.0f) steps(std::true_type{});
-                else steps(std::false_type{});
+                if (erru == 0.0f) {
+                    // every windowed sample is zero, so is every bin: all pairs
+                    // are kept (0 < 0 is false), none needs re-deciding
+                    if (lane < 24) {
+                        L.bal[lane] = ~0u;
+                        L.flg[lane] = 0u;
+                    }
+                } else if (g.thr_uniform >= 0.0f) {
+                    steps(std::true_type{});
+                } else {
+                    steps(std::false_type{});
+                }
             }
             __syncthreads();
             // keep word (k, z) = own visits' ballot | ballot of the visit plane
@@ -642,14 +775,14 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             constexpr uint32_t kSelf = (1u << 4) | (1u << 20);
             if (lane < 40) {
                 const int k = lane >> 3, z = lane & 7;
-                uint32_t w = (k <= 2) ? L.bal[lane] : 0u;
-                if (k >= 2) w |= mirror_rows(L.bal[(4 - k) * 8 + ((10 - z) & 7)]);
+                uint32_t w = (k <= 2) ? L.bal[pw(k, z)] : 0u;
+                if (k >= 2) w |= mirror_rows(L.bal[pw(4 - k, (10 - z) & 7)]);
                 if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
                 L.keep[lane] = w;
             }
             // flag words (x - 2) * 8 + z on lanes 0-23, self-mirror bins removed
             const uint32_t myflag =
-                lane < 24 ? (L.flg[lane] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
+                lane < 24 ? (L.flg[pw(lane >> 3, lane & 7)] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
             const uint64_t fwords = __ballot(myflag != 0u);
             __syncthreads();
 
@@ -676,7 +809,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                         const uint4 q0 = h ? rows[1][0] : rows[0][0], q1 = h ? rows[1][1] : rows[0][1];
                         const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
                         double rw = (h ? rowp[1] : rowp[0]) != nullptr
-                                        ? c_tab.wz_d[h ? rz[1] : rz[0]] * c_tab.wy_d[h ? ry[1] : ry[0]]
+                                        ? L.wz[h ? rz[1] : rz[0]] * L.wy[h ? ry[1] : ry[0]]
                                         : 0.0;
                         // opaque per bin pair: otherwise the 32 windowed samples
                         // are shared by the three pairs and held in registers
@@ -720,10 +853,10 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                                 b0 += rb[h] * cb + ib[h] * sb;
                                 b1 += ib[h] * cb - rb[h] * sb;
                             }
-                            wave_sum4_d(a0, a1, b0, b1);
+                            wave_sum4_to63(a0, a1, b0, b1);
                             const double th =
                                 g.thr_uniform >= 0.0f ? g.thr_uniform_d : thr64[(z * 32 + y) * 3 + (X - 2)];
-                            if (lane == 0 && decide64(a0, a1, b0, b1, th)) {
+                            if (lane == 63 && decide64(a0, a1, b0, b1, th)) {
                                 L.keep[(X - 2) * 8 + z] |= 1u << y;
                                 L.keep[(XR - 2) * 8 + zr] |= 1u << yr;
                             }
@@ -764,6 +897,29 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 }
             }
             fft<8, +1>(v);
+            // Output planes of odd line parity are turned by exp(2 pi i ky / 32)
+            // here, where the lane holds one ky: the inverse-y fold then adds
+            // rows ky and ky + 16 for every plane (the turn of ky + 16 is minus
+            // that of ky).
+            auto turn = [&](c32 b) { return c32{b.x * itw.x - b.y * itw.y, b.x * itw.y + b.y * itw.x}; };
+            // (scalar branches; the empty asm keeps the compiler from turning
+            // them into selects over all four planes)
+            if (alt) {
+                if (pbits & 1) {
+                    asm volatile("");
+                    v[0] = turn(v[0]);
+                    v[4] = turn(v[4]);
+                } else {
+                    asm volatile("");
+                    v[1] = turn(v[1]);
+                    v[5] = turn(v[5]);
+                }
+            } else {
+                if (pbits & 0x01) v[0] = turn(v[0]);
+                if (pbits & 0x02) v[1] = turn(v[1]);
+                if (pbits & 0x10) v[4] = turn(v[4]);
+                if (pbits & 0x20) v[5] = turn(v[5]);
+            }
             Sat(L, k, ky, 0) = v[0];
             Sat(L, k, ky, 1) = v[1];
             Sat(L, k, ky, 2) = v[4];
@@ -771,20 +927,14 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         }
         __syncthreads();
 
-        // ---- inverse y: fold 32 -> 16 bins for the plane's line parity ----
+        // ---- inverse y: fold 32 -> 16 bins (rows ky, ky + 16 of the plane) ----
         {
             c32 v[16];
             const bool act = lane < 20;
             const int zs = lane / 5, k = lane - 5 * (lane / 5);
             if (act) {
-                const bool odd = iy_odd;
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const c32 lo = Sat(L, k, j, zs), hi = Sat(L, k, j + 16, zs);
-                    c32 t = odd ? csub(lo, hi) : cadd(lo, hi);
-                    if (odd) t = twiddle<+1>(t, j);
-                    v[j] = t;
-                }
+                for (int j = 0; j < 16; ++j) v[j] = cadd(Sat(L, k, j, zs), Sat(L, k, j + 16, zs));
             }
             __syncthreads();  // A and S share storage
             if (act) {
