@@ -96,28 +96,88 @@ def make_batch_workload(seed: int = 3):
     return seqs
 
 
-def cpu_baseline(fields: np.ndarray, n_sample: int, gpu_frames=None):
-    """Times the FP64 oracle on the first n_sample frames; when the GPU's
-    frames of the same indices are given, also reports their error against
-    the oracle's (the oracle as the checker, not the thing measured)."""
+def host_cpu():
+    """The host the CPU baseline runs on: logical CPUs, the CPUs this process
+    may use (its affinity mask, which the box sets to its share), the OpenMP
+    thread limit and the CPU model name (/proc/cpuinfo, as lscpu reports it)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"nproc": os.cpu_count(), "affinity_cpus": avail, "omp_num_threads": omp or None, "model": model}
+
+
+def _rates(n_frames, seconds):
+    r = [n_frames / t for t in seconds]
+    return float(np.mean(r)), float(np.median(r))
+
+
+def cpu_baseline(fields: np.ndarray, n_sample: int, gpu_frames=None, c1_fields=None):
+    """The FP64 oracle (a restatement of the reference algorithm) timed on the
+    host, in the pattern of examples/nn_benchmark.c: warm-up kept apart, then
+    mean and median frames/s.  Threads: every CPU this process may use (the
+    reference's thread_count = 0 default is hardware_concurrency), capped by
+    OMP_NUM_THREADS where the host sets it.  When the GPU's frames of the same
+    indices are given, also reports their error against the oracle's (the
+    oracle as the checker, not the thing measured)."""
     from oracle import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    threads = host["affinity_cpus"]
+    if host["omp_num_threads"]:
+        threads = min(threads, host["omp_num_threads"])
     p = orc.pal_params()
+    thr3 = orc.uniform_thresholds(8)
     frames = list(range(n_sample))
-    t0 = time.perf_counter()
-    want = orc.decode_sequence(p, 8, fields, frames, orc.uniform_thresholds(8), 0, threads=threads)
-    dt = time.perf_counter() - t0
-    # one core as well (the reference's thread_count=1 decode loop), on 2 frames
-    n1 = min(2, n_sample)
-    t1 = time.perf_counter()
-    orc.decode_sequence(p, 8, fields, list(range(n1)), orc.uniform_thresholds(8), 0, threads=1)
-    dt1 = time.perf_counter() - t1
-    res = {"value": n_sample / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+    orc.decode_sequence(p, 8, fields, frames[:min(2, n_sample)], thr3, 0, threads=threads)   # warm-up
+    walls, want = [], None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        want = orc.decode_sequence(p, 8, fields, frames, thr3, 0, threads=threads)
+        walls.append(time.perf_counter() - t0)
+    mean, median = _rates(n_sample, walls)
+    # one core: the reference's thread_count = 1 chd_decode_frame loop
+    orc.decode_sequence(p, 8, fields, [0], thr3, 0, threads=1)                              # warm-up
+    per = []
+    for f in range(1, 5):
+        t1 = time.perf_counter()
+        orc.decode_sequence(p, 8, fields, [f], thr3, 0, threads=1)
+        per.append(time.perf_counter() - t1)
+    mean1, median1 = _rates(1, per)
+    res = {"value": median, "unit": "frames/s", "cores": threads, "kind": "port",
            "sample": f"{n_sample} frames of the same 64-frame sequence, FP64 oracle "
-                     f"(reference-equivalent TransformPal3D+PalColour+OutputWriter, radix-2 FFT), "
-                     f"one frame per OpenMP thread, {dt:.1f} s wall",
-           "one_core": {"value": n1 / dt1, "unit": "frames/s", "cores": 1,
-                        "sample": f"frames 0-{n1 - 1} on one thread, {dt1:.1f} s wall"}}
+                     f"(reference-equivalent TransformPal3D+PalColour+OutputWriter, radix-2 FFT; FFTW is absent "
+                     f"here), one frame per OpenMP thread on {threads} threads; median of 3 timed runs "
+                     f"after a 2-frame warm-up, {sum(walls):.1f} s wall",
+           "mean": mean, "median": median, "host": host,
+           "one_core": {"value": median1, "mean": mean1, "median": median1, "unit": "frames/s", "cores": 1,
+                        "sample": "frames 1-4 one at a time after a warm-up frame, one thread"}}
+    if c1_fields is not None:
+        # C1: Transform-2D on one bars + gradient frame
+        thr2 = orc.uniform_thresholds(7)
+        orc.decode_sequence(p, 7, c1_fields, [0], thr2, 0, threads=1)                       # warm-up
+        per2 = []
+        for _ in range(8):
+            t2 = time.perf_counter()
+            orc.decode_sequence(p, 7, c1_fields, [0], thr2, 0, threads=1)
+            per2.append(time.perf_counter() - t2)
+        m2, md2 = _rates(1, per2)
+        reps = [0] * threads
+        t3 = time.perf_counter()
+        orc.decode_sequence(p, 7, c1_fields, reps, thr2, 0, threads=threads)
+        dt3 = time.perf_counter() - t3
+        res["transform_2d_c1"] = {"one_core": {"mean": m2, "median": md2, "unit": "frames/s",
+                                               "sample": "8 decodes of the one C1 frame after a warm-up"},
+                                  "threads": {"value": threads / dt3, "unit": "frames/s", "cores": threads,
+                                              "sample": f"the C1 frame decoded {threads} times, one per thread"}}
     parity = None
     if gpu_frames is not None:
         d = gpu_frames.astype(np.float64) - want.astype(np.float64)
@@ -260,7 +320,9 @@ def main() -> int:
             plan.decode(seqs, out.data_ptr(), stream)
             torch.cuda.synchronize()
             gpu_frames = out[:args.cpu_sample].view(-1, 3, 576, 922).cpu().numpy()
-        res["cpu_baseline"], res["parity_vs_fp64_oracle"] = cpu_baseline(fields, args.cpu_sample, gpu_frames)
+        from libchromadec_amd import synth
+        c1 = synth.encode_sequence([synth.bars_gradient(0)])
+        res["cpu_baseline"], res["parity_vs_fp64_oracle"] = cpu_baseline(fields, args.cpu_sample, gpu_frames, c1)
     print(json.dumps(res))
     return 0
 
@@ -391,18 +453,30 @@ def batched_leg(steps):
     batch = abi.TpalBatch(plan, runs, out.data_ptr(), autotune=True)
     strip, groups, tile_ms = batch.config()
     stream = torch.cuda.current_stream().cuda_stream
-    batch.run(stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+
+    def replay_ms(n):
         batch.run(stream)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            batch.run(stream)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t) / n
+
+    # both grids at the chosen strip length, then the autotuner's pick
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    shapes = {}
+    for name, g in (("one_per_item", 0), ("persistent_12_per_cu", 12 * cus)):
+        batch.set_shape(strip, g)
+        shapes[name] = round(replay_ms(max(3, steps // 2)), 3)
+    batch.set_shape(strip, groups)
+    dt = replay_ms(steps) * steps / 1e3
     return {"value": round(total * steps / dt, 1), "unit": "frames/s", "frames_per_launch": total,
             "sequences": len(seqs), "lengths": [r[3] for r in runs], "ms_per_graph": round(1e3 * dt / steps, 3),
             "autotuned": {"strip_tiles": strip, "workgroups": groups,
                           "grid": "one per work item" if groups == 0 else "persistent",
-                          "tile_launch_ms": round(tile_ms, 3)}}
+                          "tile_launch_ms": round(tile_ms, 3)},
+            "ms_per_graph_by_grid": shapes}
 
 
 if __name__ == "__main__":

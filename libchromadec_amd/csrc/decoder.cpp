@@ -300,12 +300,19 @@ Worker *get_worker(chd_decoder_t *d, size_t idx)
 // Make `number` (1-based) resident in the worker's field cache and return its
 // packed device pointer.  `pinned` marks slots the current window uses.
 // Device copy of field `number` (1-based), read from the file, or, when
-// `corrected` is given, that host copy (already shifted and concealed).
+// `corrected` is given, that host copy (already shifted and concealed).  A
+// concealed field depends on the frame it was concealed in, so its key
+// holds the partner field: with both fields marked first, one field can be
+// the second field of one frame and the first of the next.
 const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool shifted, int stage_idx,
-                            std::vector<int> &pinned, chd_status_t &st, const uint16_t *corrected = nullptr)
+                            std::vector<int> &pinned, chd_status_t &st, const uint16_t *corrected = nullptr,
+                            int32_t partner = 0)
 {
     const tpal::Geom &g = w.plan->g;
-    const int64_t key = ((int64_t)number * 2 + (shifted ? 1 : 0)) * 2 + (corrected ? 1 : 0);
+    // bit 0: concealed copy; bit 1: shifted; the field number above, and for
+    // a concealed copy the partner field number in the upper 32 bits
+    const int64_t key = corrected ? (((int64_t)partner << 32) | ((int64_t)number << 2) | (shifted ? 2 : 0) | 1)
+                                  : (((int64_t)number << 2) | (shifted ? 2 : 0));
     w.tick++;
     for (int s = 0; s < kCacheSlots; ++s) {
         if (w.slot_key[s] == key) {
@@ -485,9 +492,9 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
             auto c = corrected_frame(d, f1, f2, shiftSecond, dopts, drev, d->dropParams, st);
             if (st != CHD_OK) return st;
             if (i == 2 * behind) frameStats = c->stats;
-            win.field[i] = cache_field(d, w, f1, false, i, pinned, st, c->first.data());
+            win.field[i] = cache_field(d, w, f1, false, i, pinned, st, c->first.data(), f2);
             if (st != CHD_OK) return st;
-            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, i + 1, pinned, st, c->second.data());
+            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, i + 1, pinned, st, c->second.data(), f1);
             if (st != CHD_OK) return st;
         } else {
             win.field[i] = cache_field(d, w, f1, false, i, pinned, st);
@@ -496,8 +503,9 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
             if (st != CHD_OK) return st;
         }
     }
-    // The decoded frame's own fields are always real: a black padding frame
-    // can only appear in the look-around.
+    // The decoded frame's own fields are black (nullptr) when the metadata or
+    // the .tbc do not cover it, as in SourceField::loadFields; the kernels
+    // decode a nullptr field as black.
     win.out = w.d_out;
     st = tpal_decode_windows(w.plan, &win, 1, w.stream);
     if (st != CHD_OK) return st;

@@ -1032,13 +1032,15 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device);
         if (cus <= 0) cus = 256;
         const int strips[5] = {4, 8, 12, 16, 24};
-        const int per_cu[6] = {0, 2, 3, 4, 6, 8};
+        // 0: one workgroup per work item; otherwise persistent workgroups per
+        // CU (12 fill the CU at the tile kernel's 3 waves per SIMD)
+        const int per_cu[5] = {0, 4, 6, 8, 12};
         hipEvent_t t0 = nullptr, t1 = nullptr;
         e = hipEventCreate(&t0);
         if (e == hipSuccess) e = hipEventCreate(&t1);
         double best = 1e30;
         for (int si = 0; si < 5 && e == hipSuccess; ++si) {
-            for (int pi = 0; pi < 6 && e == hipSuccess; ++pi) {
+            for (int pi = 0; pi < 5 && e == hipSuccess; ++pi) {
                 const int groups = per_cu[pi] * cus;
                 batch_tiles(b, b->cap, strips[si], groups);   // warm-up
                 double t = 1e30;

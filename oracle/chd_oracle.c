@@ -947,6 +947,69 @@ int orc_decode_window(const orc_params *p, int32_t kind, const double *thr_sq, c
                          yuv_out);
 }
 
+/* The forward half of tpal3d_filter for one Z tile (forwardFFTTile,
+ * transform_pal_3d.cpp:168-202) on every tile of its lattice, for fields
+ * given as doubles (the composite, or a true chroma signal): tile origins
+ * as tpal3d_filter's loops, lines outside the active lines or of the other
+ * field parity filled with `fill` (black16bIre for a composite, 0 for
+ * chroma; a constant line adds nothing at x bins 2..6, the window's x
+ * spectrum being bins 0 and +-1).  fields[0..7] are the tile's fields z.
+ * out_re/out_im receive, per tile in tile loop order (y then x), the bins
+ * x = 2..6 as [k = x - 2][ky][kz] (5 x 32 x 8).                           */
+int orc_tile_spectra3d(const orc_params *p, const double *const *fields, const int32_t *offsets, double fill,
+                       double *out_re, double *out_im)
+{
+    const int fw = p->field_width;
+    double (*win)[YT3][XT3] = malloc(sizeof(double) * ZT3 * YT3 * XT3);
+    for (int z = 0; z < ZT3; z++)
+        for (int y = 0; y < YT3; y++)
+            for (int x = 0; x < XT3; x++)
+                win[z][y][x] = orc_window(z, ZT3) * orc_window(y, YT3) * orc_window(x, XT3);
+    double *real = malloc(sizeof(double) * ZT3 * YT3 * XT3);
+    double *sre = malloc(sizeof(double) * ZT3 * YT3 * XT3);
+    double *sim = malloc(sizeof(double) * ZT3 * YT3 * XT3);
+    double *h_re = malloc(sizeof(double) * ZT3 * YT3 * XC3);
+    double *h_im = malloc(sizeof(double) * ZT3 * YT3 * XC3);
+    const int first = p->first_active_frame_line, last = p->last_active_frame_line;
+    const int avs = p->active_video_start, ave = p->active_video_end;
+    size_t t = 0;
+    for (int tileY = first - YT3 / 2; tileY <= last; tileY += YT3 / 2) {
+        for (int tileX = avs - XT3 / 2; tileX < ave; tileX += XT3 / 2, t++) {
+            const int startY = imax(first - tileY, 0);
+            const int endY = imin((last - tileY) + 1, YT3);
+            for (int z = 0; z < ZT3; z++) {
+                for (int y = 0; y < YT3; y++) {
+                    double *dst = real + (z * YT3 + y) * XT3;
+                    if (y < startY || y >= endY || ((tileY + y) % 2) != offsets[z]) {
+                        for (int x = 0; x < XT3; x++) dst[x] = fill * win[z][y][x];
+                        continue;
+                    }
+                    const double *b = fields[z] + (size_t)((tileY + y) / 2) * fw;
+                    for (int x = 0; x < XT3; x++) {
+                        const int sx = tileX + x;
+                        dst[x] = ((sx >= 0 && sx < fw) ? b[sx] : fill) * win[z][y][x];
+                    }
+                }
+            }
+            orc_r2c3d(real, ZT3, YT3, XT3, h_re, h_im, sre, sim);
+            double *ore = out_re + t * 5 * YT3 * ZT3, *oim = out_im + t * 5 * YT3 * ZT3;
+            for (int k = 0; k < 5; k++)
+                for (int ky = 0; ky < YT3; ky++)
+                    for (int kz = 0; kz < ZT3; kz++) {
+                        ore[(k * YT3 + ky) * ZT3 + kz] = h_re[(kz * YT3 + ky) * XC3 + k + 2];
+                        oim[(k * YT3 + ky) * ZT3 + kz] = h_im[(kz * YT3 + ky) * XC3 + k + 2];
+                    }
+        }
+    }
+    free(win);
+    free(real);
+    free(sre);
+    free(sim);
+    free(h_re);
+    free(h_im);
+    return 0;
+}
+
 /* 3D decode of one frame window with a soft mask per tile (masks: 768
  * floats per tile of the window's two Z tiles, in tile loop order). */
 int orc_decode_window_masked(const orc_params *p, const float *masks, const uint16_t *const *fields,

@@ -85,6 +85,8 @@ def lib():
                                              C.c_void_p, i32]
         _lib.orc_decode_sequence.restype = C.c_int
         _lib.orc_ynr_line.argtypes = [P(OrcParams), P(d)]
+        _lib.orc_tile_spectra3d.argtypes = [P(OrcParams), P(P(d)), P(i32), d, P(d), P(d)]
+        _lib.orc_tile_spectra3d.restype = C.c_int
     return _lib
 
 
@@ -220,6 +222,25 @@ def decode_frame_masked(p: OrcParams, seq_fields, frame, masks, out_fmt=0):
     if rc != 0:
         raise RuntimeError(f"orc_decode_window_masked failed: {rc}")
     return out
+
+
+def tile_spectra3d(p: OrcParams, fields8, offsets8, fill: float):
+    """Bins x = 2..6 of every 3D tile of one Z tile (8 fields as float
+    arrays, their line offsets): complex [n_tile_rows * n_tile_cols, 5, 32, 8]
+    (bin x = k + 2, ky, kz), tiles in lattice order; `fill` is the value of
+    the lines the tile does not read from a field (black16bIre for a
+    composite, 0 for chroma).  transform_pal_3d.cpp:123-202."""
+    L = lib()
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in fields8]
+    arr = (C.POINTER(C.c_double) * 8)(*[f.ctypes.data_as(C.POINTER(C.c_double)) for f in fs])
+    n_ty = (p.last_active_frame_line - (p.first_active_frame_line - 16)) // 16 + 1
+    n_tx = (p.active_video_end - p.active_video_start + 8 + 7) // 8
+    re = np.zeros((n_ty * n_tx, 5, 32, 8))
+    im = np.zeros_like(re)
+    o = np.ascontiguousarray(offsets8, dtype=np.int32)
+    L.orc_tile_spectra3d(C.byref(p), arr, o.ctypes.data_as(C.POINTER(C.c_int32)), fill,
+                         re.ctypes.data_as(C.POINTER(C.c_double)), im.ctypes.data_as(C.POINTER(C.c_double)))
+    return re + 1j * im
 
 
 def decode_frame_from_chroma(p: OrcParams, field0, field1, chroma0, chroma1, out_fmt=0):

@@ -49,3 +49,30 @@ def patch_capture(synth, capture_dir):
     path = os.path.join(capture_dir, "patches.tbc")
     synth.write_tbc(path, fields)
     return path, fields
+
+
+_PARITY_ROWS = []
+
+
+@pytest.fixture(scope="session")
+def parity_rows():
+    """Worst GPU - oracle errors per benchmark configuration, collected by
+    tests/test_gpu_configs.py: (config, frames, output format, stats)."""
+    return _PARITY_ROWS
+
+
+def pytest_sessionfinish(session, exitstatus):
+    """With CHD_PARITY_TABLE=<path>, write the collected rows as a markdown
+    table (float planes: full-scale units; yuv444p16: LSB)."""
+    path = os.environ.get("CHD_PARITY_TABLE")
+    if not path or not _PARITY_ROWS:
+        return
+    lines = ["| Config | Frames checked | Output | E'Y max | U/V max | U/V RMS |", "|---|---|---|---|---|---|"]
+    for name, frames, fmt, s in _PARITY_ROWS:
+        if fmt == "p16":
+            vals = f"{s['y_max']:.0f} LSB | {s['uv_max']:.0f} LSB | {s['uv_rms']:.3f} LSB"
+        else:
+            vals = f"{s['y_max']:.2e} | {s['uv_max']:.2e} | {s['uv_rms']:.2e}"
+        lines.append(f"| {name} | {frames} | {fmt} | {vals} |")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

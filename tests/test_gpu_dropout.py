@@ -49,3 +49,66 @@ def test_corrected_decode_matches_oracle(orc, dropout_capture, kind):
     dec.set_dropout(False)
     assert np.array_equal(dec.decode(3), plain.decode(3))
     assert dec.last_dropout_stats() == (0, 0, 0)
+
+
+def test_overlapping_pairs_conceal_each_pair(orc, tmp_path):
+    """Field 3 marked second and field 4 first: frame 2 pairs fields 4 and 5
+    (both marked first, decoded as is) and frame 3 pairs fields 5 and 6
+    (1-based), so field 5 is concealed twice, once with each partner, and
+    one 3D window holds both copies.  Each frame of the window is concealed
+    as its own pair, as the reference does (c_api_decoder.cpp:297-315)."""
+    import json
+    rng = np.random.default_rng(61)
+    scene = synth.PatchScene(seed=61)
+    fields = synth.encode_sequence([scene.frame(i) for i in range(6)],
+                                   synth.EncodeOptions(snr_db=35, rng=rng))
+    drops = [[] for _ in range(fields.shape[0])]
+    drops[4] = [(300, 640, 101), (500, 900, 140)]     # field 5: startx, endx, field line (1-based)
+    drops[3] = [(250, 420, 120)]
+    drops[5] = [(400, 700, 90)]
+    for i in (3, 4, 5):
+        for sx, ex, ln in drops[i]:
+            fields[i, ln - 1, sx:ex] = 0x0100
+    path = str(tmp_path / "overlap.tbc")
+    side = synth.write_tbc(path, fields, drops)
+    with open(side) as fh:
+        meta = json.load(fh)
+    meta["fields"][2]["isFirstField"] = False
+    meta["fields"][3]["isFirstField"] = True
+    with open(side, "w") as fh:
+        json.dump(meta, fh)
+    first = [bool(r["isFirstField"]) for r in meta["fields"]]
+    # the library's pairing (Metadata::fieldNumber): advance to a first field
+    pairs, k = [], 0
+    while 2 * k + 1 < fields.shape[0]:
+        a = 2 * k
+        while a < fields.shape[0] - 1 and not first[a]:
+            a += 1
+        if a + 1 >= fields.shape[0]:
+            break
+        pairs.append((a, a + 1))
+        k += 1
+    assert pairs[1] == (3, 4) and pairs[2] == (4, 5)    # 0-based: field 5 in two pairs
+    conc = []
+    for a, b in pairs:
+        ca, cb, _ = od.correct_frame(PAL_VP, fields[a], fields[b], drops[a], drops[b])
+        conc.append((ca, cb, 0 if first[a] else 1, 0 if first[b] else 1))
+    assert not np.array_equal(conc[1][1], conc[2][0])   # the two concealments of field 5 differ
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps").set_dropout(True).commit()
+    p = orc.pal_params()
+    black = np.full(fields.shape[1:], p.black16b, np.uint16)
+    for f in (1, 2, 3):
+        win, offs = [], []
+        for fr in range(f - 2, f + 5):
+            if 0 <= fr < len(pairs):
+                win += [conc[fr][0], conc[fr][1]]
+                offs += [conc[fr][2], conc[fr][3]]
+            else:
+                win += [black, black]
+                offs += [0, 1]
+        want = orc.decode_window(p, 8, np.stack(win), offs, 4, 6, orc.uniform_thresholds(8), 0)
+        got = dec.decode(f)
+        assert np.abs(got.astype(np.float64) - want).max() <= 1e-4, f
+    dec.close()
+    v.close()

@@ -195,3 +195,62 @@ def test_demod_from_chroma_reproduces_the_full_decode(orc, patch_capture):
     full, chroma = orc.decode_window(p, 8, fw, offs, s, e, orc.uniform_thresholds(8), want_chroma=True)
     got = orc.decode_frame_from_chroma(p, fields[4], fields[5], chroma[0], chroma[1])
     assert np.array_equal(got, full)
+
+
+def test_tile_spectra_match_rfftn(orc):
+    """orc_tile_spectra3d against numpy's rfftn of a brute-force windowed
+    tile built from transform_pal_3d.cpp:168-202 (lines outside the active
+    lines or of the other parity hold the fill value)."""
+    p = orc.pal_params(first_line=200, last_line=231, avs=400, ave=416)
+    rng = np.random.default_rng(1)
+    fields = [rng.integers(10000, 60000, size=(313, 1135)).astype(np.float64) for _ in range(8)]
+    offs = [z & 1 for z in range(8)]
+    spec = orc.tile_spectra3d(p, fields, offs, 16384.0)
+    assert spec.shape == (9, 5, 32, 8)
+    for t, (tile_y, tile_x) in enumerate([(184 + 16 * a, 392 + 8 * b) for a in range(3) for b in range(3)]):
+        tile = np.zeros((8, 32, 16))
+        for z in range(8):
+            for y in range(32):
+                fy = tile_y + y
+                v = (fields[z][fy // 2, tile_x:tile_x + 16]
+                     if 200 <= fy <= 231 and fy % 2 == offs[z] else np.full(16, 16384.0))
+                tile[z, y] = v * orc.window(z, 8) * orc.window(y, 32) * np.array([orc.window(x, 16) for x in range(16)])
+        want = np.transpose(np.fft.rfftn(tile)[:, :, 2:7], (2, 1, 0))
+        assert np.abs(spec[t] - want).max() <= 1e-9 * np.abs(want).max(), t
+
+
+def test_wiener_optimal_mask_ceiling(orc):
+    """The least-squares optimal soft mask (per visit, from the encoder's
+    true modulated chroma: m = sum Re(conj(C) F) / sum |F|^2 over the visit's
+    bins, clipped to [0, 1]) is an upper bound for what any learned mask of
+    this form can reach.  Against the ideal separation (the demodulator fed
+    the true chroma) it gains several dB over the 0.4 threshold; against the
+    encoder's U/V it gains well under 1 dB, because the demodulator's own
+    1.1 MHz, 7-line chroma filter sets that error for every mask: the +1 dB
+    against encoder U/V is out of reach of any chroma/luma separation mask."""
+    from libchromadec_amd import abi, synth
+    from libchromadec_amd import mask_training as mt
+    scene = synth.PatchScene(seed=5, n_patches=8)
+    fields, u, v, chroma = synth.encode_sequence([scene.frame(i) for i in range(7)],
+                                                 synth.EncodeOptions(snr_db=40, rng=np.random.default_rng(5)),
+                                                 return_truth=True)
+    p = orc.pal_params()
+    f = 2                                   # its window is frames 0..6
+    offs = [z & 1 for z in range(8)]
+    n_ty, n_tx = 37, 117
+    masks = []
+    for s0 in (0, 4):                       # the window's two Z tiles: fields 2f-4 and 2f
+        F = orc.tile_spectra3d(p, fields[s0:s0 + 8], offs, float(p.black16b))
+        Ct = orc.tile_spectra3d(p, chroma[s0:s0 + 8], offs, 0.0)
+        w, num = mn.targets(F, Ct)
+        m = np.where(w > 0, np.clip(num / np.where(w > 0, w, 1.0), 0.0, 1.0), 0.0)
+        masks.append(m.reshape(n_ty, n_tx, 768))
+    best = orc.decode_frame_masked(p, fields, f, np.stack(masks), 0)[None]
+    plain = orc.decode_frame(p, 8, fields, f, orc.uniform_thresholds(8), 0)[None]
+    ideal = orc.decode_frame_from_chroma(p, fields[2 * f], fields[2 * f + 1], chroma[2 * f], chroma[2 * f + 1], 0)[None]
+    ecb, ecr = abi.color_scales()
+    sep = {k: mt.separation_psnr(x, ideal) for k, x in (("plain", plain), ("best", best))}
+    uv = {k: mt.chroma_psnr(x, u, v, [f], ecb, ecr) for k, x in (("plain", plain), ("best", best))}
+    print("optimal mask: separation PSNR", sep, "encoder U/V PSNR", uv)
+    assert sep["best"] - sep["plain"] >= 3.0, (sep, uv)
+    assert 0.0 <= uv["best"] - uv["plain"] < 0.5, (sep, uv)
