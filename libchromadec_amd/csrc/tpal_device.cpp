@@ -825,6 +825,7 @@ struct chd_tpal_batch {
     tpal::FrameJob *d_frame = nullptr;
     float *d_partials = nullptr;
     float *d_raw = nullptr;        // uncored Y/U/V of every frame, with luma coring on
+    unsigned *d_wq = nullptr;      // work counters of the persistent tile launches
     void *d_out = nullptr;
     bool border = false;
     int strip = tpal::kStrip3, groups = 0;   // strip: set at creation
@@ -846,7 +847,7 @@ void batch_release(chd_tpal_batch *b)
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     if (b->cap) hipStreamDestroy(b->cap);
-    void *ptrs[] = {b->d_slab, b->d_field, b->d_frame, b->d_partials, b->d_spec, b->d_mask, b->d_raw};
+    void *ptrs[] = {b->d_slab, b->d_field, b->d_frame, b->d_partials, b->d_spec, b->d_mask, b->d_raw, b->d_wq};
     for (void *q : ptrs)
         if (q) hipFree(q);
 }
@@ -862,7 +863,7 @@ void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
     }
     if (plan->net == nullptr) {
         tpal::launch_tpal3d(g, b->d_slab, b->n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTilePlain, nullptr,
-                            nullptr, strip, groups);
+                            nullptr, strip, groups, b->d_wq);
         return;
     }
     chd_mask_net *net = plan->net;
@@ -870,11 +871,11 @@ void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
     for (int u0 = 0; u0 < b->n_units; u0 += b->chunk_units) {
         const int cnt = std::min(b->chunk_units, b->n_units - u0);
         tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra, b->d_spec,
-                            nullptr, strip, groups);
+                            nullptr, strip, groups, b->d_wq);
         mnet::launch_infer(reinterpret_cast<const float2 *>(b->d_spec), cnt * tps, net->theta, net->f, b->d_mask,
                            net->cus, s);
         tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply, b->d_spec,
-                            b->d_mask, strip, groups);
+                            b->d_mask, strip, groups, b->d_wq);
     }
 }
 
@@ -1019,6 +1020,8 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
         e = hipMalloc(&b->d_spec, tiles * tpal::kTileBins * 2 * sizeof(float));
         if (e == hipSuccess) e = hipMalloc((void **)&b->d_mask, tiles * mnet::kP * sizeof(float));
     }
+    if (e == hipSuccess && is3d) e = hipMalloc((void **)&b->d_wq, tpal::kWorkCounters * sizeof(unsigned));
+    if (e == hipSuccess && is3d) e = hipMemset(b->d_wq, 0, tpal::kWorkCounters * sizeof(unsigned));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e != hipSuccess) {
         batch_release(b);

@@ -114,10 +114,14 @@ enum { kTilePlain = 0, kTileSpectra = 1, kTileApply = 2 };
 constexpr int kTileBins = 5 * 32 * 8;
 // strip: tiles per work item (a multiple of 4; 0 picks one for the launch
 // size with choose_strip3); max_groups > 0 caps the grid (persistent
-// workgroups), 0 launches one workgroup per item.
+// workgroups), 0 launches one workgroup per item.  A persistent grid needs
+// wq: 9 zeroed device counters (work items taken per XCD, workgroups done),
+// which every launch leaves zeroed again; launches sharing them must not
+// overlap.  Without wq the launch is one workgroup per item.
+constexpr int kWorkCounters = 9;
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
                    hipStream_t s, int mode = kTilePlain, void *spec = nullptr, const float *mask = nullptr,
-                   int strip = 0, int max_groups = 0);
+                   int strip = 0, int max_groups = 0, unsigned *wq = nullptr);
 int choose_strip3(const Geom &g, int n_jobs);
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
                    const double *thr64, hipStream_t s);

@@ -455,7 +455,8 @@ __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
 template <int MODE, bool PERSIST>
 __global__ void __launch_bounds__(64, 3)   // 3 waves per SIMD: at most 168 VGPRs
 tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int strip_len, const float *__restrict__ thr32,
-             const double *__restrict__ thr64, float2 *__restrict__ spec, const float *__restrict__ mask)
+             const double *__restrict__ thr64, float2 *__restrict__ spec, const float *__restrict__ mask,
+             unsigned *__restrict__ wq)
 {
     __shared__ typename LdsFor<MODE>::T L;
     const int lane = threadIdx.x;
@@ -477,8 +478,22 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     const int G = gridDim.x;
     const int xcd = blockIdx.x & 7, q = total >> 3, r = total & 7;
     const int lo = xcd * q + min(xcd, r), hi = lo + q + (xcd < r ? 1 : 0);
-    const int per_xcd = (G - xcd + 7) >> 3;  // workgroups on this XCD
-    for (int w = lo + (blockIdx.x >> 3); w < hi; w += per_xcd) {
+    // One workgroup per item: item lo + (its rank on the XCD).  Persistent:
+    // the workgroups of an XCD take that XCD's items in order from its own
+    // counter wq[xcd], so uneven items balance as the hardware's dispatch of
+    // one workgroup per item would; wq[8] counts workgroups that ran out, and
+    // the last of them resets all nine counters for the next launch (every
+    // other workgroup has made its last fetch by then).
+    auto fetch = [&]() -> int {
+        if constexpr (PERSIST) {
+            unsigned k = 0;
+            if (lane == 0) k = atomicAdd(&wq[xcd], 1u);
+            return lo + (int)__builtin_amdgcn_readfirstlane(k);
+        } else {
+            return lo + (blockIdx.x >> 3);
+        }
+    };
+    for (int w = fetch(); w < hi; w = PERSIST ? fetch() : hi) {
     const int job_id = w % n_jobs;
     const int rest = w / n_jobs;
     const int ty = rest % g.n_ty3;
@@ -984,8 +999,13 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         }
         __syncthreads();
     }
-    if constexpr (!PERSIST) break;
     }  // work items
+    if constexpr (PERSIST) {
+        if (lane == 0 && atomicAdd(&wq[8], 1u) == (unsigned)G - 1u) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) atomicExch(&wq[i], 0u);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2162,17 +2182,17 @@ int choose_strip3(const Geom &g, int n_jobs)
 }
 
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
-                   hipStream_t s, int mode, void *spec, const float *mask, int strip, int max_groups)
+                   hipStream_t s, int mode, void *spec, const float *mask, int strip, int max_groups, unsigned *wq)
 {
     float2 *sp = reinterpret_cast<float2 *>(spec);
     if (strip <= 0) strip = choose_strip3(g, n_jobs);
     const int items = n_jobs * g.n_ty3 * ((g.n_tx3 + strip - 1) / strip);
     if (items <= 0) return;
     // persistent grids are whole multiples of 8 (one share per XCD)
-    const bool persist = max_groups > 0 && max_groups < items;
+    const bool persist = max_groups > 0 && max_groups < items && wq != nullptr;
     const dim3 grid(persist ? std::max(8, max_groups & ~7) : items), blk(64);
 #define CHD_LAUNCH3D(M, P)                                                                                   \
-    hipLaunchKernelGGL((tpal3d_tiles<M, P>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask)
+    hipLaunchKernelGGL((tpal3d_tiles<M, P>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask, wq)
     if (mode == kTileSpectra) {
         if (persist) CHD_LAUNCH3D(kTileSpectra, true); else CHD_LAUNCH3D(kTileSpectra, false);
     } else if (mode == kTileApply) {
