@@ -19,6 +19,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
@@ -44,8 +45,13 @@ using chd::fail;
 
 enum class OptType { F64, I32, Bool, Str };
 
-constexpr int kMaxGpuWorkers = 8;   // host workers feeding one GPU in async decode
-constexpr int kCacheSlots = 32;     // packed fields resident per worker (window needs 14)
+constexpr int kMaxGpuWorkers = 8;   // host workers feeding one GPU in async decode (workers 1..8)
+constexpr int kBatch = 4;           // consecutive frames one submission decodes together
+constexpr int kCacheSlots = 14 * kBatch;   // packed fields resident per worker (a window needs 14)
+// fields one submission stages from the file: a batch of consecutive frames
+// and a 3D window's two frames behind and four ahead
+constexpr int kStageFields = 2 * (kBatch + 6);
+static_assert(kStageFields <= tpal::kPackListMax, "staged fields are packed by one launch");
 
 bool is_pal_kind(chd_decoder_kind_t k)
 {
@@ -142,9 +148,18 @@ struct Worker {
     std::vector<int64_t> slot_key;      // field number * 2 + shifted, -1 = empty
     std::vector<uint64_t> slot_tick;
     uint64_t tick = 0;
-    uint16_t *h_stage = nullptr;        // pinned: 14 fields as read from the file
-    uint16_t *d_stage = nullptr;        // their device copies, before packing into slots
-    void *d_out = nullptr;
+    uint16_t *d_stage = nullptr;        // staged fields on the device, before packing into slots
+    // Two submissions in flight: submission slot s stages fields in
+    // h_stage[s] (pinned), decodes up to kBatch frames into d_out[s] on
+    // `stream`, and those come down on copy_stream, so one submission's
+    // download overlaps the next one's decode.  A slot is reused only after
+    // its download is waited for, which also orders its staging buffer.
+    uint16_t *h_stage[2] = {nullptr, nullptr};
+    const uint16_t *h_stage_dev[2] = {nullptr, nullptr};   // their device addresses
+    void *d_out[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_dec[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
+    bool copy_busy[2] = {false, false};
     int device = 0;
     uint64_t seen_revision = 0;         // dropout options the cached fields were corrected with
 
@@ -156,18 +171,25 @@ struct Worker {
             chd_tpal_plan_free(plan);
         }
         chd_mask_net_free(net);
+        if (copy_stream) (void)hipStreamSynchronize(copy_stream);
         if (d_slots) (void)hipFree(d_slots);
-        if (d_out) (void)hipFree(d_out);
+        for (void *o : d_out)
+            if (o) (void)hipFree(o);
         if (d_stage) (void)hipFree(d_stage);
-        if (h_stage) (void)hipHostFree(h_stage);
+        for (uint16_t *h : h_stage)
+            if (h) (void)hipHostFree(h);
+        for (hipEvent_t e : {ev_dec[0], ev_dec[1], ev_copy[0], ev_copy[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
 }  // namespace
 
-// Pinned frame buffers: at most kPoolKeep idle ones are kept.
-constexpr size_t kPoolKeep = 16;
+// Pinned frame buffers: at most kPoolKeep idle ones are kept (freeing one
+// synchronises the device).  Async decode holds two per worker in flight.
+constexpr size_t kPoolKeep = 32;
 
 FramePool::~FramePool()
 {
@@ -262,16 +284,24 @@ chd_status_t worker_init(chd_decoder_t *d, Worker &w, const char *who)
 {
     if (w.plan != nullptr) return CHD_OK;
     chd_tpal_params_t p = d->params;
-    p.max_batch_frames = 1;
+    p.max_batch_frames = kBatch;
     chd_status_t st = tpal_plan_create_impl(&p, &w.plan, who);
     if (st != CHD_OK) return st;
     w.device = p.device;
     const tpal::Geom &g = w.plan->g;
     hipError_t e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&w.d_slots, (size_t)kCacheSlots * g.field_elems * sizeof(uint16_t));
-    if (e == hipSuccess) e = hipMalloc(&w.d_out, chd_tpal_frame_bytes(w.plan));
-    const size_t stage_bytes = (size_t)14 * g.fw * g.fh * sizeof(uint16_t);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&w.h_stage, stage_bytes, 0);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMalloc(&w.d_out[k], kBatch * chd_tpal_frame_bytes(w.plan));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&w.copy_stream, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipEventCreateWithFlags(&w.ev_dec[k], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&w.ev_copy[k], hipEventDisableTiming);
+    }
+    const size_t stage_bytes = (size_t)kStageFields * g.fw * g.fh * sizeof(uint16_t);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipHostMalloc((void **)&w.h_stage[k], stage_bytes, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&w.h_stage_dev[k], w.h_stage[k], 0);
+    }
     if (e == hipSuccess) e = hipMalloc((void **)&w.d_stage, stage_bytes);
     if (e == hipSuccess)
         e = hipMemsetD16Async((hipDeviceptr_t)w.d_slots, (unsigned short)d->vp.black16bIre,
@@ -297,14 +327,41 @@ Worker *get_worker(chd_decoder_t *d, size_t idx)
     return d->workers[idx].get();
 }
 
-// Make `number` (1-based) resident in the worker's field cache and return its
-// packed device pointer.  `pinned` marks slots the current window uses.
+// Fields a submission reads from the file into its pinned staging buffer;
+// flush_stage uploads them in one copy and packs them into their cache slots
+// in one launch.
+struct Staging {
+    uint16_t *h = nullptr;
+    const uint16_t *h_dev = nullptr;   // h's device address
+    int n = 0;
+    tpal::PackList list{};
+};
+
+chd_status_t flush_stage(Worker &w, Staging &sg)
+{
+    if (sg.n == 0) return CHD_OK;
+    const tpal::Geom &g = w.plan->g;
+    const int64_t fs = (int64_t)g.fw * g.fh;
+    // One contiguous copy, then the pack kernel lays each field out with its
+    // margins (row-by-row pitched copies cost a DMA command per line).  A
+    // kernel reads the pinned staging buffer: the DMA engine that carries
+    // decoded frames down would otherwise carry the fields up in turn.
+    tpal::launch_copy_in(sg.h_dev, w.d_stage, (size_t)sg.n * fs * sizeof(uint16_t), w.stream);
+    sg.list.n = sg.n;
+    tpal::launch_pack_list(g, w.d_stage, fs, w.d_slots, sg.list, w.stream);
+    const hipError_t e = hipGetLastError();
+    sg.n = 0;
+    if (e != hipSuccess) return fail(CHD_E_INTERNAL, "chd_decode_frame: field upload failed: " + hip_err(e));
+    return CHD_OK;
+}
+
 // Device copy of field `number` (1-based), read from the file, or, when
 // `corrected` is given, that host copy (already shifted and concealed).  A
 // concealed field depends on the frame it was concealed in, so its key
 // holds the partner field: with both fields marked first, one field can be
-// the second field of one frame and the first of the next.
-const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool shifted, int stage_idx,
+// the second field of one frame and the first of the next.  `pinned` marks
+// the slots the current submission uses; a new field is staged in `sg`.
+const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool shifted, Staging &sg,
                             std::vector<int> &pinned, chd_status_t &st, const uint16_t *corrected = nullptr,
                             int32_t partner = 0)
 {
@@ -326,29 +383,30 @@ const uint16_t *cache_field(chd_decoder_t *d, Worker &w, int32_t number, bool sh
         if (std::find(pinned.begin(), pinned.end(), s) != pinned.end()) continue;
         if (victim < 0 || w.slot_tick[s] < w.slot_tick[victim]) victim = s;
     }
-    uint16_t *stage = w.h_stage + (size_t)stage_idx * g.fw * g.fh;
+    if (victim < 0) {
+        st = fail(CHD_E_INTERNAL, "chd_decode_frame: field cache exhausted");
+        return nullptr;
+    }
+    if (sg.n == kStageFields) {
+        // staging full (only with unusual field numbering): upload what is
+        // staged and wait for the copy before refilling the buffer
+        st = flush_stage(w, sg);
+        if (st == CHD_OK && hipStreamSynchronize(w.stream) != hipSuccess)
+            st = fail(CHD_E_INTERNAL, "chd_decode_frame: field upload failed");
+        if (st != CHD_OK) return nullptr;
+    }
+    uint16_t *stage = sg.h + (size_t)sg.n * g.fw * g.fh;
     if (corrected != nullptr) {
         std::memcpy(stage, corrected, (size_t)g.fw * g.fh * sizeof(uint16_t));
     } else if (!d->video->tbc.read_field(number, stage)) {
         st = fail(CHD_E_IO, "chd_decode_frame: failed to read field " + std::to_string(number));
         return nullptr;
     }
-    // One contiguous copy, then the pack kernel lays the field out with its
-    // margins (row-by-row pitched copies cost a DMA command per line).
     // SourceField::shiftSecondField (source_field.cpp:51-66) drops the first
     // two samples of the field and appends two black ones: shift 2.
-    uint16_t *d_src = w.d_stage + (size_t)stage_idx * g.fw * g.fh;
-    hipError_t e = hipMemcpyAsync(d_src, stage, (size_t)g.fw * g.fh * sizeof(uint16_t), hipMemcpyHostToDevice,
-                                  w.stream);
-    if (e == hipSuccess) {
-        tpal::launch_pack(g, d_src, (int64_t)g.fw * g.fh, w.d_slots + (size_t)victim * g.field_elems, 1, w.stream,
-                          shifted && corrected == nullptr ? 2 : 0);
-        e = hipGetLastError();
-    }
-    if (e != hipSuccess) {
-        st = fail(CHD_E_INTERNAL, "chd_decode_frame: field upload failed: " + hip_err(e));
-        return nullptr;
-    }
+    sg.list.slot[sg.n] = victim;
+    sg.list.shift[sg.n] = shifted && corrected == nullptr ? 2 : 0;
+    sg.n++;
     w.slot_key[victim] = key;
     w.slot_tick[victim] = w.tick;
     pinned.push_back(victim);
@@ -419,19 +477,42 @@ std::shared_ptr<const chd_decoder::Corrected> corrected_frame(chd_decoder_t *d, 
     return c;
 }
 
-// decodeFrameLocked (c_api_decoder.cpp:183-332, 461-515) on worker `w`.
-chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd_frame_t **out)
+// Wait for `ev` by polling: a blocking wait sleeps until an interrupt, and
+// with several workers each waiting on its own frames those wake-ups came
+// milliseconds late, leaving the GPU idle between frames.  Polls for up to
+// 50 ms, then blocks.
+hipError_t wait_event(hipEvent_t ev)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e != hipErrorNotReady) return e;
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) return hipEventSynchronize(ev);
+        std::this_thread::yield();
+    }
+}
+
+// One submission: up to kBatch frames, each with its status and, once
+// decoded, its pinned host frame.
+struct Submission {
+    int n = 0;
+    int64_t idx[kBatch] = {};
+    chd_status_t st[kBatch] = {};
+    std::unique_ptr<chd_frame> frame[kBatch];
+    bool busy = false;
+};
+
+// SourceField::loadFields (source_field.cpp:68-171) for one frame: the
+// window's fields, made resident in the worker's cache (new ones staged in
+// `sg`), and the field order offsets.
+chd_status_t load_window(chd_decoder_t *d, Worker &w, int64_t frame_index, const chd_dropout_opts_t &dopts,
+                         uint64_t drev, Staging &sg, std::vector<int> &pinned, TpalWindow &win)
 {
     chd::Metadata &meta = d->video->meta;
     const bool fff = d->firstFieldFirst();
     const int32_t numFrames = meta.numberOfFrames(fff);
     if (frame_index < 0 || frame_index >= numFrames)
         return fail(CHD_E_OUT_OF_RANGE, "chd_decode_frame: frame_index out of range");
-    chd_status_t st = worker_init(d, w, "chd_decode_frame");
-    if (st != CHD_OK) return st;
-    hipSetDevice(w.device);
-
-    // SourceField::loadFields (source_field.cpp:68-171)
     const int32_t behind = d->lookBehind, ahead = d->lookAhead;
     const int32_t nwin = 2 * (behind + 1 + ahead);
     const int32_t firstFrameNumber = (int32_t)frame_index + 1;
@@ -439,24 +520,8 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
     const int32_t numSource = d->video->tbc.available_fields;
     const bool shiftSecond = (d->vp.system == chd::System::PAL || d->vp.system == chd::System::PAL_M) &&
                              d->vp.isSubcarrierLocked;
-    TpalWindow win;
+    chd_status_t st = CHD_OK;
     std::memset(&win, 0, sizeof(win));
-    std::vector<int> pinned;
-    chd_dropout_opts_t dopts{};
-    uint64_t drev = 0;
-    {
-        std::lock_guard<std::mutex> lk(d->dropMu);
-        if (d->dropoutSet && d->dropoutOpts.enabled) {
-            dopts = d->dropoutOpts;
-            drev = d->dropoutRevision;
-        }
-    }
-    if (drev != w.seen_revision) {
-        // cached device copies of corrected fields belong to other options
-        for (int s = 0; s < kCacheSlots; ++s)
-            if (w.slot_key[s] >= 0 && (w.slot_key[s] & 1)) w.slot_key[s] = -1;
-        w.seen_revision = drev;
-    }
     chd::dropout::Stats frameStats;
     int32_t frameNumber = firstFrameNumber - behind;
     for (int32_t i = 0; i < nwin; i += 2, ++frameNumber) {
@@ -492,53 +557,168 @@ chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd
             auto c = corrected_frame(d, f1, f2, shiftSecond, dopts, drev, d->dropParams, st);
             if (st != CHD_OK) return st;
             if (i == 2 * behind) frameStats = c->stats;
-            win.field[i] = cache_field(d, w, f1, false, i, pinned, st, c->first.data(), f2);
+            win.field[i] = cache_field(d, w, f1, false, sg, pinned, st, c->first.data(), f2);
             if (st != CHD_OK) return st;
-            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, i + 1, pinned, st, c->second.data(), f1);
+            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, sg, pinned, st, c->second.data(), f1);
             if (st != CHD_OK) return st;
         } else {
-            win.field[i] = cache_field(d, w, f1, false, i, pinned, st);
+            win.field[i] = cache_field(d, w, f1, false, sg, pinned, st);
             if (st != CHD_OK) return st;
-            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, i + 1, pinned, st);
+            win.field[i + 1] = cache_field(d, w, f2, shiftSecond, sg, pinned, st);
             if (st != CHD_OK) return st;
         }
     }
     // The decoded frame's own fields are black (nullptr) when the metadata or
     // the .tbc do not cover it, as in SourceField::loadFields; the kernels
     // decode a nullptr field as black.
-    win.out = w.d_out;
-    st = tpal_decode_windows(w.plan, &win, 1, w.stream);
-    if (st != CHD_OK) return st;
+    std::lock_guard<std::mutex> lk(d->dropMu);
+    d->lastStats.corrected = frameStats.corrected;
+    d->lastStats.failed = frameStats.failed;
+    d->lastStats.total_distance = frameStats.total_distance;
+    return CHD_OK;
+}
+
+// decodeFrameLocked (c_api_decoder.cpp:183-332, 461-515) for frames
+// idx[0..nb) on worker `w`, in two halves: submit_frames loads the windows,
+// enqueues one decode of all of them into submission slot `slot` and the
+// downloads into pinned frames; finish_frames waits for the downloads.  A
+// worker holds one submission per slot; a frame whose window cannot be
+// loaded keeps its error status and the others still decode.
+void submit_frames(chd_decoder_t *d, Worker &w, const int64_t *idx, int nb, int slot, Submission &sub)
+{
+    sub.n = nb;
+    sub.busy = true;
+    for (int j = 0; j < nb; ++j) {
+        sub.idx[j] = idx[j];
+        sub.st[j] = CHD_OK;
+        sub.frame[j].reset();
+    }
+    auto fail_all = [&](chd_status_t st) {
+        for (int j = 0; j < nb; ++j)
+            if (sub.st[j] == CHD_OK) sub.st[j] = st;
+    };
+    chd_status_t st = worker_init(d, w, "chd_decode_frame");
+    if (st != CHD_OK) return fail_all(st);
+    hipSetDevice(w.device);
+
+    chd_dropout_opts_t dopts{};
+    uint64_t drev = 0;
     {
         std::lock_guard<std::mutex> lk(d->dropMu);
-        d->lastStats.corrected = frameStats.corrected;
-        d->lastStats.failed = frameStats.failed;
-        d->lastStats.total_distance = frameStats.total_distance;
+        if (d->dropoutSet && d->dropoutOpts.enabled) {
+            dopts = d->dropoutOpts;
+            drev = d->dropoutRevision;
+        }
+    }
+    if (drev != w.seen_revision) {
+        // cached device copies of corrected fields belong to other options
+        for (int s = 0; s < kCacheSlots; ++s)
+            if (w.slot_key[s] >= 0 && (w.slot_key[s] & 1)) w.slot_key[s] = -1;
+        w.seen_revision = drev;
+    }
+    const size_t fb = chd_tpal_frame_bytes(w.plan);
+    Staging sg;
+    sg.h = w.h_stage[slot];
+    sg.h_dev = w.h_stage_dev[slot];
+    std::vector<int> pinned;
+    TpalWindow win[kBatch];
+    int at[kBatch];   // frame of each decoded window
+    int nw = 0;
+    for (int j = 0; j < nb; ++j) {
+        sub.st[j] = load_window(d, w, idx[j], dopts, drev, sg, pinned, win[nw]);
+        if (sub.st[j] != CHD_OK) continue;
+        win[nw].out = static_cast<char *>(w.d_out[slot]) + (size_t)nw * fb;
+        at[nw++] = j;
+    }
+    st = flush_stage(w, sg);
+    if (st == CHD_OK && nw > 0) {
+        // the slot's previous frames must have come down before they are
+        // overwritten
+        if (w.copy_busy[slot]) (void)hipStreamWaitEvent(w.stream, w.ev_copy[slot], 0);
+        st = tpal_decode_windows(w.plan, win, nw, w.stream);
+    }
+    if (st != CHD_OK) {
+        // nothing may still read the staging buffer once the slot is reused
+        (void)hipStreamSynchronize(w.stream);
+        return fail_all(st);
+    }
+    if (nw == 0) {
+        // no decode follows: wait for the staged fields to be read here, as
+        // the slot's staging buffer may be refilled next
+        (void)hipStreamSynchronize(w.stream);
+        return;
     }
 
-    auto frame = std::make_unique<chd_frame>();
-    frame->format = d->format;
-    frame->width = d->outW;
-    frame->height = d->outH;
-    frame->planes = d->planes;
-    const size_t n = (size_t)d->planes * d->outW * d->outH;
     const bool is_float = d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS ||
                           d->format == CHD_PIXEL_RGBS;
-    const size_t bytes = n * (is_float ? sizeof(float) : sizeof(uint16_t));
-    // straight into a pinned frame buffer: a single DMA at link rate
-    frame->pool = d->framePool;
-    frame->data = d->framePool->get();
-    if (frame->data == nullptr) return fail(CHD_E_OOM, "chd_decode_frame: no pinned host memory for the frame");
-    hipError_t e = hipMemcpyAsync(frame->data, w.d_out, bytes, hipMemcpyDeviceToHost, w.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
-    if (e != hipSuccess) return fail(CHD_E_INTERNAL, "chd_decode_frame: GPU decode failed: " + hip_err(e));
-    frame->info.format = d->format;
-    frame->info.width = d->outW;
-    frame->info.height = d->outH;
-    frame->info.num_planes = d->planes;
-    frame->info.frame_index = frame_index;
-    *out = frame.release();
-    return CHD_OK;
+    const size_t bytes = (size_t)d->planes * d->outW * d->outH * (is_float ? sizeof(float) : sizeof(uint16_t));
+    hipError_t e = hipEventRecord(w.ev_dec[slot], w.stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(w.copy_stream, w.ev_dec[slot], 0);
+    for (int k = 0; k < nw && e == hipSuccess; ++k) {
+        const int j = at[k];
+        auto frame = std::make_unique<chd_frame>();
+        frame->format = d->format;
+        frame->width = d->outW;
+        frame->height = d->outH;
+        frame->planes = d->planes;
+        frame->info.format = d->format;
+        frame->info.width = d->outW;
+        frame->info.height = d->outH;
+        frame->info.num_planes = d->planes;
+        frame->info.frame_index = idx[j];
+        // straight into a pinned frame buffer: a single DMA at link rate
+        frame->pool = d->framePool;
+        frame->data = d->framePool->get();
+        if (frame->data == nullptr) {
+            sub.st[j] = fail(CHD_E_OOM, "chd_decode_frame: no pinned host memory for the frame");
+            continue;
+        }
+        e = hipMemcpyAsync(frame->data, static_cast<char *>(w.d_out[slot]) + (size_t)k * fb, bytes,
+                           hipMemcpyDeviceToHost, w.copy_stream);
+        sub.frame[j] = std::move(frame);
+    }
+    if (e == hipSuccess) e = hipEventRecord(w.ev_copy[slot], w.copy_stream);
+    if (e != hipSuccess) {
+        // the pinned buffers go back to the pool only once nothing writes them
+        (void)hipStreamSynchronize(w.copy_stream);
+        for (auto &f : sub.frame) f.reset();
+        return fail_all(fail(CHD_E_INTERNAL, "chd_decode_frame: GPU decode failed: " + hip_err(e)));
+    }
+    w.copy_busy[slot] = true;
+}
+
+// Waits for submission slot `slot`'s downloads and hands its frames, in
+// order, to take(status, frame_index, frame).
+template <class Take>
+void finish_frames(Worker &w, int slot, Submission &sub, Take &&take)
+{
+    hipError_t e = hipSuccess;
+    if (w.copy_busy[slot]) {
+        e = wait_event(w.ev_copy[slot]);
+        w.copy_busy[slot] = false;
+    }
+    const chd_status_t gpu = e == hipSuccess ? CHD_OK
+                                             : fail(CHD_E_INTERNAL, "chd_decode_frame: GPU decode failed: " + hip_err(e));
+    for (int j = 0; j < sub.n; ++j) {
+        chd_status_t st = sub.st[j] != CHD_OK ? sub.st[j] : gpu;
+        chd_frame_t *f = nullptr;
+        if (st == CHD_OK) f = sub.frame[j].release();
+        sub.frame[j].reset();
+        take(st, sub.idx[j], f);
+    }
+    sub.busy = false;
+}
+
+chd_status_t decode_locked(chd_decoder_t *d, Worker &w, int64_t frame_index, chd_frame_t **out)
+{
+    Submission sub;
+    submit_frames(d, w, &frame_index, 1, 0, sub);
+    chd_status_t rc = CHD_OK;
+    finish_frames(w, 0, sub, [&](chd_status_t st, int64_t, chd_frame_t *f) {
+        rc = st;
+        *out = f;
+    });
+    return rc;
 }
 
 int parse_format(const std::string &n, chd_pixel_format_t &f)
@@ -893,26 +1073,41 @@ chd_status_t chd_decode_frames_async(chd_decoder_t *d, const int64_t *indices, s
     std::vector<std::thread> threads;
     threads.reserve(W);
     for (size_t wi = 0; wi < W; ++wi) {
-        Worker *w = get_worker(d, wi);
+        // async workers are 1..W: worker 0 is chd_decode_frame's, so a callback
+        // that decodes a frame synchronously does not wait on its own worker
+        Worker *w = get_worker(d, wi + 1);
         threads.emplace_back([d, w, indices, n, run, cb, user, cancel, &next]() {
+            // two submissions in flight (slots alternate), each of up to
+            // kBatch consecutive frame indices; a submission's frames are
+            // delivered after the next one is submitted, in list order
+            Submission sub[2];
+            int k = 0;
+            auto deliver = [&](int s) {
+                finish_frames(*w, s, sub[s], [&](chd_status_t st, int64_t idx, chd_frame_t *f) { cb(user, st, idx, f); });
+            };
+            std::lock_guard<std::mutex> lk(w->mu);
             for (;;) {
                 const size_t i0 = next.fetch_add(run, std::memory_order_relaxed);
-                if (i0 >= n) return;
-                for (size_t i = i0; i < std::min(n, i0 + run); ++i) {
-                    const int64_t idx = indices[i];
+                if (i0 >= n) break;
+                const size_t end = std::min(n, i0 + run);
+                for (size_t i = i0; i < end;) {
+                    int nb = 1;
+                    while (nb < kBatch && i + nb < end && indices[i + nb] == indices[i + nb - 1] + 1) ++nb;
+                    const int s = k & 1;
+                    if (sub[s].busy) deliver(s);
                     if (cancel != nullptr && cancel->requested.load(std::memory_order_acquire)) {
-                        cb(user, CHD_E_CANCELLED, idx, nullptr);
-                        continue;
+                        if (sub[s ^ 1].busy) deliver(s ^ 1);   // keep list order
+                        for (int j = 0; j < nb; ++j) cb(user, CHD_E_CANCELLED, indices[i + j], nullptr);
+                    } else {
+                        submit_frames(d, *w, indices + i, nb, s, sub[s]);
+                        ++k;
                     }
-                    chd_frame_t *frame = nullptr;
-                    chd_status_t rc;
-                    {
-                        std::lock_guard<std::mutex> lk(w->mu);
-                        rc = decode_locked(d, *w, idx, &frame);
-                    }
-                    cb(user, rc, idx, frame);
+                    i += nb;
                 }
             }
+            // the last submissions, oldest first
+            if (sub[k & 1].busy) deliver(k & 1);
+            if (sub[(k + 1) & 1].busy) deliver((k + 1) & 1);
         });
     }
     for (auto &t : threads) t.join();

@@ -135,6 +135,20 @@ inline int64_t raw_frame_floats_host(const Geom &g) { return 3 * (int64_t)(g.las
 // shift: samples dropped from the start of each field (black appended)
 void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst, int n_fields,
                  hipStream_t s, int shift = 0);
+// Fields staged contiguously (src_field_elems apart), each packed into its
+// own slot of a packed store: field k goes to dst + slot[k] * g.field_elems,
+// shifted by shift[k] samples as launch_pack does.
+constexpr int kPackListMax = 32;
+struct PackList {
+    int n;
+    int32_t slot[kPackListMax];
+    int32_t shift[kPackListMax];
+};
+void launch_pack_list(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst,
+                      const PackList &list, hipStream_t s);
+// `bytes` from pinned host memory (its device address) into device memory,
+// read by a kernel over the bus, so no DMA engine carries it.
+void launch_copy_in(const void *src, void *dst, size_t bytes, hipStream_t s);
 void launch_fill_border(const Geom &g, void *out, int64_t n_frames, hipStream_t s);
 // the active lines of frame-line parity `parity` of one frame, as the zero
 // Y/U/V no field was decoded into (frames whose fields share one offset)

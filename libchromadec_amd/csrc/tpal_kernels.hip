@@ -1751,24 +1751,26 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
 // ---------------------------------------------------------------------------
 
 constexpr int kQ4Lines = 32;
+constexpr int kQ4ShortLines = 8;                       // lines per workgroup of small launches
 constexpr int kQ4Cols = 4 * kDemodThreads;             // columns held by the threads
 constexpr int kQ4MaxWidth = kQ4Cols - 2 * kQHalo;       // widest active line: halos on both sides
 constexpr int kQ4Row4 = kDemodThreads + 2;              // float4 per t row (+ 8 zero columns)
 
+template <int LINES>
 struct LdsQ4 {
     float4 t[2][4][kQ4Row4];
-    float bp[kQ4Lines], bq[kQ4Lines], vsw[kQ4Lines];
+    float bp[LINES], bq[LINES], vsw[LINES];
 };
 
 __device__ __forceinline__ float4 f4add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 __device__ __forceinline__ float f4get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
 
-template <int NC, bool RAW>
+template <int NC, bool RAW, int LINES>
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
                 const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, float *raw)
 {
-    __shared__ LdsQ4 L;
+    __shared__ LdsQ4<LINES> L;
     const int tid = threadIdx.x;
     const int lb = blockIdx.x % n_lblocks;
     const int fsel = (blockIdx.x / n_lblocks) & 1;
@@ -1781,9 +1783,9 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     const uint16_t *comp = job.field[fsel];
     const int firstLine = (g.first + 1 - off) / 2;
     const int lastLine = ((g.last - off) / 2) + 1;
-    const int n0 = firstLine + lb * kQ4Lines;
+    const int n0 = firstLine + lb * LINES;
     if (n0 >= lastLine) return;
-    const int nl = min(kQ4Lines, lastLine - n0);
+    const int nl = min(LINES, lastLine - n0);
     const int fs = g.filter_size;
     const int c0 = 4 * tid - kQHalo;         // this thread's first column, from active_video_start
     constexpr bool pal2d = NC == 0;
@@ -2051,6 +2053,35 @@ __global__ void pack_fields(const Geom g, const uint16_t *__restrict__ src, int6
     }
 }
 
+// The same for a list of fields: staged field blockIdx.y goes to its own slot.
+__global__ void pack_field_list(const Geom g, const uint16_t *__restrict__ src, int64_t src_field_elems,
+                                uint16_t *__restrict__ dst, const PackList list)
+{
+    const int k = blockIdx.y;
+    const uint16_t *f = src + (int64_t)k * src_field_elems;
+    uint16_t *out = dst + (int64_t)list.slot[k] * g.field_elems;
+    const int shift = list.shift[k];
+    const int64_t samples = (int64_t)g.fh * g.fw;
+    const uint16_t blk = (uint16_t)g.black;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < g.field_elems;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int line = (int)(r / g.pitch);
+        const int col = (int)(r - (int64_t)line * g.pitch) - g.lmargin;
+        const int64_t at = (int64_t)line * g.fw + col + shift;
+        out[r] = (col >= 0 && col < g.fw && at < samples) ? f[at] : blk;
+    }
+}
+
+// Bulk copy in 16-byte words; the last bytes of a length that is not a
+// multiple of 16 go one by one.
+__global__ void copy_in(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16,
+                        const uint8_t *__restrict__ src_tail, uint8_t *__restrict__ dst_tail, int tail)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+    if (blockIdx.x == 0 && (int)threadIdx.x < tail) dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
 // Chroma fields (field_height x field_width floats each) into the partial
 // plane layout the demodulator reads (columns of the active samples only).
 __global__ void chroma_to_planes(const Geom g, const float *__restrict__ src, float *__restrict__ dst,
@@ -2218,14 +2249,21 @@ static void demod_kernels(const Geom &g, const FrameJob *jobs, int n_jobs, const
     const bool composite = nchroma == 0;
     const int n_segs = (g.aw + kDemodThreads - 1) / kDemodThreads;
     if (g.quad && g.aw <= kQ4MaxWidth) {
-        const int nlb = (n_lblocks * kDemodLines + kQ4Lines - 1) / kQ4Lines;
+        // A few frames (one chd_decode_frame window) give too few 32-line
+        // workgroups to fill the chip, and each runs for the whole launch:
+        // 8-line workgroups cut that latency at the cost of more halo rows.
+        const bool short_blocks = (int64_t)n_jobs * 2 * ((n_lblocks * kDemodLines + kQ4Lines - 1) / kQ4Lines) < 256;
+        const int lines = short_blocks ? kQ4ShortLines : kQ4Lines;
+        const int nlb = (n_lblocks * kDemodLines + lines - 1) / lines;
         const int blocks = n_jobs * 2 * nlb;
         if (blocks > 0) {
-            auto kern = raw != nullptr
-                            ? (nchroma == 4 ? pal_demod_quad4<4, true> : nchroma == 2 ? pal_demod_quad4<2, true>
-                               : nchroma == 1 ? pal_demod_quad4<1, true> : pal_demod_quad4<0, true>)
-                            : (nchroma == 4 ? pal_demod_quad4<4, false> : nchroma == 2 ? pal_demod_quad4<2, false>
-                               : nchroma == 1 ? pal_demod_quad4<1, false> : pal_demod_quad4<0, false>);
+#define CHD_Q4(L)                                                                                              \
+    (raw != nullptr ? (nchroma == 4 ? pal_demod_quad4<4, true, L> : nchroma == 2 ? pal_demod_quad4<2, true, L>     \
+                       : nchroma == 1 ? pal_demod_quad4<1, true, L> : pal_demod_quad4<0, true, L>)               \
+                    : (nchroma == 4 ? pal_demod_quad4<4, false, L> : nchroma == 2 ? pal_demod_quad4<2, false, L>   \
+                       : nchroma == 1 ? pal_demod_quad4<1, false, L> : pal_demod_quad4<0, false, L>))
+            auto kern = short_blocks ? CHD_Q4(kQ4ShortLines) : CHD_Q4(kQ4Lines);
+#undef CHD_Q4
             hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt, nlb, raw);
         }
         return;
@@ -2260,6 +2298,22 @@ void launch_pack(const Geom &g, const uint16_t *src, int64_t src_field_elems, ui
 {
     if (n_fields > 0)
         hipLaunchKernelGGL(pack_fields, dim3(1024), dim3(256), 0, s, g, src, src_field_elems, dst, n_fields, shift);
+}
+
+void launch_copy_in(const void *src, void *dst, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return;
+    const int64_t n16 = (int64_t)(bytes / 16);
+    const int tail = (int)(bytes % 16);
+    hipLaunchKernelGGL(copy_in, dim3(512), dim3(256), 0, s, static_cast<const uint4 *>(src), static_cast<uint4 *>(dst),
+                       n16, static_cast<const uint8_t *>(src) + n16 * 16, static_cast<uint8_t *>(dst) + n16 * 16, tail);
+}
+
+void launch_pack_list(const Geom &g, const uint16_t *src, int64_t src_field_elems, uint16_t *dst,
+                      const PackList &list, hipStream_t s)
+{
+    if (list.n > 0 && list.n <= kPackListMax)
+        hipLaunchKernelGGL(pack_field_list, dim3(256, list.n), dim3(256), 0, s, g, src, src_field_elems, dst, list);
 }
 
 void launch_chroma_to_planes(const Geom &g, const float *src, float *dst, int64_t plane_stride, int n_fields,

@@ -112,3 +112,20 @@ def test_overlapping_pairs_conceal_each_pair(orc, tmp_path):
         assert np.abs(got.astype(np.float64) - want).max() <= 1e-4, f
     dec.close()
     v.close()
+
+
+@pytest.mark.parametrize("kind", [abi.CHD_DEC_TRANSFORM_3D, abi.CHD_DEC_TRANSFORM_2D])
+def test_async_batches_with_concealment_match_sync(dropout_capture, kind):
+    """Async workers decode runs of consecutive frames as one submission
+    (concealed fields staged and packed together): each frame equals its
+    synchronous decode with the same concealment."""
+    path, _, _ = dropout_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, kind, output_format="yuv444ps", thread_count=2).set_dropout(True).commit()
+    want = {f: dec.decode(f) for f in range(5)}
+    got = dec.decode_async([0, 1, 2, 3, 4, 4, 3, 2, 1, 0])
+    assert sorted(got) == list(range(5))
+    for f in range(5):
+        assert np.array_equal(got[f], want[f]), f
+    dec.close()
+    v.close()

@@ -430,6 +430,40 @@ def test_async_matches_sync(patch_capture, kind, nr):
     abi.lib().chd_cancel_free(c)
 
 
+def test_async_one_worker_two_in_flight_keeps_order(patch_capture):
+    """One async worker keeps two frames in flight (one downloading while the
+    next decodes, into alternating device buffers): every frame is still the
+    synchronous one, delivered in list order, and a callback may call
+    chd_decode_frame itself."""
+    import ctypes as C
+    import threading
+    path, _ = patch_capture
+    v = abi.Video(path)
+    dec = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps", thread_count=1).commit()
+    want = {i: dec.decode(i) for i in range(6)}
+    order, got, inner = [], {}, []
+    mu = threading.Lock()
+
+    def done(user, status, idx, frame):
+        fr = abi.Frame(C.c_void_p(frame))
+        with mu:
+            order.append(idx)
+            got.setdefault(idx, []).append(fr.planes())
+        fr.free()
+        if idx == 3 and not inner:
+            inner.append(dec.decode(4))   # re-entrant synchronous decode
+
+    cb = abi.FRAME_DONE_CB(done)
+    idx = [0, 1, 2, 3, 4, 5, 5, 2, 0]
+    arr = (C.c_int64 * len(idx))(*idx)
+    abi.check(abi.lib().chd_decode_frames_async(dec.h, arr, len(idx), cb, None, None), "chd_decode_frames_async")
+    assert order == idx
+    for i, frames in got.items():
+        for f in frames:
+            assert np.array_equal(f, want[i])
+    assert np.array_equal(inner[0], want[4])
+
+
 @pytest.mark.parametrize("kind", [6, 7, 8])
 def test_device_api_matches_abi_bitwise(patch_capture, kind):
     """chd_tpal_decode shares each 8-field slab between frames f and f+2;
