@@ -187,39 +187,42 @@ struct Worker {
 
 }  // namespace
 
-// Pinned frame buffers: at most kPoolKeep idle ones are kept (freeing one
-// synchronises the device).  Async decode holds two per worker in flight.
-constexpr size_t kPoolKeep = 32;
+// Idle pinned blocks kept (freeing one synchronises the device): async
+// decode holds up to two batch blocks per worker in flight.
+constexpr size_t kPoolKeepOne = 32;
+constexpr size_t kPoolKeepBatch = 2 * kMaxGpuWorkers + 2;
 
 FramePool::~FramePool()
 {
-    for (void *p : free_list) (void)hipHostFree(p);
+    for (void *p : free_one) (void)hipHostFree(p);
+    for (void *p : free_batch) (void)hipHostFree(p);
 }
 
-void *FramePool::get()
+std::shared_ptr<uint8_t> FramePool::get(int n)
 {
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        if (!free_list.empty()) {
-            void *p = free_list.back();
-            free_list.pop_back();
-            return p;
-        }
-    }
+    const bool one = n == 1;
+    std::vector<void *> &list = one ? free_one : free_batch;
     void *p = nullptr;
-    return hipHostMalloc(&p, bytes, 0) == hipSuccess ? p : nullptr;
-}
-
-void FramePool::put(void *p)
-{
     {
         std::lock_guard<std::mutex> lk(mu);
-        if (free_list.size() < kPoolKeep) {
-            free_list.push_back(p);
-            return;
+        if (!list.empty()) {
+            p = list.back();
+            list.pop_back();
         }
     }
-    (void)hipHostFree(p);
+    if (p == nullptr && hipHostMalloc(&p, bytes * (one ? 1 : batch), 0) != hipSuccess) return nullptr;
+    auto self = shared_from_this();
+    return std::shared_ptr<uint8_t>(static_cast<uint8_t *>(p), [self, one](uint8_t *q) {
+        {
+            std::lock_guard<std::mutex> lk(self->mu);
+            std::vector<void *> &l = one ? self->free_one : self->free_batch;
+            if (l.size() < (one ? kPoolKeepOne : kPoolKeepBatch)) {
+                l.push_back(q);
+                return;
+            }
+        }
+        (void)hipHostFree(q);
+    });
 }
 
 struct chd_decoder {
@@ -654,6 +657,19 @@ void submit_frames(chd_decoder_t *d, Worker &w, const int64_t *idx, int nb, int 
     const size_t bytes = (size_t)d->planes * d->outW * d->outH * (is_float ? sizeof(float) : sizeof(uint16_t));
     hipError_t e = hipEventRecord(w.ev_dec[slot], w.stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(w.copy_stream, w.ev_dec[slot], 0);
+    // a full batch comes down in one copy into one block (the decoded frames
+    // lie back to back in d_out); otherwise each frame in its own block
+    const bool whole = nw == kBatch && bytes == fb;
+    std::shared_ptr<uint8_t> block;
+    if (whole) {
+        block = d->framePool->get(kBatch);
+        if (block == nullptr) {
+            for (int k = 0; k < nw; ++k) sub.st[at[k]] = fail(CHD_E_OOM, "chd_decode_frame: no pinned host memory for the frame");
+            nw = 0;
+        } else if (e == hipSuccess) {
+            e = hipMemcpyAsync(block.get(), w.d_out[slot], (size_t)nw * fb, hipMemcpyDeviceToHost, w.copy_stream);
+        }
+    }
     for (int k = 0; k < nw && e == hipSuccess; ++k) {
         const int j = at[k];
         auto frame = std::make_unique<chd_frame>();
@@ -666,15 +682,20 @@ void submit_frames(chd_decoder_t *d, Worker &w, const int64_t *idx, int nb, int 
         frame->info.height = d->outH;
         frame->info.num_planes = d->planes;
         frame->info.frame_index = idx[j];
-        // straight into a pinned frame buffer: a single DMA at link rate
-        frame->pool = d->framePool;
-        frame->data = d->framePool->get();
-        if (frame->data == nullptr) {
-            sub.st[j] = fail(CHD_E_OOM, "chd_decode_frame: no pinned host memory for the frame");
-            continue;
+        // straight into a pinned block: DMA at link rate
+        if (whole) {
+            frame->block = block;
+            frame->data = block.get() + (size_t)k * fb;
+        } else {
+            frame->block = d->framePool->get(1);
+            if (frame->block == nullptr) {
+                sub.st[j] = fail(CHD_E_OOM, "chd_decode_frame: no pinned host memory for the frame");
+                continue;
+            }
+            frame->data = frame->block.get();
+            e = hipMemcpyAsync(frame->data, static_cast<char *>(w.d_out[slot]) + (size_t)k * fb, bytes,
+                               hipMemcpyDeviceToHost, w.copy_stream);
         }
-        e = hipMemcpyAsync(frame->data, static_cast<char *>(w.d_out[slot]) + (size_t)k * fb, bytes,
-                           hipMemcpyDeviceToHost, w.copy_stream);
         sub.frame[j] = std::move(frame);
     }
     if (e == hipSuccess) e = hipEventRecord(w.ev_copy[slot], w.copy_stream);
@@ -1012,10 +1033,11 @@ chd_status_t chd_decoder_commit(chd_decoder_t *d)
     d->threadCount = threads;
 
     d->framePool = std::make_shared<FramePool>((size_t)d->planes * d->outW * d->outH *
-                                               (d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS ||
-                                                        d->format == CHD_PIXEL_RGBS
-                                                    ? sizeof(float)
-                                                    : sizeof(uint16_t)));
+                                                   (d->format == CHD_PIXEL_YUV444PS || d->format == CHD_PIXEL_GRAYS ||
+                                                            d->format == CHD_PIXEL_RGBS
+                                                        ? sizeof(float)
+                                                        : sizeof(uint16_t)),
+                                               kBatch);
     // Build worker 0 now so a missing GPU fails here, not at first decode.
     Worker *w0 = get_worker(d, 0);
     chd_status_t st = worker_init(d, *w0, "chd_decoder_commit");

@@ -87,33 +87,31 @@ struct chd_cancel {
     std::atomic<bool> requested{false};
 };
 
-// Pinned host buffers of one frame size, shared by a decoder and the frames
-// it returns (which may outlive it).  A decoded frame is downloaded straight
-// into one; chd_frame_free returns it.  Reuse avoids a fresh 6 MB mapping,
-// its page faults and a bounce copy per frame.
-struct FramePool {
-    explicit FramePool(size_t b) : bytes(b) {}
+// Pinned host blocks of one frame size, shared by a decoder and the frames
+// it returns (which may outlive it).  Decoded frames are downloaded straight
+// into a block: one frame, or a batch of consecutive frames in one copy.  A
+// block returns to the pool when the last frame cut from it is freed.  Reuse
+// avoids a fresh 6 MB mapping, its page faults and a bounce copy per frame.
+struct FramePool : std::enable_shared_from_this<FramePool> {
+    FramePool(size_t frame_bytes, int batch) : bytes(frame_bytes), batch(batch) {}
     ~FramePool();
-    void *get();              // nullptr when pinned memory cannot be had
-    void put(void *p);
+    // `n` frames (1 or batch) in one block; null when pinned memory cannot be had
+    std::shared_ptr<uint8_t> get(int n);
     const size_t bytes;
+    const int batch;
     std::mutex mu;
-    std::vector<void *> free_list;
+    std::vector<void *> free_one, free_batch;
 };
 
 struct chd_frame {
     chd_frame_info_t info{};
     chd_pixel_format_t format = CHD_PIXEL_YUV444P16;
     int32_t width = 0, height = 0, planes = 0;
-    std::shared_ptr<FramePool> pool;
+    std::shared_ptr<uint8_t> block;   // the pinned block `data` lies in
     void *data = nullptr;     // planes x height x width, float or uint16 by format
     chd_frame() = default;
     chd_frame(const chd_frame &) = delete;
     chd_frame &operator=(const chd_frame &) = delete;
-    ~chd_frame()
-    {
-        if (data != nullptr) pool->put(data);
-    }
     const float *f32() const { return static_cast<const float *>(data); }
     const uint16_t *u16() const { return static_cast<const uint16_t *>(data); }
 };
