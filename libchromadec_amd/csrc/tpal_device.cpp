@@ -46,9 +46,18 @@ tpal::Tables make_tables()
     for (int x = 0; x < 16; ++x) { t.wx_d[x] = window(x, 16); t.wx[x] = (float)t.wx_d[x]; }
     for (int y = 0; y < 16; ++y) { t.wy2_d[y] = window(y, 16); t.wy2[y] = (float)t.wy2_d[y]; }
     for (int x = 0; x < 32; ++x) { t.wx2_d[x] = window(x, 32); t.wx2[x] = (float)t.wx2_d[x]; }
+    // cos / sin (2 pi j / 32) from the first octant by the exact symmetries
+    // of the circle: the quarter turns are exact and entry j + 16 is exactly
+    // minus entry j, so the FP64 re-decision of a visit and of its image
+    // half a period away in ky and kz gives bins of exactly opposite sign
+    double c8[9];
+    for (int r = 0; r <= 8; ++r) c8[r] = r == 0 ? 1.0 : r == 8 ? 0.0 : std::cos(2.0 * M_PI * r / 32.0);
     for (int j = 0; j < 32; ++j) {
-        t.cos32_d[j] = std::cos(2.0 * M_PI * j / 32.0);
-        t.sin32_d[j] = std::sin(2.0 * M_PI * j / 32.0);
+        const int q = j / 8, r = j % 8;
+        const double c = q == 0 ? c8[r] : q == 1 ? -c8[8 - r] : q == 2 ? -c8[r] : c8[8 - r];
+        const double s = q == 0 ? c8[8 - r] : q == 1 ? c8[r] : q == 2 ? -c8[8 - r] : -c8[r];
+        t.cos32_d[j] = c;
+        t.sin32_d[j] = s;
     }
     return t;
 }
@@ -363,7 +372,8 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int 
         tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply,
                             plan->d_spec, plan->d_mask);
     } else if (plan->kind == CHD_DEC_TRANSFORM_3D)
-        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s);
+        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTilePlain, nullptr,
+                            nullptr, 0, 0, nullptr, tpal::slabs_alternate(plan->h_slab_jobs, n_units));
     else if (plan->kind == CHD_DEC_TRANSFORM_2D)
         tpal::launch_tpal2d(g, ws.field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
     if (plan->timing) hipEventRecord(plan->ev[1], s);
@@ -829,6 +839,7 @@ struct chd_tpal_batch {
     void *d_out = nullptr;
     bool border = false;
     int strip = tpal::kStrip3, groups = 0;   // strip: set at creation
+    bool half = false;             // every slab alternates in line parity
     double tile_ms = 0;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
@@ -863,7 +874,7 @@ void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
     }
     if (plan->net == nullptr) {
         tpal::launch_tpal3d(g, b->d_slab, b->n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTilePlain, nullptr,
-                            nullptr, strip, groups, b->d_wq);
+                            nullptr, strip, groups, b->d_wq, b->half);
         return;
     }
     chd_mask_net *net = plan->net;
@@ -1004,6 +1015,7 @@ chd_status_t chd_tpal_batch_create(chd_tpal_plan_t *plan, const chd_tpal_sequenc
     }
     if (e == hipSuccess && g.ynr > 0.0f)
         e = hipMalloc((void **)&b->d_raw, (size_t)b->n_frames * tpal::raw_frame_floats_host(g) * sizeof(float));
+    if (is3d) b->half = tpal::slabs_alternate(slabs.data(), (int)slabs.size());
     if (e == hipSuccess && is3d) e = hipMalloc((void **)&b->d_slab, slabs.size() * sizeof(tpal::SlabJob));
     if (e == hipSuccess && !fjobs.empty())
         e = hipMalloc((void **)&b->d_field, fjobs.size() * sizeof(tpal::FieldJob2D));

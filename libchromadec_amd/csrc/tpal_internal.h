@@ -118,10 +118,21 @@ constexpr int kTileBins = 5 * 32 * 8;
 // wq: 9 zeroed device counters (work items taken per XCD, workgroups done),
 // which every launch leaves zeroed again; launches sharing them must not
 // overlap.  Without wq the launch is one workgroup per item.
+// half (plain mode): every slab's fields alternate in line parity
+// (slabs_alternate) and g.thr_uniform >= 0, so the kernel that derives the
+// rows ky >= 16 of each tile from the rows ky < 16 applies.
 constexpr int kWorkCounters = 9;
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
                    hipStream_t s, int mode = kTilePlain, void *spec = nullptr, const float *mask = nullptr,
-                   int strip = 0, int max_groups = 0, unsigned *wq = nullptr);
+                   int strip = 0, int max_groups = 0, unsigned *wq = nullptr, bool half = false);
+// True when every slab's 8 fields alternate in line parity (host copies).
+inline bool slabs_alternate(const SlabJob *jobs, int n_jobs)
+{
+    for (int i = 0; i < n_jobs; ++i)
+        for (int z = 0; z + 1 < kZT; ++z)
+            if (((jobs[i].offset[z] ^ jobs[i].offset[z + 1]) & 1) == 0) return false;
+    return true;
+}
 int choose_strip3(const Geom &g, int n_jobs);
 void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_max, const float *thr32,
                    const double *thr64, hipStream_t s);

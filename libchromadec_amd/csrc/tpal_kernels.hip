@@ -451,20 +451,22 @@ __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
 // visit (`mask`, [tile][768]) and run the inverse and overlap-add.
 // Work item = one strip of `strip` tiles of one tile row of one slab.  The
 // grid is either one workgroup per item (PERSIST false) or persistent (a
-// multiple of 8 workgroups, each looping over items).
-template <int MODE, bool PERSIST>
+// multiple of 8 workgroups, each looping over items).  HALF (plain mode):
+// every slab alternates in line parity and one threshold applies to every
+// visit.
+template <int MODE, bool PERSIST, bool HALF>
 __global__ void __launch_bounds__(64, 3)   // 3 waves per SIMD: at most 168 VGPRs
 tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int strip_len, const float *__restrict__ thr32,
              const double *__restrict__ thr64, float2 *__restrict__ spec, const float *__restrict__ mask,
              unsigned *__restrict__ wq)
 {
     __shared__ typename LdsFor<MODE>::T L;
-    const int lane = threadIdx.x;
-    if (lane < 32) {
-        L.cs[lane] = c_tab.cos32_d[lane];
-        L.sn[lane] = c_tab.sin32_d[lane];
-        L.wy[lane] = c_tab.wy_d[lane];
-        if (lane < 8) L.wz[lane] = c_tab.wz_d[lane];
+    const int lane0 = threadIdx.x;
+    if (lane0 < 32) {
+        L.cs[lane0] = c_tab.cos32_d[lane0];
+        L.sn[lane0] = c_tab.sin32_d[lane0];
+        L.wy[lane0] = c_tab.wy_d[lane0];
+        if (lane0 < 8) L.wz[lane0] = c_tab.wz_d[lane0];
     }
     // Workgroups are dispatched round-robin over the 8 XCDs, each with its
     // own L2.  Give each XCD a contiguous share of the items, ordered slab
@@ -487,13 +489,18 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     auto fetch = [&]() -> int {
         if constexpr (PERSIST) {
             unsigned k = 0;
-            if (lane == 0) k = atomicAdd(&wq[xcd], 1u);
+            if (lane0 == 0) k = atomicAdd(&wq[xcd], 1u);
             return lo + (int)__builtin_amdgcn_readfirstlane(k);
         } else {
             return lo + (blockIdx.x >> 3);
         }
     };
     for (int w = fetch(); w < hi; w = PERSIST ? fetch() : hi) {
+    // A persistent workgroup recomputes its per-lane setup for each item
+    // from an opaque copy of the lane index: hoisted out of the item loop,
+    // that setup would stay live through every tile and spill.
+    int lane = lane0;
+    if constexpr (PERSIST) asm volatile("" : "+v"(lane));
     const int job_id = w % n_jobs;
     const int rest = w / n_jobs;
     const int ty = rest % g.n_ty3;
@@ -539,8 +546,23 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     // capture): row ky+16 of plane z is (-1)^par[z] = sgn (-1)^z times row ky,
     // so after the temporal DFT T[ky+16][kz] = sgn T[ky][(kz+4) & 7] and only
     // rows 0..15 need the vertical stores and the temporal DFT.
-    const bool alt = pbits == 0xAA || pbits == 0x55;
+    constexpr bool kHalf = HALF && MODE == kTilePlain;   // the launcher checked the parity
+    const bool alt = kHalf || pbits == 0xAA || pbits == 0x55;
     const float sgn = par[0] ? -1.0f : 1.0f;
+    // With alternating parity and one threshold for every visit, the rows
+    // ky >= 16 follow from the rows ky < 16 all the way through:
+    //   * visit (z + 4, y + 16, x) compares the magnitudes of visit (z, y, x)
+    //     (its bins and their mirrors are +-the same FP32 values), so its
+    //     keep decision is the same, and the FP64 re-decision (twiddles
+    //     antisymmetric over half a period) agrees too: keep word (x, z + 4)
+    //     is keep word (x, z) rotated by 16 rows;
+    //   * the masked inverse over z then gives T[ky + 16][z] =
+    //     sgn (-1)^z T[ky][z], and for every output plane the vertical fold
+    //     T[ky] + (-1)^p T[ky + 16] is 2 T[ky].
+    // So only rows ky < 16 are stored, only planes z < 4 decided and only
+    // rows ky < 16 inverted over z.  (A kernel of its own: the general form
+    // beside it would raise the register count of both.)
+    constexpr bool half = kHalf;
     // exp(-2 pi i ky / 32) for this lane's temporal-stage ky (lane & 15; exact
     // at the quarter turns)
     const c32 ptw = {cos32(lane & 15), -sin32(lane & 15)};
@@ -676,10 +698,15 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                         for (int z = 1; z < 8; z += 2) v[z] = turn(v[z]);
                     }
                     fft<8, -1>(v);
+                    if (half) {
 #pragma unroll
-                    for (int z = 0; z < 8; ++z) {
-                        Sat(L, k, ky, z) = v[z];
-                        Sat(L, k, ky + 16, z) = c32{sgn * v[(z + 4) & 7].x, sgn * v[(z + 4) & 7].y};
+                        for (int z = 0; z < 8; ++z) Sat(L, k, ky, z) = v[z];
+                    } else {
+#pragma unroll
+                        for (int z = 0; z < 8; ++z) {
+                            Sat(L, k, ky, z) = v[z];
+                            Sat(L, k, ky + 16, z) = c32{sgn * v[(z + 4) & 7].x, sgn * v[(z + 4) & 7].y};
+                        }
                     }
                 }
             } else {
@@ -769,6 +796,57 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                         __builtin_amdgcn_sched_barrier(0);
                     }
                 };
+                // The half form: planes z < 4 only, rows ky >= 16 read from
+                // their stored images, rows ky - 16 at kz + 4 (bins of equal
+                // magnitude).  Steps 0-3 decide (x, z) = (2, s) on lanes 0-31
+                // and (3, s) on lanes 32-63; step 4 (4, 0) and (4, 1); step 5
+                // (4, 3).  Plane (4, 2) holds the pairs of (4, 0), a and b
+                // swapped, as in the full form.
+                auto steps_half = [&]() {
+                    const int yr = (40 - vy) & 31;
+                    const int a0 = (vy & 15) * kSpad + 4 * (vy >> 4);
+                    // mirror z' = (10 - z) % 8 is 2 - z for z < 3 and 7 for
+                    // z = 3; a mirror row yr >= 16 adds 4 (mod 8) to it
+                    const int b0 = (yr & 15) * kSpad + 4 * (yr >> 4), b0m = b0 - 8 * (yr >> 4);
+                    auto off_a = [&](int st) {
+                        return st < 4 ? a0 + vhalf * kSk + st : st == 4 ? a0 + 2 * kSk + vhalf : a0 + 2 * kSk + 3;
+                    };
+                    auto off_b = [&](int st) {
+                        return st < 3    ? b0 - vhalf * kSk + 4 * kSk + (2 - st)
+                               : st == 3 ? b0m - vhalf * kSk + 4 * kSk + 7
+                               : st == 4 ? b0 - vhalf + 2 * kSk + 2
+                                         : b0m + 2 * kSk + 7;
+                    };
+                    const DecideConst q = decide_const(erru, g.thr_uniform);
+                    c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
+#pragma unroll
+                    for (int st = 0; st < 6; ++st) {
+                        c32 an = a, bn = b;
+                        if (st + 1 < 6) {
+                            an = L.u.S[off_a(st + 1)];
+                            bn = L.u.S[off_b(st + 1)];
+                        }
+                        uint64_t kb, fb;
+                        decide32_ballots(a, b, q, kb, fb);
+                        if (lane == 0) {
+                            const int wlo = st < 4 ? pw(0, st) : st == 4 ? pw(2, 0) : pw(2, 3);
+                            L.bal[wlo] = (uint32_t)kb;
+                            L.flg[wlo] = (uint32_t)fb;
+                            if (st < 5) {
+                                const int whi = st < 4 ? pw(1, st) : pw(2, 1);
+                                L.bal[whi] = (uint32_t)(kb >> 32);
+                                L.flg[whi] = (uint32_t)(fb >> 32);
+                            }
+                            if (st == 4) {
+                                L.bal[pw(2, 2)] = mirror_rows((uint32_t)kb);
+                                L.flg[pw(2, 2)] = 0u;
+                            }
+                        }
+                        a = an;
+                        b = bn;
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                };
                 if (erru == 0.0f) {
                     // every windowed sample is zero, so is every bin: all pairs
                     // are kept (0 < 0 is false), none needs re-deciding
@@ -776,6 +854,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                         L.bal[lane] = ~0u;
                         L.flg[lane] = 0u;
                     }
+                } else if (half) {
+                    steps_half();
                 } else if (g.thr_uniform >= 0.0f) {
                     steps(std::true_type{});
                 } else {
@@ -788,16 +868,30 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             // (x_v = 8 - x, z_v = (10 - z) % 8); plus the self-mirror bins
             // (x = 4, y in {4, 20}, z in {1, 5}), which are always kept.
             constexpr uint32_t kSelf = (1u << 4) | (1u << 20);
+            auto rot16 = [](uint32_t w) { return (w << 16) | (w >> 16); };
             if (lane < 40) {
                 const int k = lane >> 3, z = lane & 7;
-                uint32_t w = (k <= 2) ? L.bal[pw(k, z)] : 0u;
-                if (k >= 2) w |= mirror_rows(L.bal[pw(4 - k, (10 - z) & 7)]);
-                if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
+                uint32_t w;
+                if (half) {
+                    // from the planes z < 4; plane (x, 7) is plane (x, 3)
+                    // rotated, and word (x, z + 4) is word (x, z) rotated
+                    const int zl = z & 3;
+                    w = (k <= 2) ? L.bal[pw(k, zl)] : 0u;
+                    if (k >= 2) w |= mirror_rows(zl < 3 ? L.bal[pw(4 - k, 2 - zl)] : rot16(L.bal[pw(4 - k, 3)]));
+                    if (k == 2 && zl == 1) w |= kSelf;
+                    if (z >= 4) w = rot16(w);
+                } else {
+                    w = (k <= 2) ? L.bal[pw(k, z)] : 0u;
+                    if (k >= 2) w |= mirror_rows(L.bal[pw(4 - k, (10 - z) & 7)]);
+                    if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
+                }
                 L.keep[lane] = w;
             }
-            // flag words (x - 2) * 8 + z on lanes 0-23, self-mirror bins removed
+            // flag words (x - 2) * 8 + z on lanes 0-23, self-mirror bins
+            // removed (the half form flags planes z < 4 only)
+            const bool flag_lane = lane < 24 && (!half || (lane & 7) < 4);
             const uint32_t myflag =
-                lane < 24 ? (L.flg[pw(lane >> 3, lane & 7)] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
+                flag_lane ? (L.flg[pw(lane >> 3, lane & 7)] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
             const uint64_t fwords = __ballot(myflag != 0u);
             __syncthreads();
 
@@ -874,6 +968,11 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                             if (lane == 63 && decide64(a0, a1, b0, b1, th)) {
                                 L.keep[(X - 2) * 8 + z] |= 1u << y;
                                 L.keep[(XR - 2) * 8 + zr] |= 1u << yr;
+                                if (half) {
+                                    // and the image visit (z + 4, y + 16)
+                                    L.keep[(X - 2) * 8 + z + 4] |= 1u << ((y + 16) & 31);
+                                    L.keep[(XR - 2) * 8 + ((zr + 4) & 7)] |= 1u << ((yr + 16) & 31);
+                                }
                             }
                         }
                     }
@@ -898,6 +997,35 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         }
 
         // ---- masked inverse z: outputs z = 0,1,4,5 ------------------------
+        // (rows ky < 16 only in the half form, on two passes; there the lane's
+        // ky is lane % 16 and its turn exp(2 pi i ky / 32) is conj(ptw))
+        if (half) {
+            const c32 htw = {ptw.x, -ptw.y};
+            for (int c = lane; c < 80; c += 64) {
+                const int k = c >> 4, ky = c & 15;
+                c32 v[8];
+#pragma unroll
+                for (int z = 0; z < 8; ++z) {
+                    const c32 t = Sat(L, k, ky, z);
+                    v[z] = ((L.keep[k * 8 + z] >> ky) & 1u) ? t : c32{0.0f, 0.0f};
+                }
+                fft<8, +1>(v);
+                auto turn = [&](c32 b) { return c32{b.x * htw.x - b.y * htw.y, b.x * htw.y + b.y * htw.x}; };
+                if (pbits & 1) {
+                    asm volatile("");
+                    v[0] = turn(v[0]);
+                    v[4] = turn(v[4]);
+                } else {
+                    asm volatile("");
+                    v[1] = turn(v[1]);
+                    v[5] = turn(v[5]);
+                }
+                Sat(L, k, ky, 0) = v[0];
+                Sat(L, k, ky, 1) = v[1];
+                Sat(L, k, ky, 2) = v[4];
+                Sat(L, k, ky, 3) = v[5];
+            }
+        } else
         for (int c = lane; c < 160; c += 64) {
             const int k = c >> 5, ky = c & 31;
             c32 v[8];
@@ -948,8 +1076,14 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             const bool act = lane < 20;
             const int zs = lane / 5, k = lane - 5 * (lane / 5);
             if (act) {
+                if (half) {
+                    // the fold is 2 T[ky], the 2 folded into the x scale
 #pragma unroll
-                for (int j = 0; j < 16; ++j) v[j] = cadd(Sat(L, k, j, zs), Sat(L, k, j + 16, zs));
+                    for (int j = 0; j < 16; ++j) v[j] = Sat(L, k, j, zs);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) v[j] = cadd(Sat(L, k, j, zs), Sat(L, k, j + 16, zs));
+                }
             }
             __syncthreads();  // A and S share storage
             if (act) {
@@ -963,7 +1097,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         // ---- inverse x (c2r on bins 2..6) and overlap-add ----------------
         {
             c32 C[5];
-            const float sc = 2.0f / 4096.0f;
+            const float sc = half ? 4.0f / 4096.0f : 2.0f / 4096.0f;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const c32 t = L.u.A[zsel * 5 + k][om];
@@ -1001,7 +1135,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     }
     }  // work items
     if constexpr (PERSIST) {
-        if (lane == 0 && atomicAdd(&wq[8], 1u) == (unsigned)G - 1u) {
+        if (lane0 == 0 && atomicAdd(&wq[8], 1u) == (unsigned)G - 1u) {
 #pragma unroll
             for (int i = 0; i < 9; ++i) atomicExch(&wq[i], 0u);
         }
@@ -2213,7 +2347,8 @@ int choose_strip3(const Geom &g, int n_jobs)
 }
 
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
-                   hipStream_t s, int mode, void *spec, const float *mask, int strip, int max_groups, unsigned *wq)
+                   hipStream_t s, int mode, void *spec, const float *mask, int strip, int max_groups, unsigned *wq,
+                   bool half)
 {
     float2 *sp = reinterpret_cast<float2 *>(spec);
     if (strip <= 0) strip = choose_strip3(g, n_jobs);
@@ -2222,14 +2357,16 @@ void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *
     // persistent grids are whole multiples of 8 (one share per XCD)
     const bool persist = max_groups > 0 && max_groups < items && wq != nullptr;
     const dim3 grid(persist ? std::max(8, max_groups & ~7) : items), blk(64);
-#define CHD_LAUNCH3D(M, P)                                                                                   \
-    hipLaunchKernelGGL((tpal3d_tiles<M, P>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask, wq)
+#define CHD_LAUNCH3D(M, P, H)                                                                                \
+    hipLaunchKernelGGL((tpal3d_tiles<M, P, H>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask, wq)
     if (mode == kTileSpectra) {
-        if (persist) CHD_LAUNCH3D(kTileSpectra, true); else CHD_LAUNCH3D(kTileSpectra, false);
+        if (persist) CHD_LAUNCH3D(kTileSpectra, true, false); else CHD_LAUNCH3D(kTileSpectra, false, false);
     } else if (mode == kTileApply) {
-        if (persist) CHD_LAUNCH3D(kTileApply, true); else CHD_LAUNCH3D(kTileApply, false);
+        if (persist) CHD_LAUNCH3D(kTileApply, true, false); else CHD_LAUNCH3D(kTileApply, false, false);
+    } else if (half && g.thr_uniform >= 0.0f) {
+        if (persist) CHD_LAUNCH3D(kTilePlain, true, true); else CHD_LAUNCH3D(kTilePlain, false, true);
     } else {
-        if (persist) CHD_LAUNCH3D(kTilePlain, true); else CHD_LAUNCH3D(kTilePlain, false);
+        if (persist) CHD_LAUNCH3D(kTilePlain, true, false); else CHD_LAUNCH3D(kTilePlain, false, false);
     }
 #undef CHD_LAUNCH3D
 }
