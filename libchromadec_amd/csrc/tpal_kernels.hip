@@ -444,6 +444,49 @@ __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
     }
 }
 
+// Per-phase clock stamps of the plain-mode 3D tile kernel, compiled in only
+// with -DCHD_TILE_PHASES (make -C libchromadec_amd phases; bench.py
+// --tile-phases reads them): lane i of a workgroup accumulates the shader
+// clocks its wave spends in phase i (0 forward x, 1 vertical, 2 temporal,
+// 3 mask steps, 4 keep words, 5 FP64 re-decision, 6 inverse z, 7 inverse y,
+// 8 inverse x and overlap-add, 9 tile setup) and the counts 16 (tiles with a
+// re-decision), 17 (flagged words) and 18 (tiles), and adds them to
+// g_tile_phases at exit.  The stamps wait for nothing but the clock read.
+#ifdef CHD_TILE_PHASES
+__device__ unsigned long long g_tile_phases[32];
+#define CHD_PHASE_INIT() uint64_t ph_acc = 0, ph_prev = __builtin_amdgcn_s_memtime()
+#define CHD_PHASE(i)                                                                                          \
+    do {                                                                                                      \
+        if constexpr (MODE == kTilePlain) {                                                                   \
+            const uint64_t t_ = __builtin_amdgcn_s_memtime();                                                 \
+            ph_acc += (lane0 == (i)) ? (t_ - ph_prev) : 0ull;                                                 \
+            ph_prev = t_;                                                                                     \
+        }                                                                                                     \
+    } while (0)
+#define CHD_PHASE_COUNT(i, n)                                                                                 \
+    do {                                                                                                      \
+        if constexpr (MODE == kTilePlain) ph_acc += (lane0 == (i)) ? (unsigned long long)(n) : 0ull;          \
+    } while (0)
+#define CHD_PHASE_FLUSH()                                                                                     \
+    do {                                                                                                      \
+        if constexpr (MODE == kTilePlain)                                                                     \
+            if (lane0 < 32) atomicAdd(&g_tile_phases[lane0], ph_acc);                                         \
+    } while (0)
+#else
+#define CHD_PHASE_INIT() \
+    do {                 \
+    } while (0)
+#define CHD_PHASE(i) \
+    do {             \
+    } while (0)
+#define CHD_PHASE_COUNT(i, n) \
+    do {                      \
+    } while (0)
+#define CHD_PHASE_FLUSH() \
+    do {                  \
+    } while (0)
+#endif
+
 // MODE kTilePlain: the reference's hard mask.  kTileSpectra: stop after the
 // forward transform and write each tile's bins x = 2..6 to `spec`
 // ([tile][k][ky][kz], tile = (job * n_ty3 + ty) * n_tx3 + tx).
@@ -462,6 +505,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 {
     __shared__ typename LdsFor<MODE>::T L;
     const int lane0 = threadIdx.x;
+    CHD_PHASE_INIT();
     if (lane0 < 32) {
         L.cs[lane0] = c_tab.cos32_d[lane0];
         L.sn[lane0] = c_tab.sin32_d[lane0];
@@ -625,6 +669,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         const int tileX = g.avs - kXT / 2 + (kXT / 2) * tx;
 
         const int64_t tile = ((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx;
+        CHD_PHASE(9);
         if constexpr (MODE != kTileApply) {
             // ---- forward x: two rows per lane -> x bins 2..6 ----------------
             float l1 = 0.0f;
@@ -643,6 +688,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             // on it uniformly
             const float erru = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(err)));
             __syncthreads();
+            CHD_PHASE(0);
 
             // ---- vertical: 16-point DFT of the data lines + parity twiddle ---
             // (with alternating line parity the twiddle of the odd-parity
@@ -675,6 +721,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 }
             }
             __syncthreads();
+            CHD_PHASE(1);
 
             // ---- temporal: 8-point DFT per (k, ky) ----------------------------
             if (alt) {
@@ -721,6 +768,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 }
             }
             __syncthreads();
+            CHD_PHASE(2);
 
             if constexpr (MODE == kTileSpectra) {
                 if (tx + 1 < tx_end) load_rows(tx + 1);
@@ -863,6 +911,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 }
             }
             __syncthreads();
+            CHD_PHASE(3);
             // keep word (k, z) = own visits' ballot | ballot of the visit plane
             // whose partners lie on (k, z), moved to the mirror rows
             // (x_v = 8 - x, z_v = (10 - z) % 8); plus the self-mirror bins
@@ -894,6 +943,9 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 flag_lane ? (L.flg[pw(lane >> 3, lane & 7)] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
             const uint64_t fwords = __ballot(myflag != 0u);
             __syncthreads();
+            CHD_PHASE(4);
+            CHD_PHASE_COUNT(16, fwords != 0u);
+            CHD_PHASE_COUNT(17, __popcll(fwords));
 
             // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
             // Each lane forms the FP64 DFT of its two rows at bins x and 8 - x
@@ -982,6 +1034,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 bins(std::integral_constant<int, 4>{});
                 __syncthreads();
             }
+            CHD_PHASE(5);
             // the next tile's rows, in flight through the inverse
             if (tx + 1 < tx_end) load_rows(tx + 1);
         } else {
@@ -1069,6 +1122,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             Sat(L, k, ky, 3) = v[5];
         }
         __syncthreads();
+        CHD_PHASE(6);
 
         // ---- inverse y: fold 32 -> 16 bins (rows ky, ky + 16 of the plane) ----
         {
@@ -1093,6 +1147,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             }
         }
         __syncthreads();
+        CHD_PHASE(7);
 
         // ---- inverse x (c2r on bins 2..6) and overlap-add ----------------
         {
@@ -1132,8 +1187,11 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             for (int n = 0; n < 8; ++n) carry[n] = out[8 + n];
         }
         __syncthreads();
+        CHD_PHASE(8);
+        CHD_PHASE_COUNT(18, 1);
     }
     }  // work items
+    CHD_PHASE_FLUSH();
     if constexpr (PERSIST) {
         if (lane0 == 0 && atomicAdd(&wq[8], 1u) == (unsigned)G - 1u) {
 #pragma unroll
@@ -2470,3 +2528,18 @@ void launch_fill_other_parity(const Geom &g, void *out, int parity, hipStream_t 
 }
 
 }  // namespace tpal
+
+#ifdef CHD_TILE_PHASES
+// Copy the tile-phase sums (32 counters) to `out`; clear them when `clear`.
+extern "C" __attribute__((visibility("default"))) int chd_tile_phases_read(unsigned long long *out, int clear)
+{
+    if (hipDeviceSynchronize() != hipSuccess) return -1;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(tpal::g_tile_phases), sizeof(tpal::g_tile_phases)) != hipSuccess)
+        return -1;
+    if (clear) {
+        static const unsigned long long zero[32] = {};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(tpal::g_tile_phases), zero, sizeof(zero)) != hipSuccess) return -1;
+    }
+    return 0;
+}
+#endif
