@@ -62,8 +62,10 @@ void launch_targets(const float2 *spec, const float2 *truth, int64_t n_tiles, fl
 void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n_tiles, float *mask, hipStream_t s);
 // Fused, persistent inference (features through sigmoid) for decoding: two
 // workgroups per CU (68 KB of LDS each) loop over tiles.
+// paired: every tile alternates in line parity, so each work unit infers
+// the halves z < 4 of two tiles (a tile's mask repeats at (z + 4, y + 16))
 void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
-                  int cus, hipStream_t s);
+                  int cus, hipStream_t s, bool paired = false);
 void launch_backward(const float *theta, const Frags &f, const Acts &a, const float *mask, int64_t n_tiles,
                      float scale, float *partial, float *grad, hipStream_t s);
 void launch_adam(float *theta, const float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,

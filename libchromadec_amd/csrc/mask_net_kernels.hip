@@ -350,33 +350,55 @@ struct LdsInfer {
 // neighbour at tap (dz, dy, dx) is then row ((z+dz)&7)*96 + ((y+dy)&31)*3 +
 // x+dx: a wave-uniform z term, a per-lane y term and a compile-time x, and
 // x+dx outside 0..2 is the zero row, known at compile time.
+//
+// Paired form (HALF): when the tiles alternate in line parity, a tile's
+// features, and so its activations and mask, repeat half a period away in z
+// and y: visit (z + 4, y + 16, x) equals visit (z, y, x) (its bins are the
+// same magnitudes), and the convolutions, circular in z and y, keep that
+// symmetry.  So the grid holds the halves z < 4 of two tiles, the first in
+// z rows 0-3 and the second in rows 4-7, and a neighbour that leaves a
+// half, z - 1 of z = 0 or z + 1 of z = 3, is the half's row z + 3 or z - 3
+// at y + 16.  A wave's four neighbour planes z0 - 1 .. z0 + 2 leave its half
+// at one end only: plane 0 for the even waves, plane 3 for the odd ones.
 constexpr int kInferWaves = 4;                   // 256 threads; two workgroups per CU
 constexpr int kInferBlocks = 24 / kInferWaves;   // (z, x) row blocks per wave
 constexpr int kInferVisits = 3;                  // feature/head visits per thread: x = 0..2
+
+// Per-lane y terms of the neighbour planes: mid for planes 1 and 2, lo and hi
+// for planes 0 and 3 (the same as mid, or shifted by 16 rows where the plane
+// is the other end of a paired half).
+struct YTerms {
+    int lo[3], mid[3], hi[3];
+};
+template <int ZI>
+__device__ __forceinline__ int yterm(const YTerms &y, int d)
+{
+    return ZI == 0 ? y.lo[d] : ZI == 3 ? y.hi[d] : y.mid[d];
+}
 
 // Byte offset of act row (zt, lane y term, x) for tap T of block B (compile
 // time); zt[k] = ((z0 + k - 1) & 7) * 96 * rowbytes for k = 0..3 covers
 // z0-1 .. z0+2 (z0 = 2w, blocks z0 and z0+1).
 template <int B, int T, int ROWB>
-__device__ __forceinline__ int tap_addr(const int (&zt)[4], const int (&yt)[3], int zero)
+__device__ __forceinline__ int tap_addr(const int (&zt)[4], const YTerms &yt, int zero)
 {
     constexpr int dz = T / 9 - 1, dy = (T / 3) % 3 - 1, dx = T % 3 - 1;
     constexpr int x = B % 3 + dx, zi = B / 3 + dz + 1;
     if constexpr (x < 0 || x > 2) return zero;
-    else return zt[zi] + yt[dy + 1] + x * ROWB;
+    else return zt[zi] + yterm<zi>(yt, dy + 1) + x * ROWB;
 }
 
 // Layer-1 k-step J: this lane's A operand, the features of its two taps
 // 4J + 2hi and 4J + 2hi + 1 (taps >= 27 are K padding: the zero row).
 template <int B, int T>
-__device__ __forceinline__ int l1_addr(const int (&zt)[4], const int (&yt)[3], int zero)
+__device__ __forceinline__ int l1_addr(const int (&zt)[4], const YTerms &yt, int zero)
 {
     if constexpr (T >= 27) return zero;
     else return tap_addr<B, T, 8>(zt, yt, zero);
 }
 
 template <int J, int B>
-__device__ __forceinline__ uint4 l1_operand(const char *fb, const int (&zt)[4], const int (&yt)[3], int zero, int hi)
+__device__ __forceinline__ uint4 l1_operand(const char *fb, const int (&zt)[4], const YTerms &yt, int zero, int hi)
 {
     const int qa = hi ? l1_addr<B, 4 * J + 2>(zt, yt, zero) : l1_addr<B, 4 * J>(zt, yt, zero);
     const int qb = hi ? l1_addr<B, 4 * J + 3>(zt, yt, zero) : l1_addr<B, 4 * J + 1>(zt, yt, zero);
@@ -388,7 +410,7 @@ __device__ __forceinline__ uint4 l1_operand(const char *fb, const int (&zt)[4], 
 // for step J+1 into the same registers; step J+1's weight fragment (LDS,
 // w1l = this lane's word of step 0) is read while step J's ops run.
 template <int J>
-__device__ __forceinline__ void layer1_steps(const char *fb, const int (&zt)[4], const int (&yt)[3], int zero, int hi,
+__device__ __forceinline__ void layer1_steps(const char *fb, const int (&zt)[4], const YTerms &yt, int zero, int hi,
                                              const char *w1l, uint4 &bf, uint4 (&af)[kInferBlocks],
                                              f32x16 (&acc)[kInferBlocks])
 {
@@ -423,10 +445,11 @@ constexpr int kL2Groups = 18;
 constexpr int kL2Ring = 9;   // weight fragments in flight: three groups
 
 template <int G, int ZB, int XP>
-__device__ __forceinline__ uint4 l2_operand(const char *ab, const int (&zt)[4], const int (&yt)[3])
+__device__ __forceinline__ uint4 l2_operand(const char *ab, const int (&zt)[4], const YTerms &yt)
 {
     constexpr int dz = G / 6 - 1, dy = (G / 2) % 3 - 1, h = G % 2;
-    return *reinterpret_cast<const uint4 *>(ab + zt[ZB + dz + 1] + yt[dy + 1] + XP * kActRowB + 32 * h);
+    return *reinterpret_cast<const uint4 *>(ab + zt[ZB + dz + 1] + yterm<ZB + dz + 1>(yt, dy + 1) + XP * kActRowB +
+                                            32 * h);
 }
 
 // Weight half-tap of group G and dx = D - 1: tap ((dz+1)*3 + dy+1)*3 + D,
@@ -449,7 +472,7 @@ __device__ __forceinline__ void l2_fetch_weights(const char *w2, int w2off, uint
 // its last op is issued.  The two z blocks alternate so that no accumulator
 // is hit by consecutive ops.
 template <int G>
-__device__ __forceinline__ void layer2_groups(const char *ab, const int (&zt)[4], const int (&yt)[3],
+__device__ __forceinline__ void layer2_groups(const char *ab, const int (&zt)[4], const YTerms &yt,
                                               const char *w2, int w2off, uint4 (&op)[kInferBlocks],
                                               uint4 (&wq)[kL2Ring], f32x16 (&acc)[kInferBlocks])
 {
@@ -542,11 +565,16 @@ __device__ __forceinline__ void store_act(char *act, int ebase, const f32x16 (&a
         }
 }
 
+// HALF: the paired form (every tile alternates in line parity); a work unit
+// is tiles 2u and 2u + 1 (halves z < 4 in z rows 0-3 and 4-7), and each
+// visit's mask is written at the visit and at its image (z + 4, y + 16).
+template <bool HALF>
 __global__ void __launch_bounds__(64 * kInferWaves, 2)
 k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restrict__ theta, Frags f,
         float *__restrict__ mask)
 {
     constexpr int NT = 64 * kInferWaves, NB = kInferBlocks, NV = kInferVisits;
+    const int64_t n_units = HALF ? (n_tiles + 1) / 2 : n_tiles;
     __shared__ LdsInfer L;
     __shared__ float red[kInferWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -556,36 +584,47 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     if (tid < 64) L.bias[tid >> 5][tid & 31] = theta[(tid < 32 ? kOffB1 : kOffB2) + (tid & 31)];
     for (int k = tid; k < kK1 * 64; k += 64 * kInferWaves) L.w1[k] = f.w1[k];
     const float b3 = theta[kOffB3];
-    // address terms (bytes): act rows for layer 2, feature rows for layer 1
-    int zt_act[4], zt_feat[4], yt_act[3], yt_feat[3];
+    // address terms (bytes): act rows for layer 2, feature rows for layer 1;
+    // neighbour plane k is z row z0 + k - 1, within the wave's half when
+    // paired
+    int zt_act[4], zt_feat[4], yt_act[3], yt_feat[3], ys_act[3], ys_feat[3];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int zr = ((z0 + k - 1) & 7) * 96;
+        const int zr = (HALF ? ((z0 & 4) | ((z0 + k - 1) & 3)) : ((z0 + k - 1) & 7)) * 96;
         zt_act[k] = zr * kActRowB;
         zt_feat[k] = zr * 8;
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const int yr = ((i + k - 1) & 31) * 3;
+        const int yr = ((i + k - 1) & 31) * 3, ys = ((i + k + 15) & 31) * 3;   // y + dy, y + dy + 16
         yt_act[k] = yr * kActRowB + 16 * hi;
         yt_feat[k] = yr * 8;
+        ys_act[k] = ys * kActRowB + 16 * hi;
+        ys_feat[k] = ys * 8;
     }
     const int zero_feat = kZeroRow * 8;
-    // this thread's feature/head visits: z = z0 + hi, y = i, x = 0..2
+    // this thread's feature/head visits: z row zv = z0 + hi, y = i, x = 0..2;
+    // paired, z row zv is z = zv % 4 of tile 2u + zv / 4
     const int zv = z0 + hi;
     const int pv = (zv * 32 + i) * 3;
+    const int zt_true = HALF ? (zv & 3) : zv;
+    const int pvt = (zt_true * 32 + i) * 3;   // the visit in its own tile
     int ia[NV], ib[NV];   // byte offsets of the visit's bin and its mirror in a tile spectrum
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-        visit_bins(pv + j, ia[j], ib[j]);
+        visit_bins(pvt + j, ia[j], ib[j]);
         ia[j] *= (int)sizeof(float2);
         ib[j] *= (int)sizeof(float2);
     }
+    // the wave's tile within a work unit (paired: waves 0 and 1 hold the
+    // first tile's half, waves 2 and 3 the second's)
+    const int wtile = HALF ? (wave >> 1) : 0;
     const char *ab = reinterpret_cast<const char *>(L.u.act);
     const char *fb = reinterpret_cast<const char *>(L.feat);
     // spectrum bins of this thread's visits, loaded one tile ahead
     float2 sa[NV], sb[NV];
-    auto fetch = [&](int64_t t) {
+    auto fetch = [&](int64_t u) {
+        const int64_t t = HALF ? 2 * u + wtile : u;
         const char *st = reinterpret_cast<const char *>(spec + t * kSpecC);   // wave-uniform
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
@@ -599,11 +638,32 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     fetch(blockIdx.x);
     __syncthreads();
 
-    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    for (int64_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+        const int64_t t = HALF ? 2 * u + wtile : u;   // this wave's tile
         // Keep the address terms opaque per tile: otherwise every tap's
         // address (hundreds per lane) is hoisted out of the tile loop.
 #pragma unroll
         for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(yt_act[k]), "+v"(yt_feat[k]));
+        YTerms ya, yf;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            ya.mid[k] = ya.lo[k] = ya.hi[k] = yt_act[k];
+            yf.mid[k] = yf.lo[k] = yf.hi[k] = yt_feat[k];
+        }
+        if constexpr (HALF) {
+            // plane 0 (even waves) or plane 3 (odd waves) is the far end of
+            // the half, 16 rows on
+#pragma unroll
+            for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(ys_act[k]), "+v"(ys_feat[k]));
+            const bool odd = wave & 1;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                ya.lo[k] = odd ? yt_act[k] : ys_act[k];
+                ya.hi[k] = odd ? ys_act[k] : yt_act[k];
+                yf.lo[k] = odd ? yt_feat[k] : ys_feat[k];
+                yf.hi[k] = odd ? ys_feat[k] : yt_feat[k];
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) asm volatile("" : "+s"(zt_act[k]), "+s"(zt_feat[k]));
 #pragma unroll
@@ -627,9 +687,16 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         if (lane == 0) red[wave] = s;
         __syncthreads();
         float tot = 0.0f;
+        float mean;
+        if constexpr (HALF) {
+            // the tile's mean over its half (the other half repeats it)
+            tot = red[2 * wtile] + red[2 * wtile + 1];
+            mean = tot * (2.0f / kP);
+        } else {
 #pragma unroll
-        for (int w = 0; w < kInferWaves; ++w) tot += red[w];
-        const float mean = tot * (1.0f / kP);
+            for (int w = 0; w < kInferWaves; ++w) tot += red[w];
+            mean = tot * (1.0f / kP);
+        }
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             // self-mirror bins: x = 2 (j), y in {4, 20}, z in {1, 5}
@@ -648,13 +715,13 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
             const char *w1l = reinterpret_cast<const char *>(L.w1) + 16 * lane;
             uint4 bf = *reinterpret_cast<const uint4 *>(w1l);
             uint4 af[NB];
-            af[0] = l1_operand<0, 0>(fb, zt_feat, yt_feat, zero_feat, hi);
-            af[1] = l1_operand<0, 1>(fb, zt_feat, yt_feat, zero_feat, hi);
-            af[2] = l1_operand<0, 2>(fb, zt_feat, yt_feat, zero_feat, hi);
-            af[3] = l1_operand<0, 3>(fb, zt_feat, yt_feat, zero_feat, hi);
-            af[4] = l1_operand<0, 4>(fb, zt_feat, yt_feat, zero_feat, hi);
-            af[5] = l1_operand<0, 5>(fb, zt_feat, yt_feat, zero_feat, hi);
-            layer1_steps<0>(fb, zt_feat, yt_feat, zero_feat, hi, w1l, bf, af, acc);
+            af[0] = l1_operand<0, 0>(fb, zt_feat, yf, zero_feat, hi);
+            af[1] = l1_operand<0, 1>(fb, zt_feat, yf, zero_feat, hi);
+            af[2] = l1_operand<0, 2>(fb, zt_feat, yf, zero_feat, hi);
+            af[3] = l1_operand<0, 3>(fb, zt_feat, yf, zero_feat, hi);
+            af[4] = l1_operand<0, 4>(fb, zt_feat, yf, zero_feat, hi);
+            af[5] = l1_operand<0, 5>(fb, zt_feat, yf, zero_feat, hi);
+            layer1_steps<0>(fb, zt_feat, yf, zero_feat, hi, w1l, bf, af, acc);
         }
         store_act(reinterpret_cast<char *>(L.u.act), ebase, acc);
         // the first two groups' layer-2 weight fragments, in flight across
@@ -669,13 +736,13 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         acc_from_bias(L.bias[1], hi, acc);
         {
             uint4 op[NB];
-            op[0] = l2_operand<0, 0, 0>(ab, zt_act, yt_act);
-            op[1] = l2_operand<0, 0, 1>(ab, zt_act, yt_act);
-            op[2] = l2_operand<0, 0, 2>(ab, zt_act, yt_act);
-            op[3] = l2_operand<0, 1, 0>(ab, zt_act, yt_act);
-            op[4] = l2_operand<0, 1, 1>(ab, zt_act, yt_act);
-            op[5] = l2_operand<0, 1, 2>(ab, zt_act, yt_act);
-            layer2_groups<0>(ab, zt_act, yt_act, w2, w2off, op, wq, acc);
+            op[0] = l2_operand<0, 0, 0>(ab, zt_act, ya);
+            op[1] = l2_operand<0, 0, 1>(ab, zt_act, ya);
+            op[2] = l2_operand<0, 0, 2>(ab, zt_act, ya);
+            op[3] = l2_operand<0, 1, 0>(ab, zt_act, ya);
+            op[4] = l2_operand<0, 1, 1>(ab, zt_act, ya);
+            op[5] = l2_operand<0, 1, 2>(ab, zt_act, ya);
+            layer2_groups<0>(ab, zt_act, ya, w2, w2off, op, wq, acc);
         }
         // head weights before the next tile's bins: loads complete in order,
         // so the head then waits only for its own (w2off is opaque per tile,
@@ -683,7 +750,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         const char *w3 = reinterpret_cast<const char *>(f.w3t);
         const uint4 w3a = *reinterpret_cast<const uint4 *>(w3 + w2off);
         const uint4 w3b = *reinterpret_cast<const uint4 *>(w3 + w2off + 1024);
-        fetch(t + gridDim.x);   // next tile's bins, in flight through the head
+        fetch(u + gridDim.x);   // next tile's bins, in flight through the head
         __syncthreads();  // every wave has read act1
         store_act(reinterpret_cast<char *>(L.u.act), ebase, acc);
         __syncthreads();
@@ -713,9 +780,14 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         const char *gb = reinterpret_cast<const char *>(L.u.g);
         int rb[3][3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a) {
+            // paired: a neighbour beyond the half is its far end, 16 rows on
+            const int zl = (zv & 3) + a - 1;
+            const bool wrap = HALF && (zl < 0 || zl > 3);
+            const int zn = HALF ? ((zv & 4) | (zl & 3)) : ((zv + a - 1) & 7);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) rb[a][c] = ((zv + a - 1) & 7) * 384 + ((i + c - 1) & 31) * 4;
+            for (int c = 0; c < 3; ++c) rb[a][c] = zn * 384 + ((i + c - 1 + (wrap ? 16 : 0)) & 31) * 4;
+        }
         // two rounds through LDS: taps 0-15 (j = 0, 1), then 16-26 (j = 2, 3)
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
@@ -741,8 +813,16 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
                 logit[2] = head_sum<2, 16, 27, 16>(gb, rb, logit[2]);
             }
         }
+        if (t < n_tiles) {
+            // paired: the visit (z, y) and its image (z + 4, y + 16)
+            const int pimg = (((zt_true + 4) & 7) * 32 + ((i + 16) & 31)) * 3;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) mask[t * kP + pv + j] = 1.0f / (1.0f + expf(-logit[j]));
+            for (int j = 0; j < NV; ++j) {
+                const float m = 1.0f / (1.0f + expf(-logit[j]));
+                mask[t * kP + pvt + j] = m;
+                if constexpr (HALF) mask[t * kP + pimg + j] = m;
+            }
+        }
         __syncthreads();  // G is read before the next tile's features and act1 overwrite it
     }
 }
@@ -982,11 +1062,17 @@ void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n
 }
 
 void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
-                  int cus, hipStream_t s)
+                  int cus, hipStream_t s, bool paired)
 {
     if (n_tiles <= 0) return;
-    const int64_t g = std::min<int64_t>(n_tiles, 2 * std::max(1, cus));
-    hipLaunchKernelGGL(k_infer, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, spec, n_tiles, theta, f, mask);
+    const int64_t units = paired ? (n_tiles + 1) / 2 : n_tiles;
+    const int64_t g = std::min<int64_t>(units, 2 * std::max(1, cus));
+    if (paired)
+        hipLaunchKernelGGL(k_infer<true>, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, spec, n_tiles, theta, f,
+                           mask);
+    else
+        hipLaunchKernelGGL(k_infer<false>, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, spec, n_tiles, theta, f,
+                           mask);
 }
 
 void launch_backward(const float *theta, const Frags &f, const Acts &a, const float *mask, int64_t n_tiles,

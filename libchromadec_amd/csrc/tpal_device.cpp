@@ -367,10 +367,11 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int 
         const int64_t tiles = (int64_t)n_units * g.n_ty3 * g.n_tx3;
         tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra,
                             plan->d_spec, nullptr);
+        const bool paired = tpal::slabs_alternate(plan->h_slab_jobs, n_units);
         mnet::launch_infer(reinterpret_cast<const float2 *>(plan->d_spec), tiles, net->theta, net->f, plan->d_mask,
-                           net->cus, s);
+                           net->cus, s, paired);
         tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply,
-                            plan->d_spec, plan->d_mask);
+                            plan->d_spec, plan->d_mask, 0, 0, nullptr, paired);
     } else if (plan->kind == CHD_DEC_TRANSFORM_3D)
         tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTilePlain, nullptr,
                             nullptr, 0, 0, nullptr, tpal::slabs_alternate(plan->h_slab_jobs, n_units));
@@ -884,9 +885,9 @@ void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
         tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra, b->d_spec,
                             nullptr, strip, groups, b->d_wq);
         mnet::launch_infer(reinterpret_cast<const float2 *>(b->d_spec), cnt * tps, net->theta, net->f, b->d_mask,
-                           net->cus, s);
+                           net->cus, s, b->half);
         tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply, b->d_spec,
-                            b->d_mask, strip, groups, b->d_wq);
+                            b->d_mask, strip, groups, b->d_wq, b->half);
     }
 }
 

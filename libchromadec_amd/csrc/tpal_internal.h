@@ -118,9 +118,10 @@ constexpr int kTileBins = 5 * 32 * 8;
 // wq: 9 zeroed device counters (work items taken per XCD, workgroups done),
 // which every launch leaves zeroed again; launches sharing them must not
 // overlap.  Without wq the launch is one workgroup per item.
-// half (plain mode): every slab's fields alternate in line parity
-// (slabs_alternate) and g.thr_uniform >= 0, so the kernel that derives the
-// rows ky >= 16 of each tile from the rows ky < 16 applies.
+// half: every slab's fields alternate in line parity (slabs_alternate), so
+// the kernel that derives the rows ky >= 16 of each tile from the rows
+// ky < 16 applies: in plain mode when g.thr_uniform >= 0 as well, in apply
+// mode when the masks come from the paired inference.
 constexpr int kWorkCounters = 9;
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
                    hipStream_t s, int mode = kTilePlain, void *spec = nullptr, const float *mask = nullptr,

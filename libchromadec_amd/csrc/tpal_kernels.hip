@@ -590,7 +590,9 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     // capture): row ky+16 of plane z is (-1)^par[z] = sgn (-1)^z times row ky,
     // so after the temporal DFT T[ky+16][kz] = sgn T[ky][(kz+4) & 7] and only
     // rows 0..15 need the vertical stores and the temporal DFT.
-    constexpr bool kHalf = HALF && MODE == kTilePlain;   // the launcher checked the parity
+    // (the launcher checked the parity; in apply mode the masks come from the
+    // paired inference, which makes them repeat at (z + 4, y + 16) too)
+    constexpr bool kHalf = HALF && (MODE == kTilePlain || MODE == kTileApply);
     const bool alt = kHalf || pbits == 0xAA || pbits == 0x55;
     const float sgn = par[0] ? -1.0f : 1.0f;
     // With alternating parity and one threshold for every visit, the rows
@@ -632,15 +634,18 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 
     // apply mode: the next tile's spectrum and mask are loaded into registers
     // while the current tile is inverted (its loads would otherwise wait on
-    // HBM once per tile)
-    constexpr int kSpecPerLane = (MODE == kTileApply) ? 1280 / 64 : 1;
+    // HBM once per tile).  Spectrum word j of a lane is bin lane + 64 j, row
+    // ky = (lane + 64 j) / 8 % 32: the half form needs the rows ky < 16 only,
+    // the words with j % 4 < 2.
+    constexpr int kSpecPerLane = (MODE == kTileApply) ? (kHalf ? 640 / 64 : 1280 / 64) : 1;
     constexpr int kMaskPerLane = (MODE == kTileApply) ? 768 / 64 : 1;
+    auto spec_word = [](int j) { return kHalf ? (j / 2) * 4 + (j % 2) : j; };
     float2 pre_s[kSpecPerLane];
     float pre_m[kMaskPerLane];
     auto prefetch = [&](int64_t t) {
         if constexpr (MODE == kTileApply) {
 #pragma unroll
-            for (int j = 0; j < kSpecPerLane; ++j) pre_s[j] = spec[t * 1280 + lane + 64 * j];
+            for (int j = 0; j < kSpecPerLane; ++j) pre_s[j] = spec[t * 1280 + lane + 64 * spec_word(j)];
 #pragma unroll
             for (int j = 0; j < kMaskPerLane; ++j) pre_m[j] = mask[t * 768 + lane + 64 * j];
         }
@@ -1040,7 +1045,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         } else {
 #pragma unroll
             for (int j = 0; j < kSpecPerLane; ++j) {
-                const int i = lane + 64 * j;
+                const int i = lane + 64 * spec_word(j);
                 Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{pre_s[j].x, pre_s[j].y};
             }
 #pragma unroll
@@ -1060,7 +1065,12 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 #pragma unroll
                 for (int z = 0; z < 8; ++z) {
                     const c32 t = Sat(L, k, ky, z);
-                    v[z] = ((L.keep[k * 8 + z] >> ky) & 1u) ? t : c32{0.0f, 0.0f};
+                    if constexpr (MODE == kTileApply) {
+                        const float mv = L.m[mask_visit(k, ky, z)];
+                        v[z] = c32{t.x * mv, t.y * mv};
+                    } else {
+                        v[z] = ((L.keep[k * 8 + z] >> ky) & 1u) ? t : c32{0.0f, 0.0f};
+                    }
                 }
                 fft<8, +1>(v);
                 auto turn = [&](c32 b) { return c32{b.x * htw.x - b.y * htw.y, b.x * htw.y + b.y * htw.x}; };
@@ -2419,6 +2429,8 @@ void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *
     hipLaunchKernelGGL((tpal3d_tiles<M, P, H>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask, wq)
     if (mode == kTileSpectra) {
         if (persist) CHD_LAUNCH3D(kTileSpectra, true, false); else CHD_LAUNCH3D(kTileSpectra, false, false);
+    } else if (mode == kTileApply && half) {
+        if (persist) CHD_LAUNCH3D(kTileApply, true, true); else CHD_LAUNCH3D(kTileApply, false, true);
     } else if (mode == kTileApply) {
         if (persist) CHD_LAUNCH3D(kTileApply, true, false); else CHD_LAUNCH3D(kTileApply, false, false);
     } else if (half && g.thr_uniform >= 0.0f) {
