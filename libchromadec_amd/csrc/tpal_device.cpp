@@ -344,18 +344,39 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int 
 {
     const Geom &g = plan->g;
     hipError_t e = hipSuccess;
-    if (plan->kind == CHD_DEC_TRANSFORM_3D) {
-        e = hipMemcpyAsync(ws.slab_jobs, plan->h_slab_jobs, n_units * sizeof(tpal::SlabJob),
-                           hipMemcpyHostToDevice, s);
-    } else if (plan->kind == CHD_DEC_TRANSFORM_2D) {
-        e = hipMemcpyAsync(ws.field_jobs, plan->h_field_jobs, n_units * sizeof(tpal::FieldJob2D),
-                           hipMemcpyHostToDevice, s);
+    // A repeated decode of the same frames into the same output (a benchmark
+    // loop, a re-render) builds the same job lists: when the plan's own job
+    // buffers already hold them, the two uploads (and the wait for them
+    // before the tile launch) are skipped.  Calls are ordered on the plan
+    // (call_begin), so the buffers are not rewritten under a running call.
+    const bool is3d = plan->kind == CHD_DEC_TRANSFORM_3D, is2d = plan->kind == CHD_DEC_TRANSFORM_2D;
+    const size_t unit_bytes = is3d ? n_units * sizeof(tpal::SlabJob) : is2d ? n_units * sizeof(tpal::FieldJob2D) : 0;
+    const unsigned char *unit_src = is3d ? reinterpret_cast<const unsigned char *>(plan->h_slab_jobs)
+                                         : reinterpret_cast<const unsigned char *>(plan->h_field_jobs);
+    const unsigned char *frame_src = reinterpret_cast<const unsigned char *>(plan->h_frame_jobs);
+    const size_t frame_job_bytes = n_frames * sizeof(tpal::FrameJob);
+    const bool own = ws.frame_jobs == plan->d_frame_jobs;
+    const bool same = own && plan->up_valid && plan->up_shadow.size() == unit_bytes + frame_job_bytes &&
+                      std::memcmp(plan->up_shadow.data(), unit_src, unit_bytes) == 0 &&
+                      std::memcmp(plan->up_shadow.data() + unit_bytes, frame_src, frame_job_bytes) == 0;
+    if (!same) {
+        if (is3d) {
+            e = hipMemcpyAsync(ws.slab_jobs, plan->h_slab_jobs, unit_bytes, hipMemcpyHostToDevice, s);
+        } else if (is2d) {
+            e = hipMemcpyAsync(ws.field_jobs, plan->h_field_jobs, unit_bytes, hipMemcpyHostToDevice, s);
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(ws.frame_jobs, frame_src, frame_job_bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
+        if (e != hipSuccess) {
+            plan->up_valid = false;
+            return chd::fail(CHD_E_INTERNAL, hip_msg((who + ": job upload").c_str(), e));
+        }
+        if (own) {
+            plan->up_shadow.assign(unit_src, unit_src + unit_bytes);
+            plan->up_shadow.insert(plan->up_shadow.end(), frame_src, frame_src + frame_job_bytes);
+            plan->up_valid = true;
+        }
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(ws.frame_jobs, plan->h_frame_jobs, n_frames * sizeof(tpal::FrameJob),
-                           hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
-    if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg((who + ": job upload").c_str(), e));
     const bool border = (g.out_w != g.aw || g.out_h != g.last - g.first + 1);
     if (border) {
         for (int i = 0; i < n_frames; ++i) tpal::launch_fill_border(g, plan->h_frame_jobs[i].out, 1, s);
@@ -764,6 +785,7 @@ chd_status_t chd_tpal_slab_spectra(chd_tpal_plan_t *plan, const uint16_t *d_fiel
             }
             sj.partial = nullptr;
         }
+        plan->up_valid = false;   // the decode's job buffers are rewritten
         hipError_t e = hipMemcpyAsync(plan->d_slab_jobs, plan->h_slab_jobs, cnt * sizeof(tpal::SlabJob),
                                       hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);
@@ -809,6 +831,7 @@ chd_status_t chd_tpal_decode_separated(chd_tpal_plan_t *plan, const uint16_t *d_
             fj.nchroma = 1;
             if (border) tpal::launch_fill_border(g, fj.out, 1, s);
         }
+        plan->up_valid = false;   // the decode's job buffers are rewritten
         hipError_t e = hipMemcpyAsync(plan->d_frame_jobs, plan->h_frame_jobs, cnt * sizeof(tpal::FrameJob),
                                       hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(plan->staging_done, s);

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <vector>
+
 #include "tpal_internal.h"
 
 struct chd_tpal_plan {
@@ -30,6 +32,11 @@ struct chd_tpal_plan {
     tpal::FrameJob *h_frame_jobs = nullptr;
     hipEvent_t staging_done = nullptr;
     bool staging_pending = false;
+    // The job lists of the last upload into d_slab_jobs / d_field_jobs and
+    // d_frame_jobs (unit jobs, then frame jobs): a call that decodes the same
+    // frames into the same buffers again skips the upload.
+    std::vector<unsigned char> up_shadow;
+    bool up_valid = false;
     // The partials, job lists and uncored planes are shared by every call on
     // the plan: a call on another stream first waits for the previous call's
     // last kernel (recorded in call_done on last_stream).

@@ -1216,9 +1216,12 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 // kStrip2 (+1 leading) tiles of one tile row of one field, one line per lane.
 // ---------------------------------------------------------------------------
 
+// Rows are padded to 17: in the vertical stages lane (q, k) walks row
+// (q, k), and with 16-entry rows every active lane would hit the same banks.
+constexpr int kA2pad = 17;
 struct Lds2 {
-    c32 A[4][9][16];          // per group: (k, line) after x DFT / before x inverse
-    uint8_t keep[4][9][16];
+    c32 A[4][9][kA2pad];      // per group: (k, line) after x DFT / before x inverse
+    uint8_t keep[4][9][kA2pad];
     uint16_t flag[4][80];
     int nflag[4];
 };
@@ -1389,7 +1392,7 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
 #pragma unroll
             for (int m = 0; m < 16; ++m) L.A[q][ln][m] = v[m];
         }
-        for (int i = ln; i < 9 * 16; i += 16) (&L.keep[q][0][0])[i] = 0;
+        for (int i = ln; i < 9 * kA2pad; i += 16) (&L.keep[q][0][0])[i] = 0;
         __syncthreads();
 
         // ---- mask: 80 visits per group (transform_pal_2d.cpp applyFilter) --
