@@ -30,7 +30,11 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "tpal_internal.h"
 
@@ -1970,10 +1974,12 @@ struct LdsQ4 {
 __device__ __forceinline__ float4 f4add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 __device__ __forceinline__ float f4get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
 
+// lpb: lines per workgroup (at most LINES), chosen by the launcher so that
+// the workgroups fill whole rounds of the resident slots.
 template <int NC, bool RAW, int LINES>
 __global__ void __launch_bounds__(kDemodThreads)
 pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
-                const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, float *raw)
+                const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, int lpb, float *raw)
 {
     __shared__ LdsQ4<LINES> L;
     const int tid = threadIdx.x;
@@ -1988,9 +1994,9 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     const uint16_t *comp = job.field[fsel];
     const int firstLine = (g.first + 1 - off) / 2;
     const int lastLine = ((g.last - off) / 2) + 1;
-    const int n0 = firstLine + lb * LINES;
+    const int n0 = firstLine + lb * lpb;
     if (n0 >= lastLine) return;
-    const int nl = min(LINES, lastLine - n0);
+    const int nl = min(lpb, lastLine - n0);
     const int fs = g.filter_size;
     const int c0 = 4 * tid - kQHalo;         // this thread's first column, from active_video_start
     constexpr bool pal2d = NC == 0;
@@ -2453,6 +2459,34 @@ void launch_tpal2d(const Geom &g, const FieldJob2D *jobs, int n_jobs, int n_ty_m
         hipLaunchKernelGGL(tpal2d_tiles, dim3(blocks), dim3(64), 0, s, g, jobs, n_ty_max, thr32, thr64);
 }
 
+// Compute units of the current device, and how many workgroups of a kernel
+// one of them holds (both cached per device / kernel).
+static int cu_count()
+{
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cached[dev].load(std::memory_order_relaxed);
+    if (n == 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+static int resident_per_cu(const void *kern)
+{
+    static std::mutex mu;
+    static std::vector<std::pair<const void *, int>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto &e : cache)
+        if (e.first == kern) return e.second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kDemodThreads, 0) != hipSuccess || n <= 0) n = 1;
+    cache.emplace_back(kern, n);
+    return n;
+}
+
 static void demod_kernels(const Geom &g, const FrameJob *jobs, int n_jobs, const float *sine, const float *cosine,
                           const float *cfilt, int n_lblocks, hipStream_t s, int nchroma, float *raw)
 {
@@ -2462,20 +2496,39 @@ static void demod_kernels(const Geom &g, const FrameJob *jobs, int n_jobs, const
         // A few frames (one chd_decode_frame window) give too few 32-line
         // workgroups to fill the chip, and each runs for the whole launch:
         // 8-line workgroups cut that latency at the cost of more halo rows.
-        const bool short_blocks = (int64_t)n_jobs * 2 * ((n_lblocks * kDemodLines + kQ4Lines - 1) / kQ4Lines) < 256;
-        const int lines = short_blocks ? kQ4ShortLines : kQ4Lines;
-        const int nlb = (n_lblocks * kDemodLines + lines - 1) / lines;
-        const int blocks = n_jobs * 2 * nlb;
-        if (blocks > 0) {
+        const int field_lines = n_lblocks * kDemodLines;   // at least every field's active lines
+        const bool short_blocks = (int64_t)n_jobs * 2 * ((field_lines + kQ4Lines - 1) / kQ4Lines) < 256;
 #define CHD_Q4(L)                                                                                              \
     (raw != nullptr ? (nchroma == 4 ? pal_demod_quad4<4, true, L> : nchroma == 2 ? pal_demod_quad4<2, true, L>     \
                        : nchroma == 1 ? pal_demod_quad4<1, true, L> : pal_demod_quad4<0, true, L>)               \
                     : (nchroma == 4 ? pal_demod_quad4<4, false, L> : nchroma == 2 ? pal_demod_quad4<2, false, L>   \
                        : nchroma == 1 ? pal_demod_quad4<1, false, L> : pal_demod_quad4<0, false, L>))
-            auto kern = short_blocks ? CHD_Q4(kQ4ShortLines) : CHD_Q4(kQ4Lines);
+        auto kern = short_blocks ? CHD_Q4(kQ4ShortLines) : CHD_Q4(kQ4Lines);
 #undef CHD_Q4
-            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt, nlb, raw);
+        // Every workgroup runs about as long as its lines plus the 6-line
+        // halo, and the grid drains in rounds of the resident slots: pick
+        // the lines per workgroup (16 .. 32) that makes the fewest rounds of
+        // the shortest workgroups.  (For 64 PAL frames, 24 lines give 1536
+        // workgroups, two full rounds of 3 per CU, where 32 give 1152, one
+        // and a half.)
+        int lpb = short_blocks ? kQ4ShortLines : kQ4Lines;
+        if (!short_blocks) {
+            const int64_t slots = (int64_t)std::max(1, resident_per_cu((const void *)kern)) * cu_count();
+            int64_t best = -1;
+            for (int cand = 16; cand <= kQ4Lines; cand += 4) {
+                const int64_t wgs = (int64_t)n_jobs * 2 * ((field_lines + cand - 1) / cand);
+                const int64_t cost = ((wgs + slots - 1) / slots) * (cand + 6);
+                if (best < 0 || cost < best) {
+                    best = cost;
+                    lpb = cand;
+                }
+            }
         }
+        const int nlb = (field_lines + lpb - 1) / lpb;
+        const int blocks = n_jobs * 2 * nlb;
+        if (blocks > 0)
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDemodThreads), 0, s, g, jobs, sine, cosine, cfilt, nlb, lpb,
+                               raw);
         return;
     }
     if (g.quad && !composite) {  // active lines wider than one column-blocked workgroup
