@@ -63,7 +63,8 @@ void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n
 // Fused, persistent inference (features through sigmoid) for decoding: two
 // workgroups per CU (68 KB of LDS each) loop over tiles.
 // paired: every tile alternates in line parity, so each work unit infers
-// the halves z < 4 of two tiles (a tile's mask repeats at (z + 4, y + 16))
+// the halves z < 4 of two tiles (a tile's mask repeats at (z + 4, y + 16));
+// the spectra are then the compact ones (rows ky < 16, 640 bins per tile)
 void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
                   int cus, hipStream_t s, bool paired = false);
 void launch_backward(const float *theta, const Frags &f, const Acts &a, const float *mask, int64_t n_tiles,

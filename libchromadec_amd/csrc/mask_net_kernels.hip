@@ -568,6 +568,9 @@ __device__ __forceinline__ void store_act(char *act, int ebase, const f32x16 (&a
 // HALF: the paired form (every tile alternates in line parity); a work unit
 // is tiles 2u and 2u + 1 (halves z < 4 in z rows 0-3 and 4-7), and each
 // visit's mask is written at the visit and at its image (z + 4, y + 16).
+// Its spectra are compact, rows ky < 16 only ([tile][k][ky < 16][kz]): a bin
+// of row ky >= 16 is read at its image (ky - 16, kz + 4), which has the same
+// magnitude, all the features use.
 template <bool HALF>
 __global__ void __launch_bounds__(64 * kInferWaves, 2)
 k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restrict__ theta, Frags f,
@@ -610,9 +613,17 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     const int zt_true = HALF ? (zv & 3) : zv;
     const int pvt = (zt_true * 32 + i) * 3;   // the visit in its own tile
     int ia[NV], ib[NV];   // byte offsets of the visit's bin and its mirror in a tile spectrum
+    auto compact = [](int b) {
+        const int k = b >> 8, ky = (b >> 3) & 31, kz = b & 7;
+        return ky < 16 ? (k * 16 + ky) * 8 + kz : (k * 16 + ky - 16) * 8 + ((kz + 4) & 7);
+    };
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         visit_bins(pvt + j, ia[j], ib[j]);
+        if constexpr (HALF) {
+            ia[j] = compact(ia[j]);
+            ib[j] = compact(ib[j]);
+        }
         ia[j] *= (int)sizeof(float2);
         ib[j] *= (int)sizeof(float2);
     }
@@ -625,7 +636,7 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
     float2 sa[NV], sb[NV];
     auto fetch = [&](int64_t u) {
         const int64_t t = HALF ? 2 * u + wtile : u;
-        const char *st = reinterpret_cast<const char *>(spec + t * kSpecC);   // wave-uniform
+        const char *st = reinterpret_cast<const char *>(spec + t * (HALF ? kSpecC / 2 : kSpecC));   // wave-uniform
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             sa[j] = sb[j] = make_float2(0.0f, 0.0f);

@@ -120,8 +120,10 @@ constexpr int kTileBins = 5 * 32 * 8;
 // overlap.  Without wq the launch is one workgroup per item.
 // half: every slab's fields alternate in line parity (slabs_alternate), so
 // the kernel that derives the rows ky >= 16 of each tile from the rows
-// ky < 16 applies: in plain mode when g.thr_uniform >= 0 as well, in apply
-// mode when the masks come from the paired inference.
+// ky < 16 applies: in plain mode when g.thr_uniform >= 0 as well; in
+// spectra mode it writes compact spectra (rows ky < 16, 640 bins per tile)
+// for the paired inference, and in apply mode it reads them with the paired
+// inference's masks.
 constexpr int kWorkCounters = 9;
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
                    hipStream_t s, int mode = kTilePlain, void *spec = nullptr, const float *mask = nullptr,

@@ -594,9 +594,10 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     // capture): row ky+16 of plane z is (-1)^par[z] = sgn (-1)^z times row ky,
     // so after the temporal DFT T[ky+16][kz] = sgn T[ky][(kz+4) & 7] and only
     // rows 0..15 need the vertical stores and the temporal DFT.
-    // (the launcher checked the parity; in apply mode the masks come from the
-    // paired inference, which makes them repeat at (z + 4, y + 16) too)
-    constexpr bool kHalf = HALF && (MODE == kTilePlain || MODE == kTileApply);
+    // (the launcher checked the parity.  Spectra mode then writes the compact
+    // spectra of the paired learned path, rows ky < 16; apply mode reads them
+    // with masks from the paired inference, which repeat at (z + 4, y + 16).)
+    constexpr bool kHalf = HALF;
     const bool alt = kHalf || pbits == 0xAA || pbits == 0x55;
     const float sgn = par[0] ? -1.0f : 1.0f;
     // With alternating parity and one threshold for every visit, the rows
@@ -638,18 +639,17 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 
     // apply mode: the next tile's spectrum and mask are loaded into registers
     // while the current tile is inverted (its loads would otherwise wait on
-    // HBM once per tile).  Spectrum word j of a lane is bin lane + 64 j, row
-    // ky = (lane + 64 j) / 8 % 32: the half form needs the rows ky < 16 only,
-    // the words with j % 4 < 2.
-    constexpr int kSpecPerLane = (MODE == kTileApply) ? (kHalf ? 640 / 64 : 1280 / 64) : 1;
+    // HBM once per tile).  The half form reads the compact spectra of the
+    // paired path, rows ky < 16 only ([tile][k][ky < 16][kz], 640 bins).
+    constexpr int kSpecBins = kHalf ? 640 : 1280;
+    constexpr int kSpecPerLane = (MODE == kTileApply) ? kSpecBins / 64 : 1;
     constexpr int kMaskPerLane = (MODE == kTileApply) ? 768 / 64 : 1;
-    auto spec_word = [](int j) { return kHalf ? (j / 2) * 4 + (j % 2) : j; };
     float2 pre_s[kSpecPerLane];
     float pre_m[kMaskPerLane];
     auto prefetch = [&](int64_t t) {
         if constexpr (MODE == kTileApply) {
 #pragma unroll
-            for (int j = 0; j < kSpecPerLane; ++j) pre_s[j] = spec[t * 1280 + lane + 64 * spec_word(j)];
+            for (int j = 0; j < kSpecPerLane; ++j) pre_s[j] = spec[t * kSpecBins + lane + 64 * j];
 #pragma unroll
             for (int j = 0; j < kMaskPerLane; ++j) pre_m[j] = mask[t * 768 + lane + 64 * j];
         }
@@ -781,9 +781,17 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 
             if constexpr (MODE == kTileSpectra) {
                 if (tx + 1 < tx_end) load_rows(tx + 1);
-                for (int i = lane; i < 1280; i += 64) {
-                    const c32 v = Sat(L, i >> 8, (i >> 3) & 31, i & 7);
-                    spec[tile * 1280 + i] = make_float2(v.x, v.y);
+                if constexpr (half) {
+                    // compact: rows ky < 16 only, [tile][k][ky < 16][kz]
+                    for (int i = lane; i < 640; i += 64) {
+                        const c32 v = Sat(L, i >> 7, (i >> 3) & 15, i & 7);
+                        spec[tile * 640 + i] = make_float2(v.x, v.y);
+                    }
+                } else {
+                    for (int i = lane; i < 1280; i += 64) {
+                        const c32 v = Sat(L, i >> 8, (i >> 3) & 31, i & 7);
+                        spec[tile * 1280 + i] = make_float2(v.x, v.y);
+                    }
                 }
                 __syncthreads();
                 continue;
@@ -1049,8 +1057,9 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         } else {
 #pragma unroll
             for (int j = 0; j < kSpecPerLane; ++j) {
-                const int i = lane + 64 * spec_word(j);
-                Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{pre_s[j].x, pre_s[j].y};
+                const int i = lane + 64 * j;
+                if constexpr (kHalf) Sat(L, i >> 7, (i >> 3) & 15, i & 7) = c32{pre_s[j].x, pre_s[j].y};
+                else Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{pre_s[j].x, pre_s[j].y};
             }
 #pragma unroll
             for (int j = 0; j < kMaskPerLane; ++j) L.m[lane + 64 * j] = pre_m[j];
@@ -2436,7 +2445,9 @@ void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *
     const dim3 grid(persist ? std::max(8, max_groups & ~7) : items), blk(64);
 #define CHD_LAUNCH3D(M, P, H)                                                                                \
     hipLaunchKernelGGL((tpal3d_tiles<M, P, H>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask, wq)
-    if (mode == kTileSpectra) {
+    if (mode == kTileSpectra && half) {
+        if (persist) CHD_LAUNCH3D(kTileSpectra, true, true); else CHD_LAUNCH3D(kTileSpectra, false, true);
+    } else if (mode == kTileSpectra) {
         if (persist) CHD_LAUNCH3D(kTileSpectra, true, false); else CHD_LAUNCH3D(kTileSpectra, false, false);
     } else if (mode == kTileApply && half) {
         if (persist) CHD_LAUNCH3D(kTileApply, true, true); else CHD_LAUNCH3D(kTileApply, false, true);
