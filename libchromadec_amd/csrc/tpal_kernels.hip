@@ -864,47 +864,63 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 // The half form: planes z < 4 only, rows ky >= 16 read from
                 // their stored images, rows ky - 16 at kz + 4 (bins of equal
                 // magnitude).  Steps 0-3 decide (x, z) = (2, s) on lanes 0-31
-                // and (3, s) on lanes 32-63; step 4 (4, 0) and (4, 1); step 5
-                // (4, 3).  Plane (4, 2) holds the pairs of (4, 0), a and b
-                // swapped, as in the full form.
+                // and (3, s) on lanes 32-63.  Step 4 decides the x = 4 pairs,
+                // each once: plane (4, 0) on lanes 0-31 (plane (4, 2) holds
+                // its pairs, a and b swapped, as in the full form); plane
+                // (4, 1), whose visit y and 8 - y share a pair, on lanes 32-46
+                // for y = 5..19 (its self-mirror bins y = 4, 20 are kept
+                // anyway); plane (4, 3), whose visit y and 24 - y share a pair
+                // through the image rows, on lanes 47-63 for y = 12..28.  The
+                // other visits of those planes take the decision of their
+                // pair's (a mirrored ballot); only the deciding visit is
+                // flagged, and its FP64 re-decision sets both bins.
                 auto steps_half = [&]() {
                     const int yr = (40 - vy) & 31;
                     const int a0 = (vy & 15) * kSpad + 4 * (vy >> 4);
                     // mirror z' = (10 - z) % 8 is 2 - z for z < 3 and 7 for
                     // z = 3; a mirror row yr >= 16 adds 4 (mod 8) to it
                     const int b0 = (yr & 15) * kSpad + 4 * (yr >> 4), b0m = b0 - 8 * (yr >> 4);
-                    auto off_a = [&](int st) {
-                        return st < 4 ? a0 + vhalf * kSk + st : st == 4 ? a0 + 2 * kSk + vhalf : a0 + 2 * kSk + 3;
-                    };
+                    // step 4: this lane's visit (z4, y4) of x = 4 and its mirror
+                    const int y4 = lane < 32 ? lane : lane < 47 ? lane - 27 : lane - 35;
+                    const int z4 = lane < 32 ? 0 : lane < 47 ? 1 : 3;
+                    const int yr4 = (40 - y4) & 31, zr4 = (10 - z4) & 7;
+                    const int a4 = 2 * kSk + (y4 & 15) * kSpad + z4 + 4 * (y4 >> 4);
+                    const int b4 = 2 * kSk + (yr4 & 15) * kSpad + ((zr4 + 4 * (yr4 >> 4)) & 7);
+                    auto off_a = [&](int st) { return st < 4 ? a0 + vhalf * kSk + st : a4; };
                     auto off_b = [&](int st) {
                         return st < 3    ? b0 - vhalf * kSk + 4 * kSk + (2 - st)
                                : st == 3 ? b0m - vhalf * kSk + 4 * kSk + 7
-                               : st == 4 ? b0 - vhalf + 2 * kSk + 2
-                                         : b0m + 2 * kSk + 7;
+                                         : b4;
                     };
                     const DecideConst q = decide_const(erru, g.thr_uniform);
                     c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
 #pragma unroll
-                    for (int st = 0; st < 6; ++st) {
+                    for (int st = 0; st < 5; ++st) {
                         c32 an = a, bn = b;
-                        if (st + 1 < 6) {
+                        if (st + 1 < 5) {
                             an = L.u.S[off_a(st + 1)];
                             bn = L.u.S[off_b(st + 1)];
                         }
                         uint64_t kb, fb;
                         decide32_ballots(a, b, q, kb, fb);
                         if (lane == 0) {
-                            const int wlo = st < 4 ? pw(0, st) : st == 4 ? pw(2, 0) : pw(2, 3);
-                            L.bal[wlo] = (uint32_t)kb;
-                            L.flg[wlo] = (uint32_t)fb;
-                            if (st < 5) {
-                                const int whi = st < 4 ? pw(1, st) : pw(2, 1);
-                                L.bal[whi] = (uint32_t)(kb >> 32);
-                                L.flg[whi] = (uint32_t)(fb >> 32);
-                            }
-                            if (st == 4) {
+                            if (st < 4) {
+                                L.bal[pw(0, st)] = (uint32_t)kb;
+                                L.flg[pw(0, st)] = (uint32_t)fb;
+                                L.bal[pw(1, st)] = (uint32_t)(kb >> 32);
+                                L.flg[pw(1, st)] = (uint32_t)(fb >> 32);
+                            } else {
+                                auto rot16 = [](uint32_t w) { return (w << 16) | (w >> 16); };
+                                const uint32_t w1 = (uint32_t)((kb >> 32) & 0x7FFFu) << 5;
+                                const uint32_t w3 = (uint32_t)((kb >> 47) & 0x1FFFFu) << 12;
+                                L.bal[pw(2, 0)] = (uint32_t)kb;
+                                L.flg[pw(2, 0)] = (uint32_t)fb;
                                 L.bal[pw(2, 2)] = mirror_rows((uint32_t)kb);
                                 L.flg[pw(2, 2)] = 0u;
+                                L.bal[pw(2, 1)] = w1 | mirror_rows(w1);
+                                L.flg[pw(2, 1)] = (uint32_t)((fb >> 32) & 0x7FFFu) << 5;
+                                L.bal[pw(2, 3)] = w3 | rot16(mirror_rows(w3));
+                                L.flg[pw(2, 3)] = (uint32_t)((fb >> 47) & 0x1FFFFu) << 12;
                             }
                         }
                         a = an;
