@@ -509,6 +509,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 {
     __shared__ typename LdsFor<MODE>::T L;
     const int lane0 = threadIdx.x;
+    // the c2r scale of the inverse (2 / 4096; the half form's fold doubles it)
+    constexpr float kInvScale = (HALF ? 4.0f : 2.0f) / 4096.0f;
     CHD_PHASE_INIT();
     if (lane0 < 32) {
         L.cs[lane0] = c_tab.cos32_d[lane0];
@@ -585,7 +587,9 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         const uint16_t *fp = job.field[rz[h]];
         const bool ok = fp != nullptr && ry[h] >= startY && ry[h] < endY;
         rowp[h] = ok ? fp + (int64_t)((tileY + ry[h]) >> 1) * g.pitch + g.lmargin : nullptr;
-        roww[h] = c_tab.wz[rz[h]] * c_tab.wy[ry[h]];
+        // plain mode folds the inverse's scale (a power of two, so the
+        // products round the same) into the forward row weight
+        roww[h] = c_tab.wz[rz[h]] * c_tab.wy[ry[h]] * (MODE == kTilePlain ? kInvScale : 1.0f);
     }
     // vertical-stage roles: forward lanes (z, k) = lane / 5, inverse lanes
     // (zs, k) = lane / 5 with z = 0,1,4,5
@@ -1191,11 +1195,12 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         // ---- inverse x (c2r on bins 2..6) and overlap-add ----------------
         {
             c32 C[5];
-            const float sc = half ? 4.0f / 4096.0f : 2.0f / 4096.0f;
+            // (plain mode: the scale is in the forward row weights)
+            constexpr float sc = MODE == kTilePlain ? 1.0f : kInvScale;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const c32 t = L.u.A[zsel * 5 + k][om];
-                C[k] = c32{t.x * sc, t.y * sc};
+                C[k] = sc == 1.0f ? t : c32{t.x * sc, t.y * sc};
             }
             float out[16];
             xidft16(C, out);
@@ -1205,10 +1210,17 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 const int c0 = tileX - g.avs + kColOrigin;
                 float *dst = orow + c0;
                 float v8[8];
+                if (tileX >= g.avs && tileX + 8 <= g.ave) {
+                    // an interior tile: every column is active (a scalar
+                    // branch; the edge tiles take the per-column test)
 #pragma unroll
-                for (int n = 0; n < 8; ++n) {
-                    const int col = tileX + n;
-                    v8[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
+                    for (int n = 0; n < 8; ++n) v8[n] = carry[n] + out[n];
+                } else {
+#pragma unroll
+                    for (int n = 0; n < 8; ++n) {
+                        const int col = tileX + n;
+                        v8[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
+                    }
                 }
                 gstore_f4(dst, v8[0], v8[1], v8[2], v8[3]);
                 gstore_f4(dst + 4, v8[4], v8[5], v8[6], v8[7]);
