@@ -194,6 +194,28 @@ def test_learned_decode_matches_oracle_with_same_masks(orc, patch_capture):
         assert d.max() <= F32_TOL, (f, d.max())
 
 
+def test_learned_decode_independent_of_chunking(patch_capture):
+    """With alternating field parity the network runs paired: two tiles' halves
+    per work unit.  A tile's mask depends on that tile alone, so decodes in
+    chunks of 3 frames (5 slabs: an odd tile count, the last unit half
+    empty) and in one chunk of 6 frames (8 slabs) are bit-identical."""
+    torch = _torch()
+    _, fields = patch_capture
+    net = abi.MaskNet(max_tiles=4096)
+    net.set_params(_params(17))
+    nf = fields.shape[0] // 2
+    outs = []
+    for chunk in (3, 6):
+        plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, max_batch_frames=chunk))
+        assert (chunk + 2) * plan.tiles_per_slab % 2 == (1 if chunk == 3 else 0)
+        plan.set_mask_net(net)
+        packed, _ = _device_spectra(plan, fields, 0, 1)
+        outs.append(_decode_device(plan, packed, nf, 0, nf))
+        plan.close()
+    net.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_all_ones_mask_equals_threshold_zero_bitwise(patch_capture):
     _, fields = patch_capture
     nf = fields.shape[0] // 2

@@ -486,6 +486,40 @@ def test_device_api_matches_abi_bitwise(patch_capture, kind):
         assert np.array_equal(host[f], dec.decode(f)), f
 
 
+@pytest.mark.parametrize("kind", [7, 8])
+def test_repeated_decodes_reuse_job_lists(patch_capture, kind):
+    """A call that decodes the same frames into the same buffer as the plan's
+    previous call skips the job-list upload; any other call uploads.  Decode
+    A, A, B, A into one output buffer and compare each with a fresh plan."""
+    import torch
+    _, fields = patch_capture
+    dev = torch.device("cuda:0")
+    params = abi.pal_tpal_params(kind=kind, pixel_format=abi.CHD_PIXEL_YUV444PS, max_batch_frames=4)
+    plan = abi.TpalPlan(params)
+    src = torch.from_numpy(fields.view(np.int16)).to(dev)
+    packed = torch.empty((fields.shape[0], plan.packed_field_elems), dtype=torch.int16, device=dev)
+    plan.pack(src.data_ptr(), fields.shape[0], packed.data_ptr())
+    out = torch.zeros((2, plan.frame_bytes // 4), dtype=torch.float32, device=dev)
+    runs = {"A": [(packed.data_ptr(), 6, 1, 2)], "B": [(packed.data_ptr(), 6, 3, 2)]}
+
+    def fresh(run):
+        p = abi.TpalPlan(params)
+        o = torch.zeros_like(out)
+        p.decode(runs[run], o.data_ptr())
+        torch.cuda.synchronize()
+        p.close()
+        return o.cpu().numpy()
+
+    want = {r: fresh(r) for r in runs}
+    assert not np.array_equal(want["A"], want["B"])
+    for r in "AABA":
+        out.zero_()
+        plan.decode(runs[r], out.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), want[r]), r
+    plan.close()
+
+
 def test_truncated_capture_decodes_black_neighbours(orc, patch_capture, tmp_path):
     """Fields the sidecar lists but the file lacks read as black frames
     (source_field.cpp:104-132)."""
