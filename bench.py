@@ -9,8 +9,8 @@ already resident in HBM, output planes written to HBM.
 
 Prints one JSON line.  `roofline` is for one decode, the step's two kernels
 (the 3D tile transform and the demodulator, each timed with HIP events on
-the decode stream), against the algorithmic bytes of the step (composite in,
-planes out); `traffic` is their measured HBM bytes
+the decode stream over the K timed steps), against the algorithmic bytes of
+the step (composite in, planes out); `traffic` is their measured HBM bytes
 (profiles/tpal3d_tiles_c2_traffic.json).  `cpu_baseline` times the FP64
 oracle (a port of the reference algorithm) on the host.
 
@@ -22,7 +22,11 @@ Beside the headline (plain threshold) the line carries:
   * `batched`: 16 sequences of seeded lengths 8-40 frames (about 384 frames;
     bars, moving patches and luma zone plates up to 5 MHz) decoded as one
     fixed batch, one persistent tile launch replayed from a HIP graph, with
-    the strip length and grid picked by the library's autotuner.
+    the strip length and grid picked by the library's autotuner;
+  * `transform_2d`: the same 64 frames decoded with CHD_DEC_TRANSFORM_2D.
+These three legs run before the headline's warm-up and timed steps, so that
+the headline is timed at the GPU's sustained clocks; `abi_end_to_end` and the
+CPU baseline run after it.
 """
 from __future__ import annotations
 
@@ -249,28 +253,34 @@ def main() -> int:
         del a, b
     torch.cuda.synchronize()
 
+    # The other GPU legs run first.  From idle the GPU's clocks take about 25
+    # decodes (30 ms of load) to reach their sustained level, so timed right
+    # after a few warm-up steps the headline would measure that transient.
+    extras = set() if args.no_extras else {e for e in args.extras.split(",") if e}
+    # (the learned leg's PSNR evaluation runs on the host, so it goes first)
+    legs = {}
+    if "learned" in extras:
+        legs["learned"] = learned_leg(plan, packed, fields, truth, tu, tv, out, seqs, stream, max(3, args.steps // 4))
+    if "batched" in extras:
+        legs["batched"] = batched_leg(args.steps)
+    if "2d" in extras:
+        legs["transform_2d"] = transform2d_leg(fields, args.steps)
+
     for _ in range(args.warmup):
         plan.decode(seqs, out.data_ptr(), stream)
     torch.cuda.synchronize()
+    # kernel durations over the timed steps: HIP events around each launch on
+    # the decode stream, recorded without waiting and summed afterwards
+    plan.set_timing(2)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.decode(seqs, out.data_ptr(), stream)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    tile_ms, demod_ms, n_tile, _ = plan.last_timing()
+    plan.set_timing(0)
     ms_per_step = 1e3 * elapsed / args.steps
     fps = N_FRAMES * args.steps / elapsed
-
-    # kernel durations: HIP events around each launch on the decode stream
-    plan.set_timing(True)
-    tile_ms, demod_ms, n_tile = 0.0, 0.0, 0
-    for _ in range(max(3, min(args.steps, 10))):
-        plan.decode(seqs, out.data_ptr(), stream)
-        a, b, nt, _ = plan.last_timing()
-        tile_ms += a
-        demod_ms += b
-        n_tile += nt
-    plan.set_timing(False)
-    torch.cuda.synchronize()
     tile_avg = tile_ms / n_tile
     demod_avg = demod_ms / n_tile
     frames_per_launch = N_FRAMES
@@ -321,14 +331,12 @@ def main() -> int:
                               "peak_tflops": FP32_PEAK_TFLOPS,
                               "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)}},
     }
-    extras = set() if args.no_extras else {e for e in args.extras.split(",") if e}
-    if "learned" in extras:
-        res["learned"] = learned_leg(plan, packed, fields, truth, tu, tv, out, seqs, stream, max(3, args.steps // 4))
+    if "learned" in legs:
+        res["learned"] = legs["learned"]
         res["learned"]["per_frame_time_vs_plain"] = round(fps / res["learned"]["value"], 2)
-    if "batched" in extras:
-        res["batched"] = batched_leg(args.steps)
-    if "2d" in extras:
-        res["transform_2d"] = transform2d_leg(fields, args.steps)
+    for k in ("batched", "transform_2d"):
+        if k in legs:
+            res[k] = legs[k]
     if "abi" in extras:
         res["abi_end_to_end"] = abi_leg(fields)
     if args.cpu_sample > 0:
@@ -446,24 +454,17 @@ def transform2d_leg(fields, steps):
     out = torch.empty((N_FRAMES, plan.frame_bytes // 4), dtype=torch.float32, device=dev)
     seqs = [(packed.data_ptr(), N_FRAMES, 0, N_FRAMES)]
     stream = torch.cuda.current_stream().cuda_stream
-    for _ in range(2):
+    for _ in range(10):
         plan.decode(seqs, out.data_ptr(), stream)
     torch.cuda.synchronize()
+    plan.set_timing(2)
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.decode(seqs, out.data_ptr(), stream)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    plan.set_timing(True)
-    tile_ms, demod_ms, n = 0.0, 0.0, 0
-    for _ in range(max(3, min(steps, 10))):
-        plan.decode(seqs, out.data_ptr(), stream)
-        a, b, nt, _ = plan.last_timing()
-        tile_ms += a
-        demod_ms += b
-        n += nt
-    plan.set_timing(False)
-    torch.cuda.synchronize()
+    tile_ms, demod_ms, n, _ = plan.last_timing()
+    plan.set_timing(0)
     tile_avg, demod_avg = tile_ms / n, demod_ms / n
     algo = (ALGO_IN + ALGO_OUT_PS) * N_FRAMES
     tflops = NOMINAL_GFLOP_PER_FRAME_2D * N_FRAMES / ((tile_avg + demod_avg) * 1e-3) / 1e3
@@ -486,7 +487,8 @@ def learned_leg(plain_plan, packed, fields, truth, tu, tv, out, seqs, stream, st
     plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, pixel_format=abi.CHD_PIXEL_YUV444PS,
                                             max_batch_frames=N_FRAMES))
     plan.set_mask_net(net)
-    plan.decode(seqs, out.data_ptr(), stream)
+    for _ in range(5):
+        plan.decode(seqs, out.data_ptr(), stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):

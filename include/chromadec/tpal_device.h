@@ -140,9 +140,11 @@ chd_status_t chd_tpal_batch_config(const chd_tpal_batch_t *batch, int32_t *strip
 chd_status_t chd_tpal_batch_set_shape(chd_tpal_batch_t *batch, int32_t strip, int32_t groups);
 void         chd_tpal_batch_free(chd_tpal_batch_t *batch);
 
-/* Kernel-level timing of the last chd_tpal_decode on this plan when timing
- * was enabled: milliseconds in the tile-transform and demodulation kernels
- * (HIP events on the decode stream).  enable != 0 turns recording on. */
+/* Kernel-level timing: milliseconds in the tile-transform and demodulation
+ * kernels, from HIP events recorded on the decode stream without waiting.
+ * enable: 0 off; 1 the totals cover the last decode call on this plan;
+ * 2 the totals run over every decode call since chd_tpal_set_timing.
+ * chd_tpal_last_timing waits for the recorded kernels, then reports. */
 chd_status_t chd_tpal_set_timing(chd_tpal_plan_t *plan, int enable);
 chd_status_t chd_tpal_last_timing(chd_tpal_plan_t *plan, double *tile_ms, double *demod_ms,
                                   int32_t *n_tile_launches, int32_t *n_demod_launches);

@@ -314,7 +314,6 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     ok = ok && hipHostMalloc((void **)&plan->h_frame_jobs, plan->max_frames * sizeof(tpal::FrameJob), 0) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&plan->staging_done, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&plan->call_done, hipEventDisableTiming) == hipSuccess;
-    for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&plan->ev[i]) == hipSuccess;
     if (!ok) {
         chd_tpal_plan_free(plan);
         return chd::fail(CHD_E_OOM, W + ": device allocation failed");
@@ -330,6 +329,55 @@ chd_status_t tpal_plan_create_impl(const chd_tpal_params_t *p, chd_tpal_plan **o
     }
     *out = plan;
     return CHD_OK;
+}
+
+// Kernel timing.  The events of the recorded chunks are summed into the
+// totals when they are read, so that recording never waits for the GPU.
+static void timing_reset(chd_tpal_plan *plan)
+{
+    plan->t_pending = 0;
+    plan->last_tile_ms = plan->last_demod_ms = 0;
+    plan->n_tile_launches = plan->n_demod_launches = 0;
+}
+
+// A decode call starts new totals unless they run over every call (mode 2).
+static void timing_begin(chd_tpal_plan *plan)
+{
+    if (plan->timing != 2) timing_reset(plan);
+}
+
+// Adds the recorded chunks' tile and demod times to the totals (waits for
+// the last recorded chunk).
+static void timing_resolve(chd_tpal_plan *plan)
+{
+    if (plan->t_pending == 0) return;
+    hipEventSynchronize(plan->t_events[3 * plan->t_pending - 1]);
+    for (int i = 0; i < plan->t_pending; ++i) {
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, plan->t_events[3 * i], plan->t_events[3 * i + 1]);
+        hipEventElapsedTime(&b, plan->t_events[3 * i + 1], plan->t_events[3 * i + 2]);
+        plan->last_tile_ms += a;
+        plan->last_demod_ms += b;
+        plan->n_tile_launches += 1;
+        plan->n_demod_launches += 1;
+    }
+    plan->t_pending = 0;
+}
+
+// The three events of the next recorded chunk, or nullptr when an event
+// cannot be created (the chunk then goes untimed).  At most 1024 chunks are
+// held before they are summed.
+static hipEvent_t *timing_slot(chd_tpal_plan *plan)
+{
+    constexpr int kMaxTimedChunks = 1024;
+    if (plan->t_pending == kMaxTimedChunks) timing_resolve(plan);
+    const size_t need = 3 * (size_t)(plan->t_pending + 1);
+    while (plan->t_events.size() < need) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        plan->t_events.push_back(e);
+    }
+    return &plan->t_events[3 * (size_t)plan->t_pending++];
 }
 
 // Enqueue one chunk: copy job lists, launch the tile and demod kernels.
@@ -381,7 +429,8 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int 
     if (border) {
         for (int i = 0; i < n_frames; ++i) tpal::launch_fill_border(g, plan->h_frame_jobs[i].out, 1, s);
     }
-    if (plan->timing) hipEventRecord(plan->ev[0], s);
+    hipEvent_t *tev = plan->timing ? timing_slot(plan) : nullptr;
+    if (tev) hipEventRecord(tev[0], s);
     if (plan->kind == CHD_DEC_TRANSFORM_3D && plan->net != nullptr) {
         // learned mask: tile spectra -> network -> masked inverse
         chd_mask_net *net = plan->net;
@@ -399,7 +448,7 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int 
                             nullptr, 0, 0, nullptr, tpal::slabs_alternate(plan->h_slab_jobs, n_units));
     else if (plan->kind == CHD_DEC_TRANSFORM_2D)
         tpal::launch_tpal2d(g, ws.field_jobs, n_units, plan->n_ty2_max, plan->d_thr32, plan->d_thr64, s);
-    if (plan->timing) hipEventRecord(plan->ev[1], s);
+    if (tev) hipEventRecord(tev[1], s);
     e = tpal::launch_demod(g, ws.frame_jobs, n_frames, plan->d_sine, plan->d_cosine, plan->d_cfilt,
                            plan->n_lblocks, s, plan_nchroma(plan), ws.raw);
     if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg((who + ": demod launch").c_str(), e));
@@ -407,20 +456,9 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int 
         const tpal::FrameJob &fj = plan->h_frame_jobs[i];
         if (fj.offset[0] == fj.offset[1]) tpal::launch_fill_other_parity(g, fj.out, fj.offset[0] ^ 1, s);
     }
-    if (plan->timing) hipEventRecord(plan->ev[2], s);
+    if (tev) hipEventRecord(tev[2], s);
     e = hipGetLastError();
     if (e != hipSuccess) return chd::fail(CHD_E_INTERNAL, hip_msg((who + ": kernel launch").c_str(), e));
-    if (plan->timing) {
-        // accumulate kernel time of this chunk (waits for it; timing mode only)
-        hipEventSynchronize(plan->ev[2]);
-        float t0 = 0, t1 = 0;
-        hipEventElapsedTime(&t0, plan->ev[0], plan->ev[1]);
-        hipEventElapsedTime(&t1, plan->ev[1], plan->ev[2]);
-        plan->last_tile_ms += t0;
-        plan->last_demod_ms += t1;
-        plan->n_tile_launches += 1;
-        plan->n_demod_launches += 1;
-    }
     return CHD_OK;
 }
 
@@ -476,8 +514,7 @@ chd_status_t tpal_decode_windows(chd_tpal_plan *plan, const TpalWindow *w, int n
     // Each window is decoded on its own lattice (two slabs / two fields).
     const Geom &g = plan->g;
     hipSetDevice(plan->device);
-    plan->last_tile_ms = plan->last_demod_ms = 0;
-    plan->n_tile_launches = plan->n_demod_launches = 0;
+    timing_begin(plan);
     const TpalWorkspace ws = plan_ws(plan);
     call_begin(plan, s);
     const int per_chunk = std::max(1, std::min(plan->max_frames, plan->max_units / 2));
@@ -570,8 +607,7 @@ void chd_tpal_plan_free(chd_tpal_plan_t *plan)
     if (plan->h_frame_jobs) hipHostFree(plan->h_frame_jobs);
     if (plan->staging_done) hipEventDestroy(plan->staging_done);
     if (plan->call_done) hipEventDestroy(plan->call_done);
-    for (int i = 0; i < 4; ++i)
-        if (plan->ev[i]) hipEventDestroy(plan->ev[i]);
+    for (hipEvent_t e : plan->t_events) hipEventDestroy(e);
     delete plan;
 }
 
@@ -619,8 +655,7 @@ chd_status_t tpal_decode_runs(chd_tpal_plan *plan, const TpalWorkspace &ws, cons
     const Geom &g = plan->g;
     const int64_t fb = frame_bytes(plan);
     const bool is3d = plan->kind == CHD_DEC_TRANSFORM_3D;
-    plan->last_tile_ms = plan->last_demod_ms = 0;
-    plan->n_tile_launches = plan->n_demod_launches = 0;
+    timing_begin(plan);
 
     int units = 0, frames = 0, runs = 0;
     int64_t out_index = 0;
@@ -1167,7 +1202,8 @@ void chd_tpal_batch_free(chd_tpal_batch_t *batch)
 chd_status_t chd_tpal_set_timing(chd_tpal_plan_t *plan, int enable)
 {
     if (plan == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_set_timing: null argument");
-    plan->timing = enable != 0;
+    plan->timing = enable == 2 ? 2 : (enable != 0 ? 1 : 0);
+    timing_reset(plan);
     return CHD_OK;
 }
 
@@ -1175,6 +1211,8 @@ chd_status_t chd_tpal_last_timing(chd_tpal_plan_t *plan, double *tile_ms, double
                                   int32_t *n_demod)
 {
     if (plan == nullptr) return chd::fail(CHD_E_INVALID_ARG, "chd_tpal_last_timing: null argument");
+    hipSetDevice(plan->device);
+    timing_resolve(plan);
     if (tile_ms) *tile_ms = plan->last_tile_ms;
     if (demod_ms) *demod_ms = plan->last_demod_ms;
     if (n_tile) *n_tile = plan->n_tile_launches;

@@ -43,8 +43,13 @@ struct chd_tpal_plan {
     hipEvent_t call_done = nullptr;
     hipStream_t last_stream = nullptr;
     bool call_pending = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool timing = false;
+    // Kernel timing: 0 off; 1 totals of the last decode call; 2 running
+    // totals since chd_tpal_set_timing.  Each chunk records three events
+    // (before the tile launch, between tile and demod launches, after the
+    // demod) without waiting; they are summed when the totals are read.
+    int timing = 0;
+    std::vector<hipEvent_t> t_events;  // 3 per recorded chunk, created on first use
+    int t_pending = 0;                 // chunks recorded and not yet summed
     double last_tile_ms = 0, last_demod_ms = 0;
     int n_tile_launches = 0, n_demod_launches = 0;
     // learned mask (3D): the network, and tile spectra / masks of one chunk
