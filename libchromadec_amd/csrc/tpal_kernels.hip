@@ -272,6 +272,13 @@ __device__ __forceinline__ void wave_sum4_d(double &a, double &b, double &c, dou
     }
 }
 
+// The value of the neighbouring lane of a pair (lane ^ 1): one DPP move,
+// quad_perm 1,0,3,2.
+__device__ __forceinline__ float swap_pair(float x)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+
 // One DPP move of a double (its two halves); rows or banks outside the masks
 // read 0.
 template <int CTRL, int ROWMASK = 0xF, int BANKMASK = 0xF>
@@ -1168,25 +1175,40 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         CHD_PHASE(6);
 
         // ---- inverse y: fold 32 -> 16 bins (rows ky, ky + 16 of the plane) ----
+        // Two lanes per 16-point transform (plane zs, x bin k): lane 2p + h
+        // transforms the inputs of parity h (8 points), the odd half is turned
+        // by exp(2 pi i m / 16), and the pair swaps halves (a DPP move) to form
+        // outputs 8h .. 8h + 7: the last radix-2 stage of fft<16>, the same
+        // operations in the same order.
         {
-            c32 v[16];
-            const bool act = lane < 20;
-            const int zs = lane / 5, k = lane - 5 * (lane / 5);
+            c32 v[8];
+            const int pr = lane >> 1, h = lane & 1;
+            const bool act = lane < 40;
+            const int zs = pr / 5, k = pr - 5 * (pr / 5);
             if (act) {
                 if (half) {
                     // the fold is 2 T[ky], the 2 folded into the x scale
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) v[j] = Sat(L, k, j, zs);
+                    for (int j = 0; j < 8; ++j) v[j] = Sat(L, k, 2 * j + h, zs);
                 } else {
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) v[j] = cadd(Sat(L, k, j, zs), Sat(L, k, j + 16, zs));
+                    for (int j = 0; j < 8; ++j) v[j] = cadd(Sat(L, k, 2 * j + h, zs), Sat(L, k, 2 * j + h + 16, zs));
                 }
             }
             __syncthreads();  // A and S share storage
             if (act) {
-                fft<16, +1>(v);
+                fft<8, +1>(v);
+                if (h) {
 #pragma unroll
-                for (int m = 0; m < 16; ++m) L.u.A[lane][m] = v[m];
+                    for (int m = 1; m < 8; ++m) v[m] = twiddle<+1>(v[m], 2 * m);
+                }
+                // lane h = 0: E + W O (outputs 0..7); h = 1: E - W O (8..15)
+                const float sg = h ? -1.0f : 1.0f;
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const float ox = swap_pair(v[m].x), oy = swap_pair(v[m].y);
+                    L.u.A[pr][8 * h + m] = c32{fmaf(sg, v[m].x, ox), fmaf(sg, v[m].y, oy)};
+                }
             }
         }
         __syncthreads();
