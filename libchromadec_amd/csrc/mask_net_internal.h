@@ -62,11 +62,14 @@ void launch_targets(const float2 *spec, const float2 *truth, int64_t n_tiles, fl
 void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n_tiles, float *mask, hipStream_t s);
 // Fused, persistent inference (features through sigmoid) for decoding: two
 // workgroups per CU (68 KB of LDS each) loop over tiles.
-// paired: every tile alternates in line parity, so each work unit infers
-// the halves z < 4 of two tiles (a tile's mask repeats at (z + 4, y + 16));
-// the spectra are then the compact ones (rows ky < 16, 640 bins per tile)
-void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
-                  int cus, hipStream_t s, bool paired = false);
+// in: complex tile spectra ([tile][1280] float2).  paired: every tile
+// alternates in line parity, so each work unit infers the halves z < 4 of
+// two tiles (a tile's mask repeats at (z + 4, y + 16)); `in` is then the
+// magnitude features of rows ky < 16 ([tile][640] floats, the tile kernel's
+// kTileFeatures mode), and the masks are those of the visits of planes z < 4
+// ([tile][384]: visit (z + 4, y + 16) has the mask of visit (z, y))
+void launch_infer(const void *in, int64_t n_tiles, const float *theta, const Frags &f, float *mask, int cus,
+                  hipStream_t s, bool paired = false);
 void launch_backward(const float *theta, const Frags &f, const Acts &a, const float *mask, int64_t n_tiles,
                      float scale, float *partial, float *grad, hipStream_t s);
 void launch_adam(float *theta, const float *grad, float *m, float *v, float lr, float beta1, float beta2, float eps,

@@ -22,7 +22,9 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
+#include "mask_feature.h"
 #include "mask_net_internal.h"
 
 namespace mnet {
@@ -41,14 +43,8 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi)
     return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
 }
 
-// Magnitude feature of a bin: (ln(|a| + 1) - 10) / 3, from the hardware
-// square root and base-2 logarithm (the argument is >= 1: no denormals).
-// Its error, a few 1e-7, is far below the bf16 step the feature is stored in.
-__device__ __forceinline__ float mag_feature(float2 a)
-{
-    const float m = __builtin_amdgcn_sqrtf(a.x * a.x + a.y * a.y);
-    return fmaf(__builtin_amdgcn_logf(m + 1.0f), 0.23104906f /* ln 2 / 3 */, -10.0f / 3.0f);
-}
+// Magnitude feature of a bin: (ln(|a| + 1) - 10) / 3 (mask_feature.h).
+__device__ __forceinline__ float mag_feature(float2 a) { return chd::mag_feature(a.x, a.y); }
 
 __device__ __forceinline__ f32x16 mfma(uint4 a, uint4 b, f32x16 c)
 {
@@ -566,14 +562,16 @@ __device__ __forceinline__ void store_act(char *act, int ebase, const f32x16 (&a
 }
 
 // HALF: the paired form (every tile alternates in line parity); a work unit
-// is tiles 2u and 2u + 1 (halves z < 4 in z rows 0-3 and 4-7), and each
-// visit's mask is written at the visit and at its image (z + 4, y + 16).
-// Its spectra are compact, rows ky < 16 only ([tile][k][ky < 16][kz]): a bin
-// of row ky >= 16 is read at its image (ky - 16, kz + 4), which has the same
-// magnitude, all the features use.
+// is tiles 2u and 2u + 1 (halves z < 4 in z rows 0-3 and 4-7), and the masks
+// of the visits of planes z < 4 are written, [tile][384] (visit (z + 4,
+// y + 16) has the mask of visit (z, y)).
+// Its input is the magnitude feature of each bin of rows ky < 16
+// ([tile][k][ky < 16][kz] floats, written by the tile kernel): a bin of row
+// ky >= 16 is read at its image (ky - 16, kz + 4), which has the same
+// magnitude.  Otherwise the input is complex tile spectra ([tile][1280]).
 template <bool HALF>
 __global__ void __launch_bounds__(64 * kInferWaves, 2)
-k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restrict__ theta, Frags f,
+k_infer(const void *__restrict__ in, int64_t n_tiles, const float *__restrict__ theta, Frags f,
         float *__restrict__ mask)
 {
     constexpr int NT = 64 * kInferWaves, NB = kInferBlocks, NV = kInferVisits;
@@ -624,25 +622,28 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
             ia[j] = compact(ia[j]);
             ib[j] = compact(ib[j]);
         }
-        ia[j] *= (int)sizeof(float2);
-        ib[j] *= (int)sizeof(float2);
+        ia[j] *= HALF ? (int)sizeof(float) : (int)sizeof(float2);
+        ib[j] *= HALF ? (int)sizeof(float) : (int)sizeof(float2);
     }
     // the wave's tile within a work unit (paired: waves 0 and 1 hold the
     // first tile's half, waves 2 and 3 the second's)
     const int wtile = HALF ? (wave >> 1) : 0;
     const char *ab = reinterpret_cast<const char *>(L.u.act);
     const char *fb = reinterpret_cast<const char *>(L.feat);
-    // spectrum bins of this thread's visits, loaded one tile ahead
-    float2 sa[NV], sb[NV];
+    // the bins (features, paired) of this thread's visits, loaded one tile
+    // ahead
+    using In = typename std::conditional<HALF, float, float2>::type;
+    constexpr int64_t kInBytes = HALF ? (int64_t)(kSpecC / 2) * sizeof(float) : (int64_t)kSpecC * sizeof(float2);
+    In sa[NV], sb[NV];
     auto fetch = [&](int64_t u) {
         const int64_t t = HALF ? 2 * u + wtile : u;
-        const char *st = reinterpret_cast<const char *>(spec + t * (HALF ? kSpecC / 2 : kSpecC));   // wave-uniform
+        const char *st = reinterpret_cast<const char *>(in) + t * kInBytes;   // wave-uniform
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            sa[j] = sb[j] = make_float2(0.0f, 0.0f);
+            sa[j] = sb[j] = In{};
             if (t < n_tiles) {
-                sa[j] = *reinterpret_cast<const float2 *>(st + ia[j]);
-                sb[j] = *reinterpret_cast<const float2 *>(st + ib[j]);
+                sa[j] = *reinterpret_cast<const In *>(st + ia[j]);
+                sb[j] = *reinterpret_cast<const In *>(st + ib[j]);
             }
         }
     };
@@ -690,8 +691,13 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
         float f0[NV], f1[NV], s = 0.0f;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            f0[j] = mag_feature(sa[j]);
-            f1[j] = mag_feature(sb[j]);
+            if constexpr (HALF) {
+                f0[j] = sa[j];
+                f1[j] = sb[j];
+            } else {
+                f0[j] = mag_feature(sa[j]);
+                f1[j] = mag_feature(sb[j]);
+            }
             s += f0[j];
         }
         s = wave_sum(s);
@@ -825,13 +831,12 @@ k_infer(const float2 *__restrict__ spec, int64_t n_tiles, const float *__restric
             }
         }
         if (t < n_tiles) {
-            // paired: the visit (z, y) and its image (z + 4, y + 16)
-            const int pimg = (((zt_true + 4) & 7) * 32 + ((i + 16) & 31)) * 3;
+            // paired: the visits of planes z < 4 only ([tile][384]); visit
+            // (z + 4, y + 16) has the mask of visit (z, y)
 #pragma unroll
             for (int j = 0; j < NV; ++j) {
                 const float m = 1.0f / (1.0f + expf(-logit[j]));
-                mask[t * kP + pvt + j] = m;
-                if constexpr (HALF) mask[t * kP + pimg + j] = m;
+                mask[t * (HALF ? kP / 2 : kP) + pvt + j] = m;
             }
         }
         __syncthreads();  // G is read before the next tile's features and act1 overwrite it
@@ -1072,17 +1077,17 @@ void launch_forward(const float *theta, const Frags &f, const Acts &a, int64_t n
     hipLaunchKernelGGL(k_head, grid, blk, 0, s, a.act2, f.w3t, theta + kOffB3, a.logit, mask);
 }
 
-void launch_infer(const float2 *spec, int64_t n_tiles, const float *theta, const Frags &f, float *mask,
-                  int cus, hipStream_t s, bool paired)
+void launch_infer(const void *in, int64_t n_tiles, const float *theta, const Frags &f, float *mask, int cus,
+                  hipStream_t s, bool paired)
 {
     if (n_tiles <= 0) return;
     const int64_t units = paired ? (n_tiles + 1) / 2 : n_tiles;
     const int64_t g = std::min<int64_t>(units, 2 * std::max(1, cus));
     if (paired)
-        hipLaunchKernelGGL(k_infer<true>, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, spec, n_tiles, theta, f,
+        hipLaunchKernelGGL(k_infer<true>, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, in, n_tiles, theta, f,
                            mask);
     else
-        hipLaunchKernelGGL(k_infer<false>, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, spec, n_tiles, theta, f,
+        hipLaunchKernelGGL(k_infer<false>, dim3((uint32_t)g), dim3(64 * kInferWaves), 0, s, in, n_tiles, theta, f,
                            mask);
 }
 

@@ -435,14 +435,18 @@ static chd_status_t run_chunk(chd_tpal_plan *plan, const TpalWorkspace &ws, int 
         // learned mask: tile spectra -> network -> masked inverse
         chd_mask_net *net = plan->net;
         const int64_t tiles = (int64_t)n_units * g.n_ty3 * g.n_tx3;
-        // alternating parity: compact spectra, paired inference, half inverse
+        // alternating parity: magnitude features of rows ky < 16, paired
+        // inference, the half-form inverse after the tile transform computed
+        // again from the composite; otherwise full spectra stored and read
+        // back
         const bool paired = tpal::slabs_alternate(plan->h_slab_jobs, n_units);
-        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra,
-                            plan->d_spec, nullptr, 0, 0, nullptr, paired);
-        mnet::launch_infer(reinterpret_cast<const float2 *>(plan->d_spec), tiles, net->theta, net->f, plan->d_mask,
-                           net->cus, s, paired);
-        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply,
-                            plan->d_spec, plan->d_mask, 0, 0, nullptr, paired);
+        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s,
+                            paired ? tpal::kTileFeatures : tpal::kTileSpectra, plan->d_spec, nullptr, 0, 0, nullptr,
+                            paired);
+        mnet::launch_infer(plan->d_spec, tiles, net->theta, net->f, plan->d_mask, net->cus, s, paired);
+        tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s,
+                            paired ? tpal::kTileApplyFwd : tpal::kTileApply, plan->d_spec, plan->d_mask, 0, 0, nullptr,
+                            paired);
     } else if (plan->kind == CHD_DEC_TRANSFORM_3D)
         tpal::launch_tpal3d(g, ws.slab_jobs, n_units, plan->d_thr32, plan->d_thr64, s, tpal::kTilePlain, nullptr,
                             nullptr, 0, 0, nullptr, tpal::slabs_alternate(plan->h_slab_jobs, n_units));
@@ -941,12 +945,13 @@ void batch_tiles(chd_tpal_batch *b, hipStream_t s, int strip, int groups)
     const int64_t tps = (int64_t)g.n_ty3 * g.n_tx3;
     for (int u0 = 0; u0 < b->n_units; u0 += b->chunk_units) {
         const int cnt = std::min(b->chunk_units, b->n_units - u0);
-        tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileSpectra, b->d_spec,
-                            nullptr, strip, groups, b->d_wq, b->half);
-        mnet::launch_infer(reinterpret_cast<const float2 *>(b->d_spec), cnt * tps, net->theta, net->f, b->d_mask,
-                           net->cus, s, b->half);
-        tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s, tpal::kTileApply, b->d_spec,
-                            b->d_mask, strip, groups, b->d_wq, b->half);
+        tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s,
+                            b->half ? tpal::kTileFeatures : tpal::kTileSpectra, b->d_spec, nullptr, strip, groups,
+                            b->d_wq, b->half);
+        mnet::launch_infer(b->d_spec, cnt * tps, net->theta, net->f, b->d_mask, net->cus, s, b->half);
+        tpal::launch_tpal3d(g, b->d_slab + u0, cnt, plan->d_thr32, plan->d_thr64, s,
+                            b->half ? tpal::kTileApplyFwd : tpal::kTileApply, b->d_spec, b->d_mask, strip, groups,
+                            b->d_wq, b->half);
     }
 }
 

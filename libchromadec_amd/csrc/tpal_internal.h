@@ -109,8 +109,12 @@ struct Tables {
 hipError_t upload_tables(const Tables &t);
 // 3D tile kernel modes: the hard carrier-mirror mask; tile spectra out
 // (bins x = 2..6, kTileBins complex per tile); a soft per-visit mask applied
-// to stored spectra.
-enum { kTilePlain = 0, kTileSpectra = 1, kTileApply = 2 };
+// to stored spectra.  The paired learned path (half form only): the
+// magnitude features of rows ky < 16 out (kTileHalfBins floats per tile),
+// then the soft mask applied to the tile transform computed again from the
+// composite.
+enum { kTilePlain = 0, kTileSpectra = 1, kTileApply = 2, kTileFeatures = 3, kTileApplyFwd = 4 };
+constexpr int kTileHalfBins = 5 * 16 * 8;
 constexpr int kTileBins = 5 * 32 * 8;
 // strip: tiles per work item (a multiple of 4; 0 picks one for the launch
 // size with choose_strip3); max_groups > 0 caps the grid (persistent
@@ -120,10 +124,9 @@ constexpr int kTileBins = 5 * 32 * 8;
 // overlap.  Without wq the launch is one workgroup per item.
 // half: every slab's fields alternate in line parity (slabs_alternate), so
 // the kernel that derives the rows ky >= 16 of each tile from the rows
-// ky < 16 applies: in plain mode when g.thr_uniform >= 0 as well; in
-// spectra mode it writes compact spectra (rows ky < 16, 640 bins per tile)
-// for the paired inference, and in apply mode it reads them with the paired
-// inference's masks.
+// ky < 16 applies: in plain mode when g.thr_uniform >= 0 as well.  The
+// features and apply-forward modes require it (they launch nothing
+// without); the spectra and apply modes (full spectra in and out) ignore it.
 constexpr int kWorkCounters = 9;
 void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *thr32, const double *thr64,
                    hipStream_t s, int mode = kTilePlain, void *spec = nullptr, const float *mask = nullptr,

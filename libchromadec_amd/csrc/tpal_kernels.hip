@@ -36,6 +36,7 @@
 #include <utility>
 #include <vector>
 
+#include "mask_feature.h"
 #include "tpal_internal.h"
 
 namespace tpal {
@@ -144,6 +145,11 @@ __device__ __forceinline__ void fft(c32 (&v)[N])
 
 // Constant tables, filled once per device by the host (tpal_device.cpp).
 __constant__ Tables c_tab;
+
+// What a tile-row load reads when its row is black (outside the tile's lines
+// or an absent field): every lane loads, so that the loads sit on every path
+// and the compiler's wait counts stay exact; the value is then dropped.
+__device__ __align__(16) uint16_t g_no_row[16];
 
 // Error bound for FP32 bins: |bin error| <= kErrK * 2^-24 * L1, with L1 the
 // sum of |windowed, black-subtracted samples| of the tile.  A worst-case
@@ -322,9 +328,12 @@ __device__ __forceinline__ int decide64(double ar, double ai, double br, double 
 // are never live at the same time, so they share storage.
 constexpr int kApad = 17;
 constexpr int kSpad = 9;
-constexpr int kSk = 32 * kSpad + 2;
 
+// HALF: the half form stores only the spectrum rows ky < 16, so its k blocks
+// are 16 rows (and the workgroup needs 7 KB instead of 12.5).
+template <bool HALF>
 struct Lds3 {
+    static constexpr int kSk = (HALF ? 16 : 32) * kSpad + 2;   // k block stride of S
     union {
         c32 A[40][kApad];     // forward: row z*5+k holds the 16 data lines m
                               // inverse: row zs*5+k holds the 16 output lines
@@ -339,15 +348,18 @@ struct Lds3 {
     double wz[8], wy[32];     // its z and y window factors
 };
 
-// Apply mode holds the tile's soft mask (one value per visit) as well.
-struct Lds3M : Lds3 {
-    float m[768];
+// The soft-mask modes hold the tile's mask as well: one value per visit, or
+// per visit of planes z < 4 in the paired form.
+template <bool HALF, int NM>
+struct Lds3M : Lds3<HALF> {
+    float m[NM];
 };
-template <int MODE> struct LdsFor { using T = Lds3; };
-template <> struct LdsFor<kTileApply> { using T = Lds3M; };
+template <int MODE, bool HALF> struct LdsFor { using T = Lds3<HALF>; };
+template <bool HALF> struct LdsFor<kTileApply, HALF> { using T = Lds3M<HALF, 768>; };
+template <bool HALF> struct LdsFor<kTileApplyFwd, HALF> { using T = Lds3M<HALF, 384>; };
 
 template <class LT>
-__device__ __forceinline__ c32 &Sat(LT &L, int k, int ky, int kz) { return L.u.S[k * kSk + ky * kSpad + kz]; }
+__device__ __forceinline__ c32 &Sat(LT &L, int k, int ky, int kz) { return L.u.S[k * LT::kSk + ky * kSpad + kz]; }
 
 // Visit whose mask value applies to bin (x = k+2, ky, kz): its own visit for
 // x = 2..4, the visit it is the carrier-mirror partner of for x = 5, 6.
@@ -503,6 +515,11 @@ __device__ unsigned long long g_tile_phases[32];
 // ([tile][k][ky][kz], tile = (job * n_ty3 + ty) * n_tx3 + tx).
 // kTileApply: read those bins back, weight each by the soft mask of its
 // visit (`mask`, [tile][768]) and run the inverse and overlap-add.
+// The paired learned path (HALF only): kTileFeatures stops after the forward
+// transform and writes the magnitude feature of each bin of rows ky < 16
+// (`spec` as floats, [tile][k][ky < 16][kz], 640 per tile), all the paired
+// inference reads; kTileApplyFwd computes the forward transform again from
+// the composite (bit for bit the same bins) and applies the soft mask.
 // Work item = one strip of `strip` tiles of one tile row of one slab.  The
 // grid is either one workgroup per item (PERSIST false) or persistent (a
 // multiple of 8 workgroups, each looping over items).  HALF (plain mode):
@@ -514,10 +531,18 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
              const double *__restrict__ thr64, float2 *__restrict__ spec, const float *__restrict__ mask,
              unsigned *__restrict__ wq)
 {
-    __shared__ typename LdsFor<MODE>::T L;
+    __shared__ typename LdsFor<MODE, HALF>::T L;
+    constexpr int kSk = LdsFor<MODE, HALF>::T::kSk;
     const int lane0 = threadIdx.x;
     // the c2r scale of the inverse (2 / 4096; the half form's fold doubles it)
     constexpr float kInvScale = (HALF ? 4.0f : 2.0f) / 4096.0f;
+    static_assert(MODE == kTilePlain || HALF == (MODE == kTileFeatures || MODE == kTileApplyFwd),
+                  "the paired modes are half-form only, the stored-spectra modes full-form only");
+    // forward transform from the composite / soft mask in the inverse /
+    // inverse scale folded into the forward row weights
+    constexpr bool kForward = MODE != kTileApply;
+    constexpr bool kSoft = MODE == kTileApply || MODE == kTileApplyFwd;
+    constexpr bool kFoldScale = MODE == kTilePlain || MODE == kTileApplyFwd;
     CHD_PHASE_INIT();
     if (lane0 < 32) {
         L.cs[lane0] = c_tab.cos32_d[lane0];
@@ -580,7 +605,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     const int tx_own0 = strip * strip_len;
     const int tx_end = min(tx_own0 + strip_len, g.n_tx3);
     // the leading tile completes the first owned columns; spectra need none
-    const int tx_begin = MODE == kTileSpectra ? tx_own0 : max(tx_own0 - 1, 0);
+    const int tx_begin = (MODE == kTileSpectra || MODE == kTileFeatures) ? tx_own0 : max(tx_own0 - 1, 0);
 
     // forward-stage role: rows (z, m) = lane and lane + 64
     const uint16_t *rowp[2];
@@ -594,9 +619,10 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         const uint16_t *fp = job.field[rz[h]];
         const bool ok = fp != nullptr && ry[h] >= startY && ry[h] < endY;
         rowp[h] = ok ? fp + (int64_t)((tileY + ry[h]) >> 1) * g.pitch + g.lmargin : nullptr;
-        // plain mode folds the inverse's scale (a power of two, so the
-        // products round the same) into the forward row weight
-        roww[h] = c_tab.wz[rz[h]] * c_tab.wy[ry[h]] * (MODE == kTilePlain ? kInvScale : 1.0f);
+        // the modes that run the inverse after their own forward transform
+        // fold its scale (a power of two, so the products round the same)
+        // into the forward row weight
+        roww[h] = c_tab.wz[rz[h]] * c_tab.wy[ry[h]] * (kFoldScale ? kInvScale : 1.0f);
     }
     // vertical-stage roles: forward lanes (z, k) = lane / 5, inverse lanes
     // (zs, k) = lane / 5 with z = 0,1,4,5
@@ -640,9 +666,15 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     const int opar = (job.offset[oz] - tileY) & 1;
     const int oy = 2 * om + opar;
     const bool orow_ok = (oy >= startY) && (oy < endY);
-    float *orow = MODE == kTileSpectra ? nullptr
+    float *orow = (MODE == kTileSpectra || MODE == kTileFeatures) ? nullptr
                                        : job.partial + (int64_t)(zsel * 2 + (ty & 1)) * g.part_plane +
                                              (int64_t)((tileY + oy) >> 1) * g.cpitch;
+    // where a lane with no output columns stores: columns 0..7 of the plane's
+    // first row (the demodulators read the active columns, from kColOrigin
+    // on, and the tile stores start at kColOrigin - 8)
+    float *orow_none = (MODE == kTileSpectra || MODE == kTileFeatures)
+                           ? nullptr
+                           : job.partial + (int64_t)(zsel * 2 + (ty & 1)) * g.part_plane;
     float carry[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) carry[i] = 0.0f;
@@ -652,20 +684,25 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     // while the current tile is inverted (its loads would otherwise wait on
     // HBM once per tile).  The half form reads the compact spectra of the
     // paired path, rows ky < 16 only ([tile][k][ky < 16][kz], 640 bins).
-    constexpr int kSpecBins = kHalf ? 640 : 1280;
+    // soft masks: one per visit ([tile][768]), or in the paired form one per
+    // visit of planes z < 4 ([tile][384]; visit (z + 4, y + 16) repeats
+    // visit (z, y))
+    constexpr int kSpecBins = 1280;
     constexpr int kSpecPerLane = (MODE == kTileApply) ? kSpecBins / 64 : 1;
-    constexpr int kMaskPerLane = (MODE == kTileApply) ? 768 / 64 : 1;
+    constexpr int kMaskVisits = MODE == kTileApplyFwd ? 384 : 768;
+    constexpr int kMaskPerLane = kSoft ? kMaskVisits / 64 : 1;
     float2 pre_s[kSpecPerLane];
     float pre_m[kMaskPerLane];
     auto prefetch = [&](int64_t t) {
         if constexpr (MODE == kTileApply) {
 #pragma unroll
             for (int j = 0; j < kSpecPerLane; ++j) pre_s[j] = spec[t * kSpecBins + lane + 64 * j];
+        }
+        if constexpr (kSoft) {
 #pragma unroll
-            for (int j = 0; j < kMaskPerLane; ++j) pre_m[j] = mask[t * 768 + lane + 64 * j];
+            for (int j = 0; j < kMaskPerLane; ++j) pre_m[j] = mask[t * kMaskVisits + lane + 64 * j];
         }
     };
-    prefetch(((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx_begin);
 
     // This lane's two field rows of a tile (16 samples each); a row outside
     // the tile's lines reads as black (zero after black subtraction).  They
@@ -676,27 +713,37 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         const int tX = g.avs - kXT / 2 + (kXT / 2) * txl;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            rows[h][0] = rows[h][1] = make_uint4(blk2, blk2, blk2, blk2);
-            if (rowp[h] != nullptr) {
-                rows[h][0] = gload_u4(rowp[h] + tX);
-                rows[h][1] = gload_u4(rowp[h] + tX + 8);
-            }
+            // (a black row is substituted where the rows are used: written
+            // here, it would wait for the load it replaces)
+            const uint16_t *p = rowp[h] != nullptr ? rowp[h] + tX : g_no_row;
+            rows[h][0] = gload_u4(p);
+            rows[h][1] = gload_u4(p + 8);
         }
     };
-    if constexpr (MODE != kTileApply) load_rows(tx_begin);
+    if constexpr (kForward) {
+        load_rows(tx_begin);
+        // The first tile's rows are waited for here (s_waitcnt vmcnt(0)):
+        // the loop entry then has no row load in flight, and the loop waits
+        // for each next tile's rows by count, with the tile's stores (and the
+        // next mask) still in flight.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    prefetch(((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx_begin);
 
     for (int tx = tx_begin; tx < tx_end; ++tx) {
         const int tileX = g.avs - kXT / 2 + (kXT / 2) * tx;
 
         const int64_t tile = ((int64_t)job_id * g.n_ty3 + ty) * g.n_tx3 + tx;
         CHD_PHASE(9);
-        if constexpr (MODE != kTileApply) {
+        if constexpr (kForward) {
             // ---- forward x: two rows per lane -> x bins 2..6 ----------------
             float l1 = 0.0f;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 float s[16];
-                unpack16(rows[h][0], rows[h][1], g.black, s);
+                const uint4 blk4 = make_uint4(blk2, blk2, blk2, blk2);
+                unpack16(rowp[h] != nullptr ? rows[h][0] : blk4, rowp[h] != nullptr ? rows[h][1] : blk4, g.black, s);
                 c32 X[5];
                 l1 += xdft16(s, X) * roww[h];
                 const int m = (lane + 64 * h) & 15;
@@ -791,22 +838,35 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             CHD_PHASE(2);
 
             if constexpr (MODE == kTileSpectra) {
-                if (tx + 1 < tx_end) load_rows(tx + 1);
-                if constexpr (half) {
-                    // compact: rows ky < 16 only, [tile][k][ky < 16][kz]
-                    for (int i = lane; i < 640; i += 64) {
-                        const c32 v = Sat(L, i >> 7, (i >> 3) & 15, i & 7);
-                        spec[tile * 640 + i] = make_float2(v.x, v.y);
-                    }
-                } else {
-                    for (int i = lane; i < 1280; i += 64) {
-                        const c32 v = Sat(L, i >> 8, (i >> 3) & 31, i & 7);
-                        spec[tile * 1280 + i] = make_float2(v.x, v.y);
-                    }
+                load_rows(min(tx + 1, tx_end - 1));
+                for (int i = lane; i < 1280; i += 64) {
+                    const c32 v = Sat(L, i >> 8, (i >> 3) & 31, i & 7);
+                    spec[tile * 1280 + i] = make_float2(v.x, v.y);
                 }
                 __syncthreads();
                 continue;
             }
+            if constexpr (MODE == kTileFeatures) {
+                load_rows(min(tx + 1, tx_end - 1));
+                float *feat = reinterpret_cast<float *>(spec) + tile * 640;
+                for (int i = lane; i < 640; i += 64) {
+                    const c32 v = Sat(L, i >> 7, (i >> 3) & 15, i & 7);
+                    feat[i] = chd::mag_feature(v.x, v.y);
+                }
+                __syncthreads();
+                continue;
+            }
+            if constexpr (MODE == kTileApplyFwd) {
+                // the tile's soft mask (loaded during the previous tile)
+#pragma unroll
+                for (int j = 0; j < kMaskPerLane; ++j) L.m[lane + 64 * j] = pre_m[j];
+                __syncthreads();
+                // the next tile's rows, then its mask (the current tile's
+                // again on the last tile: the loads sit on every path)
+                load_rows(min(tx + 1, tx_end - 1));
+                __builtin_amdgcn_sched_barrier(0);   // the rows first: the next header waits for them only
+                prefetch(tile + (tx + 1 < tx_end ? 1 : 0));
+            } else {
 
             // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
             // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v].  Decisions are
@@ -1079,19 +1139,21 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 __syncthreads();
             }
             CHD_PHASE(5);
-            // the next tile's rows, in flight through the inverse
-            if (tx + 1 < tx_end) load_rows(tx + 1);
+            // the next tile's rows, in flight through the inverse (the
+            // current tile's again on the last tile: the loads sit on every
+            // path)
+            load_rows(min(tx + 1, tx_end - 1));
+            }  // hard mask
         } else {
 #pragma unroll
             for (int j = 0; j < kSpecPerLane; ++j) {
                 const int i = lane + 64 * j;
-                if constexpr (kHalf) Sat(L, i >> 7, (i >> 3) & 15, i & 7) = c32{pre_s[j].x, pre_s[j].y};
-                else Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{pre_s[j].x, pre_s[j].y};
+                Sat(L, i >> 8, (i >> 3) & 31, i & 7) = c32{pre_s[j].x, pre_s[j].y};
             }
 #pragma unroll
             for (int j = 0; j < kMaskPerLane; ++j) L.m[lane + 64 * j] = pre_m[j];
             __syncthreads();
-            if (tx + 1 < tx_end) prefetch(tile + 1);
+            prefetch(tile + (tx + 1 < tx_end ? 1 : 0));
         }
 
         // ---- masked inverse z: outputs z = 0,1,4,5 ------------------------
@@ -1101,12 +1163,20 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             const c32 htw = {ptw.x, -ptw.y};
             for (int c = lane; c < 80; c += 64) {
                 const int k = c >> 4, ky = c & 15;
+                // the paired masks' visit of bin (k, ky, z): its own visit
+                // (x <= 4) or the one it is the partner of, moved to planes
+                // z < 4 (visit (z + 4, y + 16) is visit (z, y)); with z a
+                // constant, a lane base plus an immediate
+                const int mown = ky * 3 + k, mlo = ((40 - ky) & 31) * 3 + (4 - k), mhi = (24 - ky) * 3 + (4 - k);
                 c32 v[8];
 #pragma unroll
                 for (int z = 0; z < 8; ++z) {
                     const c32 t = Sat(L, k, ky, z);
-                    if constexpr (MODE == kTileApply) {
-                        const float mv = L.m[mask_visit(k, ky, z)];
+                    if constexpr (MODE == kTileApplyFwd) {
+                        const int zm = (10 - z) & 7;
+                        const int own = mown + (z & 3) * 96 + (z >= 4 ? 48 : 0);
+                        const int mir = zm >= 4 ? mhi + (zm - 4) * 96 : mlo + zm * 96;
+                        const float mv = L.m[k <= 2 ? own : mir];
                         v[z] = c32{t.x * mv, t.y * mv};
                     } else {
                         v[z] = ((L.keep[k * 8 + z] >> ky) & 1u) ? t : c32{0.0f, 0.0f};
@@ -1135,7 +1205,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 #pragma unroll
             for (int z = 0; z < 8; ++z) {
                 const c32 t = Sat(L, k, ky, z);
-                if constexpr (MODE == kTileApply) {
+                if constexpr (kSoft) {
                     const float mv = L.m[mask_visit(k, ky, z)];
                     v[z] = c32{t.x * mv, t.y * mv};
                 } else {
@@ -1218,7 +1288,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         {
             c32 C[5];
             // (plain mode: the scale is in the forward row weights)
-            constexpr float sc = MODE == kTilePlain ? 1.0f : kInvScale;
+            constexpr float sc = kFoldScale ? 1.0f : kInvScale;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const c32 t = L.u.A[zsel * 5 + k][om];
@@ -1226,11 +1296,17 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             }
             float out[16];
             xidft16(C, out);
-            const bool owned = tx >= tx_own0;
-            if (owned && orow_ok) {
-                // columns tileX .. tileX+7 are complete (tiles tx-1 and tx)
+            // Columns tileX .. tileX+7 are complete (tiles tx-1 and tx).
+            // Every lane stores on every tile: a lane with nothing to write
+            // (a row outside the tile's lines, the strip's leading tile)
+            // stores to columns 0..7 of its plane's first row, which no
+            // kernel reads.  With the same stores on every path, the next
+            // tile's loads are waited for by count, not together with these
+            // stores.
+            const bool put = tx >= tx_own0 && orow_ok;
+            {
                 const int c0 = tileX - g.avs + kColOrigin;
-                float *dst = orow + c0;
+                float *dst = put ? orow + c0 : orow_none;
                 float v8[8];
                 if (tileX >= g.avs && tileX + 8 <= g.ave) {
                     // an interior tile: every column is active (a scalar
@@ -1246,7 +1322,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 }
                 gstore_f4(dst, v8[0], v8[1], v8[2], v8[3]);
                 gstore_f4(dst + 4, v8[4], v8[5], v8[6], v8[7]);
-                if (tx == g.n_tx3 - 1) {
+                if (put && tx == g.n_tx3 - 1) {
 #pragma unroll
                     for (int n = 0; n < 8; ++n) {
                         const int col = tileX + 8 + n;
@@ -2495,12 +2571,16 @@ void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *
     const dim3 grid(persist ? std::max(8, max_groups & ~7) : items), blk(64);
 #define CHD_LAUNCH3D(M, P, H)                                                                                \
     hipLaunchKernelGGL((tpal3d_tiles<M, P, H>), grid, blk, 0, s, g, jobs, n_jobs, strip, thr32, thr64, sp, mask, wq)
-    if (mode == kTileSpectra && half) {
-        if (persist) CHD_LAUNCH3D(kTileSpectra, true, true); else CHD_LAUNCH3D(kTileSpectra, false, true);
+    if (mode == kTileFeatures || mode == kTileApplyFwd) {
+        // the paired learned path: half form only (the host checked the slabs)
+        if (!half) return;
+        if (mode == kTileFeatures) {
+            if (persist) CHD_LAUNCH3D(kTileFeatures, true, true); else CHD_LAUNCH3D(kTileFeatures, false, true);
+        } else {
+            if (persist) CHD_LAUNCH3D(kTileApplyFwd, true, true); else CHD_LAUNCH3D(kTileApplyFwd, false, true);
+        }
     } else if (mode == kTileSpectra) {
         if (persist) CHD_LAUNCH3D(kTileSpectra, true, false); else CHD_LAUNCH3D(kTileSpectra, false, false);
-    } else if (mode == kTileApply && half) {
-        if (persist) CHD_LAUNCH3D(kTileApply, true, true); else CHD_LAUNCH3D(kTileApply, false, true);
     } else if (mode == kTileApply) {
         if (persist) CHD_LAUNCH3D(kTileApply, true, false); else CHD_LAUNCH3D(kTileApply, false, false);
     } else if (half && g.thr_uniform >= 0.0f) {

@@ -238,6 +238,42 @@ def test_all_ones_mask_equals_threshold_zero_bitwise(patch_capture):
     assert np.array_equal(c, d)
 
 
+def test_all_ones_mask_non_alternating_windows_bitwise(patch_capture, tmp_path):
+    """A second field marked first (SourceField::getOffset,
+    source_field.h:77-79) breaks the line-parity alternation of every window
+    holding it, so those frames take the learned path's full form (stored
+    spectra, one tile per inference unit, soft mask applied to the stored
+    bins) while the later frames take the paired form.  With a network whose
+    mask is 1 everywhere, both equal the threshold-0 decode bit for bit
+    through chd_decode_frame."""
+    import json
+    from libchromadec_amd import synth
+    _, fields = patch_capture
+    path = str(tmp_path / "order.tbc")
+    side = synth.write_tbc(path, fields)
+    with open(side) as fh:
+        meta = json.load(fh)
+    meta["fields"][3]["isFirstField"] = True   # second field of frame 1
+    with open(side, "w") as fh:
+        json.dump(meta, fh)
+    th = np.zeros(abi.MASK_NET_PARAMS, np.float32)
+    th[-1] = 40.0                                      # sigmoid(40) rounds to 1.0f
+    net = abi.MaskNet(max_tiles=1)
+    net.set_params(th)
+    weights = str(tmp_path / "ones.bin")
+    net.save(weights)
+    net.close()
+    v = abi.Video(path)
+    learned = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps", transform_mask="learned",
+                          transform_mask_weights=weights).commit()
+    plain = abi.Decoder(v, abi.CHD_DEC_TRANSFORM_3D, output_format="yuv444ps", transform_threshold=0.0).commit()
+    nf = fields.shape[0] // 2
+    for f in (0, 1, 3, nf - 1):                         # windows 0-3 hold field 3; the last does not
+        assert np.array_equal(learned.decode(f), plain.decode(f)), f
+    learned.close()
+    plain.close()
+
+
 def test_decode_separated_matches_oracle(orc):
     """Demodulation of given (true) chroma: the ideal-separation reference
     the learned mask is measured against."""
