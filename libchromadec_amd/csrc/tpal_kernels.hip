@@ -354,9 +354,16 @@ template <bool HALF, int NM>
 struct Lds3M : Lds3<HALF> {
     float m[NM];
 };
+// The masked inverse of the paired learned path stages each lane's columns
+// of three tiles and stores whole 128-byte lines with the fourth:
+// [tile % 4][half][lane].
+template <class B>
+struct LdsStaged : B {
+    float4 stage[3][2][64];
+};
 template <int MODE, bool HALF> struct LdsFor { using T = Lds3<HALF>; };
 template <bool HALF> struct LdsFor<kTileApply, HALF> { using T = Lds3M<HALF, 768>; };
-template <bool HALF> struct LdsFor<kTileApplyFwd, HALF> { using T = Lds3M<HALF, 384>; };
+template <bool HALF> struct LdsFor<kTileApplyFwd, HALF> { using T = LdsStaged<Lds3M<HALF, 384>>; };
 
 template <class LT>
 __device__ __forceinline__ c32 &Sat(LT &L, int k, int ky, int kz) { return L.u.S[k * LT::kSk + ky * kSpad + kz]; }
@@ -543,6 +550,8 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     constexpr bool kForward = MODE != kTileApply;
     constexpr bool kSoft = MODE == kTileApply || MODE == kTileApplyFwd;
     constexpr bool kFoldScale = MODE == kTilePlain || MODE == kTileApplyFwd;
+    // overlap-add stores of whole lines through an LDS stage
+    constexpr bool kStaged = MODE == kTileApplyFwd;
     CHD_PHASE_INIT();
     if (lane0 < 32) {
         L.cs[lane0] = c_tab.cos32_d[lane0];
@@ -669,9 +678,11 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     float *orow = (MODE == kTileSpectra || MODE == kTileFeatures) ? nullptr
                                        : job.partial + (int64_t)(zsel * 2 + (ty & 1)) * g.part_plane +
                                              (int64_t)((tileY + oy) >> 1) * g.cpitch;
-    // where a lane with no output columns stores: columns 0..7 of the plane's
-    // first row (the demodulators read the active columns, from kColOrigin
-    // on, and the tile stores start at kColOrigin - 8)
+    // where a lane with no output columns stores: columns 0..31 of its
+    // plane's first row, the same for every lane of the plane, so that these
+    // stores meet in one line of the cache (the demodulators read the active
+    // columns, from kColOrigin on, and the tile stores start at
+    // kColOrigin - 8)
     float *orow_none = (MODE == kTileSpectra || MODE == kTileFeatures)
                            ? nullptr
                            : job.partial + (int64_t)(zsel * 2 + (ty & 1)) * g.part_plane;
@@ -1299,11 +1310,20 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
             // Columns tileX .. tileX+7 are complete (tiles tx-1 and tx).
             // Every lane stores on every tile: a lane with nothing to write
             // (a row outside the tile's lines, the strip's leading tile)
-            // stores to columns 0..7 of its plane's first row, which no
-            // kernel reads.  With the same stores on every path, the next
-            // tile's loads are waited for by count, not together with these
-            // stores.
-            const bool put = tx >= tx_own0 && orow_ok;
+            // stores to columns 0..31 of its plane's first row, which no
+            // kernel reads.  With at least the same stores on every path,
+            // the next tile's loads are waited for by count, not together
+            // with these stores.
+            // Staged (the paired learned path's masked inverse): tiles
+            // 4j .. 4j+2 of a strip (its first owned tile is a multiple of 4,
+            // its columns start a 128-byte line) go to LDS, and tile 4j+3, or
+            // the strip's last tile, stores the group's columns, whole lines
+            // of each row.  That path also streams the masks in; with both
+            // the partial-line stores of the direct form and the mask loads,
+            // the kernel ran 1.5x slower.  The plain kernel, which loads
+            // only the composite, runs faster with the direct stores.
+            const bool owned = tx >= tx_own0;
+            const bool put = owned && orow_ok;
             {
                 const int c0 = tileX - g.avs + kColOrigin;
                 float *dst = put ? orow + c0 : orow_none;
@@ -1320,8 +1340,32 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                         v8[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
                     }
                 }
-                gstore_f4(dst, v8[0], v8[1], v8[2], v8[3]);
-                gstore_f4(dst + 4, v8[4], v8[5], v8[6], v8[7]);
+                if constexpr (kStaged) {
+                    const int slot = tx & 3;
+                    if (owned && slot < 3 && tx + 1 < tx_end) {
+                        // (a scalar branch) stage this tile's columns
+                        L.stage[slot][0][lane] = make_float4(v8[0], v8[1], v8[2], v8[3]);
+                        L.stage[slot][1][lane] = make_float4(v8[4], v8[5], v8[6], v8[7]);
+                        gstore_f4(orow_none, v8[0], v8[1], v8[2], v8[3]);
+                        gstore_f4(orow_none + 4, v8[4], v8[5], v8[6], v8[7]);
+                    } else if (owned) {
+                        // the group's staged tiles, then this one
+                        float *gd = put ? dst - 8 * slot : orow_none;
+                        for (int i = 0; i < slot; ++i) {
+                            const float4 a = L.stage[i][0][lane], b = L.stage[i][1][lane];
+                            gstore_f4(gd + 8 * i, a.x, a.y, a.z, a.w);
+                            gstore_f4(gd + 8 * i + 4, b.x, b.y, b.z, b.w);
+                        }
+                        gstore_f4(gd + 8 * slot, v8[0], v8[1], v8[2], v8[3]);
+                        gstore_f4(gd + 8 * slot + 4, v8[4], v8[5], v8[6], v8[7]);
+                    } else {
+                        gstore_f4(orow_none, v8[0], v8[1], v8[2], v8[3]);
+                        gstore_f4(orow_none + 4, v8[4], v8[5], v8[6], v8[7]);
+                    }
+                } else {
+                    gstore_f4(dst, v8[0], v8[1], v8[2], v8[3]);
+                    gstore_f4(dst + 4, v8[4], v8[5], v8[6], v8[7]);
+                }
                 if (put && tx == g.n_tx3 - 1) {
 #pragma unroll
                     for (int n = 0; n < 8; ++n) {
@@ -2564,6 +2608,7 @@ void launch_tpal3d(const Geom &g, const SlabJob *jobs, int n_jobs, const float *
 {
     float2 *sp = reinterpret_cast<float2 *>(spec);
     if (strip <= 0) strip = choose_strip3(g, n_jobs);
+    strip = (strip + 3) & ~3;   // strips start on a 4-tile group (whole-line stores)
     const int items = n_jobs * g.n_ty3 * ((g.n_tx3 + strip - 1) / strip);
     if (items <= 0) return;
     // persistent grids are whole multiples of 8 (one share per XCD)
