@@ -80,6 +80,36 @@ def test_every_launch_shape_gives_the_same_frames(sequences):
         b.set_shape(5, 0)
 
 
+def test_learned_batch_matches_decode_every_shape(sequences):
+    """The learned mask in a captured batch (magnitude features, paired
+    inference, the masked inverse with its staged whole-line stores) gives
+    the frames chd_tpal_decode gives, for strip lengths 4/8/24 and for one
+    workgroup per work item as well as persistent grids."""
+    import os
+    import torch
+    net = abi.MaskNet(max_tiles=1)
+    net.load(os.path.join(os.path.dirname(abi.__file__), "data", "tpal3d_mask.bin"))
+    plan = abi.TpalPlan(abi.pal_tpal_params(kind=abi.CHD_DEC_TRANSFORM_3D, max_batch_frames=16))
+    plan.set_mask_net(net)
+    keep, runs = _upload(plan, sequences)
+    total = sum(r[3] for r in runs)
+    ref = torch.zeros((total, plan.frame_bytes // 4), dtype=torch.float32, device="cuda:0")
+    plan.decode(runs, ref.data_ptr())
+    torch.cuda.synchronize()
+    out = torch.zeros_like(ref)
+    b = abi.TpalBatch(plan, runs, out.data_ptr(), autotune=False)
+    for strip in (4, 8, 24):
+        for groups in (0, 8, 256):
+            b.set_shape(strip, groups)
+            out.zero_()
+            b.run()
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (strip, groups, (out - ref).abs().max().item())
+    b.close()
+    plan.close()
+    net.close()
+
+
 @pytest.mark.parametrize("kind", [abi.CHD_DEC_TRANSFORM_3D, abi.CHD_DEC_PAL_2D])
 def test_luma_nr_plan_and_batch(orc, sequences, kind):
     """Luma coring (luma_nr_level, PalColour::doYNR) through chd_tpal_decode in
