@@ -879,281 +879,281 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                 prefetch(tile + (tx + 1 < tx_end ? 1 : 0));
             } else {
 
-            // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
-            // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v].  Decisions are
-            // gathered by wave ballots into keep words (plane (x-2, z), bit y);
-            // a kept visit also keeps its partner, whose word is the mirror of
-            // the ballot.  Keep is the OR over visits, as in the reference.
-            {
-                // Step i (unrolled) decides plane (x, z) = (2 + i/4, i%4) on
-                // lanes 0-31 and plane (x, z + 4) on lanes 32-63, so the
-                // half enters every offset as a lane term and the rest is an
-                // immediate.  The mirror of plane z lies at z' = (10 - z) % 8,
-                // which is 4 above the lower half's for z < 3 and 4 below for
-                // z = 3.  The next step's bins are read while this one decides.
-                // The uniform-threshold case (the default) is a loop of its
-                // own with no memory access but LDS: a load under a branch in
-                // every step would put a wait for all outstanding memory
-                // operations after each join.
-                const int yo = vy * kSpad + 4 * vhalf;
-                const int yrp = ((40 - vy) & 31) * kSpad + 4 * vhalf, yrm = yrp - 8 * vhalf;
-                const int tho = vy * 3 + 384 * vhalf;
-                auto off_a = [&](int i) { return (i >> 2) * kSk + (i & 3) + yo; };
-                auto off_b = [&](int i) {
-                    const int zr = (10 - (i & 3)) & 7;
-                    return (4 - (i >> 2)) * kSk + zr + ((i & 3) < 3 ? yrp : yrm);
-                };
-                auto steps = [&](auto uc) {
-                    constexpr bool kUniform = decltype(uc)::value;
-                    // With one threshold for every visit, plane (x, z) = (4, 2)
-                    // holds the pairs of plane (4, 0) with a and b swapped
-                    // (z' = (10 - z) % 8, rows mirrored), and (4, 6) those of
-                    // (4, 4).  decide32 is symmetric in a and b, so step 10 is
-                    // step 8's ballot moved to the mirror rows, and its pairs
-                    // are re-decided (if at all) through plane (4, 0) / (4, 4).
-                    constexpr int kSteps = kUniform ? 11 : 12;
-                    const DecideConst qu = decide_const(erru, g.thr_uniform);
-                    auto step_of = [](int j) { return (kUniform && j >= 10) ? j + 1 : j; };
-                    c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
-#pragma unroll
-                    for (int j = 0; j < kSteps; ++j) {
-                        const int i = step_of(j);
-                        c32 an = a, bn = b;
-                        if (j + 1 < kSteps) {
-                            an = L.u.S[off_a(step_of(j + 1))];
-                            bn = L.u.S[off_b(step_of(j + 1))];
-                        }
-                        const int k = i >> 2, z = i & 3;
-                        const DecideConst q = kUniform ? qu : decide_const(erru, thr32[z * 96 + k + tho]);
-                        uint64_t kb, fb;
-                        decide32_ballots(a, b, q, kb, fb);
-                        if (lane == 0) {
-                            // planes (k, z) on lanes 0-31 and (k, z + 4) on lanes 32-63
-                            *reinterpret_cast<uint64_t *>(&L.bal[pw(k, z)]) = kb;
-                            *reinterpret_cast<uint64_t *>(&L.flg[pw(k, z)]) = fb;
-                            if (kUniform && i == 8) {
-                                const uint64_t m = mirror_rows((uint32_t)kb) |
-                                                   ((uint64_t)mirror_rows((uint32_t)(kb >> 32)) << 32);
-                                *reinterpret_cast<uint64_t *>(&L.bal[pw(k, 2)]) = m;
-                                *reinterpret_cast<uint64_t *>(&L.flg[pw(k, 2)]) = 0u;
-                            }
-                        }
-                        a = an;
-                        b = bn;
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
-                };
-                // The half form: planes z < 4 only, rows ky >= 16 read from
-                // their stored images, rows ky - 16 at kz + 4 (bins of equal
-                // magnitude).  Steps 0-3 decide (x, z) = (2, s) on lanes 0-31
-                // and (3, s) on lanes 32-63.  Step 4 decides the x = 4 pairs,
-                // each once: plane (4, 0) on lanes 0-31 (plane (4, 2) holds
-                // its pairs, a and b swapped, as in the full form); plane
-                // (4, 1), whose visit y and 8 - y share a pair, on lanes 32-46
-                // for y = 5..19 (its self-mirror bins y = 4, 20 are kept
-                // anyway); plane (4, 3), whose visit y and 24 - y share a pair
-                // through the image rows, on lanes 47-63 for y = 12..28.  The
-                // other visits of those planes take the decision of their
-                // pair's (a mirrored ballot); only the deciding visit is
-                // flagged, and its FP64 re-decision sets both bins.
-                auto steps_half = [&]() {
-                    const int yr = (40 - vy) & 31;
-                    const int a0 = (vy & 15) * kSpad + 4 * (vy >> 4);
-                    // mirror z' = (10 - z) % 8 is 2 - z for z < 3 and 7 for
-                    // z = 3; a mirror row yr >= 16 adds 4 (mod 8) to it
-                    const int b0 = (yr & 15) * kSpad + 4 * (yr >> 4), b0m = b0 - 8 * (yr >> 4);
-                    // step 4: this lane's visit (z4, y4) of x = 4 and its mirror
-                    const int y4 = lane < 32 ? lane : lane < 47 ? lane - 27 : lane - 35;
-                    const int z4 = lane < 32 ? 0 : lane < 47 ? 1 : 3;
-                    const int yr4 = (40 - y4) & 31, zr4 = (10 - z4) & 7;
-                    const int a4 = 2 * kSk + (y4 & 15) * kSpad + z4 + 4 * (y4 >> 4);
-                    const int b4 = 2 * kSk + (yr4 & 15) * kSpad + ((zr4 + 4 * (yr4 >> 4)) & 7);
-                    auto off_a = [&](int st) { return st < 4 ? a0 + vhalf * kSk + st : a4; };
-                    auto off_b = [&](int st) {
-                        return st < 3    ? b0 - vhalf * kSk + 4 * kSk + (2 - st)
-                               : st == 3 ? b0m - vhalf * kSk + 4 * kSk + 7
-                                         : b4;
+                // ---- carrier-mirror mask (transform_pal_3d.cpp:277-330) -----------
+                // visit v = (z*32 + y)*3 + (x-2) has threshold thr[v].  Decisions are
+                // gathered by wave ballots into keep words (plane (x-2, z), bit y);
+                // a kept visit also keeps its partner, whose word is the mirror of
+                // the ballot.  Keep is the OR over visits, as in the reference.
+                {
+                    // Step i (unrolled) decides plane (x, z) = (2 + i/4, i%4) on
+                    // lanes 0-31 and plane (x, z + 4) on lanes 32-63, so the
+                    // half enters every offset as a lane term and the rest is an
+                    // immediate.  The mirror of plane z lies at z' = (10 - z) % 8,
+                    // which is 4 above the lower half's for z < 3 and 4 below for
+                    // z = 3.  The next step's bins are read while this one decides.
+                    // The uniform-threshold case (the default) is a loop of its
+                    // own with no memory access but LDS: a load under a branch in
+                    // every step would put a wait for all outstanding memory
+                    // operations after each join.
+                    const int yo = vy * kSpad + 4 * vhalf;
+                    const int yrp = ((40 - vy) & 31) * kSpad + 4 * vhalf, yrm = yrp - 8 * vhalf;
+                    const int tho = vy * 3 + 384 * vhalf;
+                    auto off_a = [&](int i) { return (i >> 2) * kSk + (i & 3) + yo; };
+                    auto off_b = [&](int i) {
+                        const int zr = (10 - (i & 3)) & 7;
+                        return (4 - (i >> 2)) * kSk + zr + ((i & 3) < 3 ? yrp : yrm);
                     };
-                    const DecideConst q = decide_const(erru, g.thr_uniform);
-                    c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
+                    auto steps = [&](auto uc) {
+                        constexpr bool kUniform = decltype(uc)::value;
+                        // With one threshold for every visit, plane (x, z) = (4, 2)
+                        // holds the pairs of plane (4, 0) with a and b swapped
+                        // (z' = (10 - z) % 8, rows mirrored), and (4, 6) those of
+                        // (4, 4).  decide32 is symmetric in a and b, so step 10 is
+                        // step 8's ballot moved to the mirror rows, and its pairs
+                        // are re-decided (if at all) through plane (4, 0) / (4, 4).
+                        constexpr int kSteps = kUniform ? 11 : 12;
+                        const DecideConst qu = decide_const(erru, g.thr_uniform);
+                        auto step_of = [](int j) { return (kUniform && j >= 10) ? j + 1 : j; };
+                        c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
 #pragma unroll
-                    for (int st = 0; st < 5; ++st) {
-                        c32 an = a, bn = b;
-                        if (st + 1 < 5) {
-                            an = L.u.S[off_a(st + 1)];
-                            bn = L.u.S[off_b(st + 1)];
-                        }
-                        uint64_t kb, fb;
-                        decide32_ballots(a, b, q, kb, fb);
-                        if (lane == 0) {
-                            if (st < 4) {
-                                L.bal[pw(0, st)] = (uint32_t)kb;
-                                L.flg[pw(0, st)] = (uint32_t)fb;
-                                L.bal[pw(1, st)] = (uint32_t)(kb >> 32);
-                                L.flg[pw(1, st)] = (uint32_t)(fb >> 32);
-                            } else {
-                                auto rot16 = [](uint32_t w) { return (w << 16) | (w >> 16); };
-                                const uint32_t w1 = (uint32_t)((kb >> 32) & 0x7FFFu) << 5;
-                                const uint32_t w3 = (uint32_t)((kb >> 47) & 0x1FFFFu) << 12;
-                                L.bal[pw(2, 0)] = (uint32_t)kb;
-                                L.flg[pw(2, 0)] = (uint32_t)fb;
-                                L.bal[pw(2, 2)] = mirror_rows((uint32_t)kb);
-                                L.flg[pw(2, 2)] = 0u;
-                                L.bal[pw(2, 1)] = w1 | mirror_rows(w1);
-                                L.flg[pw(2, 1)] = (uint32_t)((fb >> 32) & 0x7FFFu) << 5;
-                                L.bal[pw(2, 3)] = w3 | rot16(mirror_rows(w3));
-                                L.flg[pw(2, 3)] = (uint32_t)((fb >> 47) & 0x1FFFFu) << 12;
+                        for (int j = 0; j < kSteps; ++j) {
+                            const int i = step_of(j);
+                            c32 an = a, bn = b;
+                            if (j + 1 < kSteps) {
+                                an = L.u.S[off_a(step_of(j + 1))];
+                                bn = L.u.S[off_b(step_of(j + 1))];
                             }
+                            const int k = i >> 2, z = i & 3;
+                            const DecideConst q = kUniform ? qu : decide_const(erru, thr32[z * 96 + k + tho]);
+                            uint64_t kb, fb;
+                            decide32_ballots(a, b, q, kb, fb);
+                            if (lane == 0) {
+                                // planes (k, z) on lanes 0-31 and (k, z + 4) on lanes 32-63
+                                *reinterpret_cast<uint64_t *>(&L.bal[pw(k, z)]) = kb;
+                                *reinterpret_cast<uint64_t *>(&L.flg[pw(k, z)]) = fb;
+                                if (kUniform && i == 8) {
+                                    const uint64_t m = mirror_rows((uint32_t)kb) |
+                                                       ((uint64_t)mirror_rows((uint32_t)(kb >> 32)) << 32);
+                                    *reinterpret_cast<uint64_t *>(&L.bal[pw(k, 2)]) = m;
+                                    *reinterpret_cast<uint64_t *>(&L.flg[pw(k, 2)]) = 0u;
+                                }
+                            }
+                            a = an;
+                            b = bn;
+                            __builtin_amdgcn_sched_barrier(0);
                         }
-                        a = an;
-                        b = bn;
-                        __builtin_amdgcn_sched_barrier(0);
+                    };
+                    // The half form: planes z < 4 only, rows ky >= 16 read from
+                    // their stored images, rows ky - 16 at kz + 4 (bins of equal
+                    // magnitude).  Steps 0-3 decide (x, z) = (2, s) on lanes 0-31
+                    // and (3, s) on lanes 32-63.  Step 4 decides the x = 4 pairs,
+                    // each once: plane (4, 0) on lanes 0-31 (plane (4, 2) holds
+                    // its pairs, a and b swapped, as in the full form); plane
+                    // (4, 1), whose visit y and 8 - y share a pair, on lanes 32-46
+                    // for y = 5..19 (its self-mirror bins y = 4, 20 are kept
+                    // anyway); plane (4, 3), whose visit y and 24 - y share a pair
+                    // through the image rows, on lanes 47-63 for y = 12..28.  The
+                    // other visits of those planes take the decision of their
+                    // pair's (a mirrored ballot); only the deciding visit is
+                    // flagged, and its FP64 re-decision sets both bins.
+                    auto steps_half = [&]() {
+                        const int yr = (40 - vy) & 31;
+                        const int a0 = (vy & 15) * kSpad + 4 * (vy >> 4);
+                        // mirror z' = (10 - z) % 8 is 2 - z for z < 3 and 7 for
+                        // z = 3; a mirror row yr >= 16 adds 4 (mod 8) to it
+                        const int b0 = (yr & 15) * kSpad + 4 * (yr >> 4), b0m = b0 - 8 * (yr >> 4);
+                        // step 4: this lane's visit (z4, y4) of x = 4 and its mirror
+                        const int y4 = lane < 32 ? lane : lane < 47 ? lane - 27 : lane - 35;
+                        const int z4 = lane < 32 ? 0 : lane < 47 ? 1 : 3;
+                        const int yr4 = (40 - y4) & 31, zr4 = (10 - z4) & 7;
+                        const int a4 = 2 * kSk + (y4 & 15) * kSpad + z4 + 4 * (y4 >> 4);
+                        const int b4 = 2 * kSk + (yr4 & 15) * kSpad + ((zr4 + 4 * (yr4 >> 4)) & 7);
+                        auto off_a = [&](int st) { return st < 4 ? a0 + vhalf * kSk + st : a4; };
+                        auto off_b = [&](int st) {
+                            return st < 3    ? b0 - vhalf * kSk + 4 * kSk + (2 - st)
+                                   : st == 3 ? b0m - vhalf * kSk + 4 * kSk + 7
+                                             : b4;
+                        };
+                        const DecideConst q = decide_const(erru, g.thr_uniform);
+                        c32 a = L.u.S[off_a(0)], b = L.u.S[off_b(0)];
+#pragma unroll
+                        for (int st = 0; st < 5; ++st) {
+                            c32 an = a, bn = b;
+                            if (st + 1 < 5) {
+                                an = L.u.S[off_a(st + 1)];
+                                bn = L.u.S[off_b(st + 1)];
+                            }
+                            uint64_t kb, fb;
+                            decide32_ballots(a, b, q, kb, fb);
+                            if (lane == 0) {
+                                if (st < 4) {
+                                    L.bal[pw(0, st)] = (uint32_t)kb;
+                                    L.flg[pw(0, st)] = (uint32_t)fb;
+                                    L.bal[pw(1, st)] = (uint32_t)(kb >> 32);
+                                    L.flg[pw(1, st)] = (uint32_t)(fb >> 32);
+                                } else {
+                                    auto rot16 = [](uint32_t w) { return (w << 16) | (w >> 16); };
+                                    const uint32_t w1 = (uint32_t)((kb >> 32) & 0x7FFFu) << 5;
+                                    const uint32_t w3 = (uint32_t)((kb >> 47) & 0x1FFFFu) << 12;
+                                    L.bal[pw(2, 0)] = (uint32_t)kb;
+                                    L.flg[pw(2, 0)] = (uint32_t)fb;
+                                    L.bal[pw(2, 2)] = mirror_rows((uint32_t)kb);
+                                    L.flg[pw(2, 2)] = 0u;
+                                    L.bal[pw(2, 1)] = w1 | mirror_rows(w1);
+                                    L.flg[pw(2, 1)] = (uint32_t)((fb >> 32) & 0x7FFFu) << 5;
+                                    L.bal[pw(2, 3)] = w3 | rot16(mirror_rows(w3));
+                                    L.flg[pw(2, 3)] = (uint32_t)((fb >> 47) & 0x1FFFFu) << 12;
+                                }
+                            }
+                            a = an;
+                            b = bn;
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    };
+                    if (erru == 0.0f) {
+                        // every windowed sample is zero, so is every bin: all pairs
+                        // are kept (0 < 0 is false), none needs re-deciding
+                        if (lane < 24) {
+                            L.bal[lane] = ~0u;
+                            L.flg[lane] = 0u;
+                        }
+                    } else if (half) {
+                        steps_half();
+                    } else if (g.thr_uniform >= 0.0f) {
+                        steps(std::true_type{});
+                    } else {
+                        steps(std::false_type{});
                     }
-                };
-                if (erru == 0.0f) {
-                    // every windowed sample is zero, so is every bin: all pairs
-                    // are kept (0 < 0 is false), none needs re-deciding
-                    if (lane < 24) {
-                        L.bal[lane] = ~0u;
-                        L.flg[lane] = 0u;
+                }
+                __syncthreads();
+                CHD_PHASE(3);
+                // keep word (k, z) = own visits' ballot | ballot of the visit plane
+                // whose partners lie on (k, z), moved to the mirror rows
+                // (x_v = 8 - x, z_v = (10 - z) % 8); plus the self-mirror bins
+                // (x = 4, y in {4, 20}, z in {1, 5}), which are always kept.
+                constexpr uint32_t kSelf = (1u << 4) | (1u << 20);
+                auto rot16 = [](uint32_t w) { return (w << 16) | (w >> 16); };
+                if (lane < 40) {
+                    const int k = lane >> 3, z = lane & 7;
+                    uint32_t w;
+                    if (half) {
+                        // from the planes z < 4; plane (x, 7) is plane (x, 3)
+                        // rotated, and word (x, z + 4) is word (x, z) rotated
+                        const int zl = z & 3;
+                        w = (k <= 2) ? L.bal[pw(k, zl)] : 0u;
+                        if (k >= 2) w |= mirror_rows(zl < 3 ? L.bal[pw(4 - k, 2 - zl)] : rot16(L.bal[pw(4 - k, 3)]));
+                        if (k == 2 && zl == 1) w |= kSelf;
+                        if (z >= 4) w = rot16(w);
+                    } else {
+                        w = (k <= 2) ? L.bal[pw(k, z)] : 0u;
+                        if (k >= 2) w |= mirror_rows(L.bal[pw(4 - k, (10 - z) & 7)]);
+                        if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
                     }
-                } else if (half) {
-                    steps_half();
-                } else if (g.thr_uniform >= 0.0f) {
-                    steps(std::true_type{});
-                } else {
-                    steps(std::false_type{});
+                    L.keep[lane] = w;
                 }
-            }
-            __syncthreads();
-            CHD_PHASE(3);
-            // keep word (k, z) = own visits' ballot | ballot of the visit plane
-            // whose partners lie on (k, z), moved to the mirror rows
-            // (x_v = 8 - x, z_v = (10 - z) % 8); plus the self-mirror bins
-            // (x = 4, y in {4, 20}, z in {1, 5}), which are always kept.
-            constexpr uint32_t kSelf = (1u << 4) | (1u << 20);
-            auto rot16 = [](uint32_t w) { return (w << 16) | (w >> 16); };
-            if (lane < 40) {
-                const int k = lane >> 3, z = lane & 7;
-                uint32_t w;
-                if (half) {
-                    // from the planes z < 4; plane (x, 7) is plane (x, 3)
-                    // rotated, and word (x, z + 4) is word (x, z) rotated
-                    const int zl = z & 3;
-                    w = (k <= 2) ? L.bal[pw(k, zl)] : 0u;
-                    if (k >= 2) w |= mirror_rows(zl < 3 ? L.bal[pw(4 - k, 2 - zl)] : rot16(L.bal[pw(4 - k, 3)]));
-                    if (k == 2 && zl == 1) w |= kSelf;
-                    if (z >= 4) w = rot16(w);
-                } else {
-                    w = (k <= 2) ? L.bal[pw(k, z)] : 0u;
-                    if (k >= 2) w |= mirror_rows(L.bal[pw(4 - k, (10 - z) & 7)]);
-                    if (k == 2 && (z == 1 || z == 5)) w |= kSelf;
-                }
-                L.keep[lane] = w;
-            }
-            // flag words (x - 2) * 8 + z on lanes 0-23, self-mirror bins
-            // removed (the half form flags planes z < 4 only)
-            const bool flag_lane = lane < 24 && (!half || (lane & 7) < 4);
-            const uint32_t myflag =
-                flag_lane ? (L.flg[pw(lane >> 3, lane & 7)] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
-            const uint64_t fwords = __ballot(myflag != 0u);
-            __syncthreads();
-            CHD_PHASE(4);
-            CHD_PHASE_COUNT(16, fwords != 0u);
-            CHD_PHASE_COUNT(17, __popcll(fwords));
+                // flag words (x - 2) * 8 + z on lanes 0-23, self-mirror bins
+                // removed (the half form flags planes z < 4 only)
+                const bool flag_lane = lane < 24 && (!half || (lane & 7) < 4);
+                const uint32_t myflag =
+                    flag_lane ? (L.flg[pw(lane >> 3, lane & 7)] & ((lane == 17 || lane == 21) ? ~kSelf : ~0u)) : 0u;
+                const uint64_t fwords = __ballot(myflag != 0u);
+                __syncthreads();
+                CHD_PHASE(4);
+                CHD_PHASE_COUNT(16, fwords != 0u);
+                CHD_PHASE_COUNT(17, __popcll(fwords));
 
-            // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
-            // Each lane forms the FP64 DFT of its two rows at bins x and 8 - x
-            // once per flagged x; a flagged visit turns them by the (ky, kz)
-            // phase of its bins and a wave reduction gives the exact bins.  x is
-            // a compile-time constant, so the row DFT's twiddles are immediates.
-            if (fwords != 0u) {
-                const double black = (double)g.black;
-                auto bins = [&](auto xc) {
-                    constexpr int X = decltype(xc)::value, XR = 8 - X;
-                    uint32_t zmask = (uint32_t)(fwords >> (8 * (X - 2))) & 0xFFu;
-                    if (zmask == 0u) return;
-                    // With c, s the cos/sin of 2 pi X n / 16, bin X of a row is
-                    // sum u (c - i s) and bin 8-X is sum (-1)^n u (c + i s): split
-                    // by n parity.  A row outside the tile's lines holds black:
-                    // u = 0.
-                    // one row at a time: both rows' FP64 samples at once would
-                    // not fit the tile kernel's register budget
-                    double ra[2], ia[2], rb[2], ib[2];
+                // ---- FP64 re-decision of comparisons FP32 cannot settle ----------
+                // Each lane forms the FP64 DFT of its two rows at bins x and 8 - x
+                // once per flagged x; a flagged visit turns them by the (ky, kz)
+                // phase of its bins and a wave reduction gives the exact bins.  x is
+                // a compile-time constant, so the row DFT's twiddles are immediates.
+                if (fwords != 0u) {
+                    const double black = (double)g.black;
+                    auto bins = [&](auto xc) {
+                        constexpr int X = decltype(xc)::value, XR = 8 - X;
+                        uint32_t zmask = (uint32_t)(fwords >> (8 * (X - 2))) & 0xFFu;
+                        if (zmask == 0u) return;
+                        // With c, s the cos/sin of 2 pi X n / 16, bin X of a row is
+                        // sum u (c - i s) and bin 8-X is sum (-1)^n u (c + i s): split
+                        // by n parity.  A row outside the tile's lines holds black:
+                        // u = 0.
+                        // one row at a time: both rows' FP64 samples at once would
+                        // not fit the tile kernel's register budget
+                        double ra[2], ia[2], rb[2], ib[2];
 #pragma unroll 1
-                    for (int h = 0; h < 2; ++h) {
-                        const uint4 q0 = h ? rows[1][0] : rows[0][0], q1 = h ? rows[1][1] : rows[0][1];
-                        const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-                        double rw = (h ? rowp[1] : rowp[0]) != nullptr
-                                        ? L.wz[h ? rz[1] : rz[0]] * L.wy[h ? ry[1] : ry[0]]
-                                        : 0.0;
-                        // opaque per bin pair: otherwise the 32 windowed samples
-                        // are shared by the three pairs and held in registers
-                        asm volatile("" : "+v"(rw));
-                        double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
+                        for (int h = 0; h < 2; ++h) {
+                            const uint4 q0 = h ? rows[1][0] : rows[0][0], q1 = h ? rows[1][1] : rows[0][1];
+                            const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+                            double rw = (h ? rowp[1] : rowp[0]) != nullptr
+                                            ? L.wz[h ? rz[1] : rz[0]] * L.wy[h ? ry[1] : ry[0]]
+                                            : 0.0;
+                            // opaque per bin pair: otherwise the 32 windowed samples
+                            // are shared by the three pairs and held in registers
+                            asm volatile("" : "+v"(rw));
+                            double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
 #pragma unroll
-                        for (int n = 0; n < 16; n += 2) {
-                            const uint32_t w2 = raw[n >> 1];
-                            const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
-                            const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
-                            ce = fma(u0, kCos16[(X * n) & 15], ce);
-                            se = fma(u0, kSin16[(X * n) & 15], se);
-                            co = fma(u1, kCos16[(X * (n + 1)) & 15], co);
-                            so = fma(u1, kSin16[(X * (n + 1)) & 15], so);
-                        }
-                        const double r0 = ce + co, i0 = -(se + so), r1 = ce - co, i1 = se - so;
-                        if (h == 0) {
-                            ra[0] = r0; ia[0] = i0; rb[0] = r1; ib[0] = i1;
-                        } else {
-                            ra[1] = r0; ia[1] = i0; rb[1] = r1; ib[1] = i1;
-                        }
-                    }
-                    while (zmask != 0u) {
-                        const int z = __builtin_ctz(zmask);
-                        zmask &= zmask - 1u;
-                        uint32_t bits = (uint32_t)__builtin_amdgcn_readlane((int)myflag, (X - 2) * 8 + z);
-                        const int zr = (10 - z) & 7;
-                        while (bits != 0u) {
-                            const int y = __builtin_ctz(bits);
-                            bits &= bits - 1u;
-                            const int yr = (40 - y) & 31;
-                            double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-#pragma unroll
-                            for (int h = 0; h < 2; ++h) {
-                                // turn by the (ky, kz) phase of the row: 4 kz z + ky y (32ths)
-                                const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
-                                const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
-                                const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
-                                a0 += ra[h] * ca + ia[h] * sa;
-                                a1 += ia[h] * ca - ra[h] * sa;
-                                b0 += rb[h] * cb + ib[h] * sb;
-                                b1 += ib[h] * cb - rb[h] * sb;
+                            for (int n = 0; n < 16; n += 2) {
+                                const uint32_t w2 = raw[n >> 1];
+                                const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
+                                const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
+                                ce = fma(u0, kCos16[(X * n) & 15], ce);
+                                se = fma(u0, kSin16[(X * n) & 15], se);
+                                co = fma(u1, kCos16[(X * (n + 1)) & 15], co);
+                                so = fma(u1, kSin16[(X * (n + 1)) & 15], so);
                             }
-                            wave_sum4_to63(a0, a1, b0, b1);
-                            const double th =
-                                g.thr_uniform >= 0.0f ? g.thr_uniform_d : thr64[(z * 32 + y) * 3 + (X - 2)];
-                            if (lane == 63 && decide64(a0, a1, b0, b1, th)) {
-                                L.keep[(X - 2) * 8 + z] |= 1u << y;
-                                L.keep[(XR - 2) * 8 + zr] |= 1u << yr;
-                                if (half) {
-                                    // and the image visit (z + 4, y + 16)
-                                    L.keep[(X - 2) * 8 + z + 4] |= 1u << ((y + 16) & 31);
-                                    L.keep[(XR - 2) * 8 + ((zr + 4) & 7)] |= 1u << ((yr + 16) & 31);
+                            const double r0 = ce + co, i0 = -(se + so), r1 = ce - co, i1 = se - so;
+                            if (h == 0) {
+                                ra[0] = r0; ia[0] = i0; rb[0] = r1; ib[0] = i1;
+                            } else {
+                                ra[1] = r0; ia[1] = i0; rb[1] = r1; ib[1] = i1;
+                            }
+                        }
+                        while (zmask != 0u) {
+                            const int z = __builtin_ctz(zmask);
+                            zmask &= zmask - 1u;
+                            uint32_t bits = (uint32_t)__builtin_amdgcn_readlane((int)myflag, (X - 2) * 8 + z);
+                            const int zr = (10 - z) & 7;
+                            while (bits != 0u) {
+                                const int y = __builtin_ctz(bits);
+                                bits &= bits - 1u;
+                                const int yr = (40 - y) & 31;
+                                double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+                                for (int h = 0; h < 2; ++h) {
+                                    // turn by the (ky, kz) phase of the row: 4 kz z + ky y (32ths)
+                                    const int pa = (4 * z * rz[h] + y * ry[h]) & 31;
+                                    const int pb = (4 * zr * rz[h] + yr * ry[h]) & 31;
+                                    const double ca = L.cs[pa], sa = L.sn[pa], cb = L.cs[pb], sb = L.sn[pb];
+                                    a0 += ra[h] * ca + ia[h] * sa;
+                                    a1 += ia[h] * ca - ra[h] * sa;
+                                    b0 += rb[h] * cb + ib[h] * sb;
+                                    b1 += ib[h] * cb - rb[h] * sb;
+                                }
+                                wave_sum4_to63(a0, a1, b0, b1);
+                                const double th =
+                                    g.thr_uniform >= 0.0f ? g.thr_uniform_d : thr64[(z * 32 + y) * 3 + (X - 2)];
+                                if (lane == 63 && decide64(a0, a1, b0, b1, th)) {
+                                    L.keep[(X - 2) * 8 + z] |= 1u << y;
+                                    L.keep[(XR - 2) * 8 + zr] |= 1u << yr;
+                                    if (half) {
+                                        // and the image visit (z + 4, y + 16)
+                                        L.keep[(X - 2) * 8 + z + 4] |= 1u << ((y + 16) & 31);
+                                        L.keep[(XR - 2) * 8 + ((zr + 4) & 7)] |= 1u << ((yr + 16) & 31);
+                                    }
                                 }
                             }
                         }
-                    }
-                };
-                bins(std::integral_constant<int, 2>{});
-                bins(std::integral_constant<int, 3>{});
-                bins(std::integral_constant<int, 4>{});
-                __syncthreads();
-            }
-            CHD_PHASE(5);
-            // the next tile's rows, in flight through the inverse (the
-            // current tile's again on the last tile: the loads sit on every
-            // path)
-            load_rows(min(tx + 1, tx_end - 1));
+                    };
+                    bins(std::integral_constant<int, 2>{});
+                    bins(std::integral_constant<int, 3>{});
+                    bins(std::integral_constant<int, 4>{});
+                    __syncthreads();
+                }
+                CHD_PHASE(5);
+                // the next tile's rows, in flight through the inverse (the
+                // current tile's again on the last tile: the loads sit on every
+                // path)
+                load_rows(min(tx + 1, tx_end - 1));
             }  // hard mask
         } else {
 #pragma unroll
