@@ -43,6 +43,18 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi)
     return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
 }
 
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// ReLU and BF16 of two values: converted first, then the negative ones
+// cleared as signed 16-bit integers (a BF16 with its sign bit set is a
+// negative int16), one packed integer max for the pair.  The bits are those
+// of converting the ReLU'd values (rounding keeps the sign).
+__device__ __forceinline__ uint32_t pack2_relu(float lo, float hi)
+{
+    const s16x2 v = __builtin_bit_cast(s16x2, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, (s16x2){0, 0}));
+}
+
 // Magnitude feature of a bin: (ln(|a| + 1) - 10) / 3 (mask_feature.h).
 __device__ __forceinline__ float mag_feature(float2 a) { return chd::mag_feature(a.x, a.y); }
 
@@ -555,8 +567,8 @@ __device__ __forceinline__ void store_act(char *act, int ebase, const f32x16 (&a
     for (int b = 0; b < kInferBlocks; ++b)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const uint2 pk = make_uint2(pack2(fmaxf(acc[b][4 * j], 0.0f), fmaxf(acc[b][4 * j + 1], 0.0f)),
-                                        pack2(fmaxf(acc[b][4 * j + 2], 0.0f), fmaxf(acc[b][4 * j + 3], 0.0f)));
+            const uint2 pk = make_uint2(pack2_relu(acc[b][4 * j], acc[b][4 * j + 1]),
+                                        pack2_relu(acc[b][4 * j + 2], acc[b][4 * j + 3]));
             *reinterpret_cast<uint2 *>(act + ebase + ((b / 3) * 96 + b % 3) * kActRowB + 16 * j) = pk;
         }
 }
