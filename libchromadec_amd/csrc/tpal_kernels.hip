@@ -286,14 +286,44 @@ __device__ __forceinline__ float swap_pair(float x)
 }
 
 // One DPP move of a double (its two halves); rows or banks outside the masks
-// read 0.
+// read 0.  With every row and bank enabled (the permutations used here read
+// a lane of every row) no lane keeps its old value, so none is zeroed first.
 template <int CTRL, int ROWMASK = 0xF, int BANKMASK = 0xF>
 __device__ __forceinline__ double dpp_d(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, BANKMASK, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, BANKMASK, false);
+    int lo, hi;
+    if constexpr (ROWMASK == 0xF && BANKMASK == 0xF) {
+        lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+        hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    } else {
+        lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, BANKMASK, false);
+        hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, BANKMASK, false);
+    }
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One DPP move of a float, as dpp_d.
+template <int CTRL, int ROWMASK = 0xF, int BANKMASK = 0xF>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    if constexpr (ROWMASK == 0xF && BANKMASK == 0xF)
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+    else
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, BANKMASK, false));
+}
+
+// The wave's total of v, the same in every lane (a scalar): DPP sums into
+// lane 63, as wave_sum4_to63 below, then that lane read out.
+__device__ __forceinline__ float wave_total(float v)
+{
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    v += dpp_f<0x142, 0xA>(v);
+    v += dpp_f<0x143, 0xC>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Four wave sums into lane 63 with DPP moves, no LDS round trips: pairs
@@ -761,10 +791,9 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 #pragma unroll
                 for (int j = 0; j < 5; ++j) L.u.A[rz[h] * 5 + j][m] = c32{X[j].x * roww[h], X[j].y * roww[h]};
             }
-            const float err = kErrK * kU * wave_sum(l1);
             // the same in every lane: a scalar, so that the mask steps branch
             // on it uniformly
-            const float erru = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(err)));
+            const float erru = kErrK * kU * wave_total(l1);
             __syncthreads();
             CHD_PHASE(0);
 
