@@ -161,20 +161,6 @@ constexpr float kErrK = 8.0f;    // 3D tile
 constexpr float kErrK2 = 12.0f;  // 2D tile (32-term x DFT)
 constexpr float kU = 5.9604645e-08f;  // 2^-24
 
-// Wave-wide reductions (64 lanes).
-__device__ __forceinline__ float wave_sum(float v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_sum_d(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // Keep/discard decision with an FP32 uncertainty test
 // (transform_pal_3d.cpp:313-327: discard when |a|^2 < |b|^2 t^2 or
 // |b|^2 < |a|^2 t^2).  Returns 1 keep, 0 discard, -1 when FP32 error `e` on
@@ -263,20 +249,6 @@ __device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
 constexpr double kC1 = 0.9238795325112867, kS1 = 0.3826834323650898, kR2 = 0.7071067811865476;
 constexpr double kCos16[16] = {1.0, kC1, kR2, kS1, 0.0, -kS1, -kR2, -kC1, -1.0, -kC1, -kR2, -kS1, 0.0, kS1, kR2, kC1};
 constexpr double kSin16[16] = {0.0, kS1, kR2, kC1, 1.0, kC1, kR2, kS1, 0.0, -kS1, -kR2, -kC1, -1.0, -kC1, -kR2, -kS1};
-
-// Four wave sums at once, so that the shuffles' latencies overlap.
-__device__ __forceinline__ void wave_sum4_d(double &a, double &b, double &c, double &d)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double ta = __shfl_xor(a, o, 64), tb = __shfl_xor(b, o, 64);
-        const double tc = __shfl_xor(c, o, 64), td = __shfl_xor(d, o, 64);
-        a += ta;
-        b += tb;
-        c += tc;
-        d += td;
-    }
-}
 
 // The value of the neighbouring lane of a pair (lane ^ 1): one DPP move,
 // quad_perm 1,0,3,2.
@@ -1455,23 +1427,22 @@ __device__ __forceinline__ void load32(const uint16_t *p, float black, float (&s
     }
 }
 
+// Sums over each group of 16 lanes (one DPP row), in every lane of the
+// group: pairs, quads, half rows, rows.
 __device__ __forceinline__ float group_sum16(float v)
 {
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
     return v;
 }
 __device__ __forceinline__ void group_sum16_4d(double &a, double &b, double &c, double &d)
 {
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-        const double ta = __shfl_xor(a, o, 64), tb = __shfl_xor(b, o, 64);
-        const double tc = __shfl_xor(c, o, 64), td = __shfl_xor(d, o, 64);
-        a += ta;
-        b += tb;
-        c += tc;
-        d += td;
-    }
+    a += dpp_d<0xB1>(a); b += dpp_d<0xB1>(b); c += dpp_d<0xB1>(c); d += dpp_d<0xB1>(d);
+    a += dpp_d<0x4E>(a); b += dpp_d<0x4E>(b); c += dpp_d<0x4E>(c); d += dpp_d<0x4E>(d);
+    a += dpp_d<0x141>(a); b += dpp_d<0x141>(b); c += dpp_d<0x141>(c); d += dpp_d<0x141>(d);
+    a += dpp_d<0x140>(a); b += dpp_d<0x140>(b); c += dpp_d<0x140>(c); d += dpp_d<0x140>(d);
 }
 
 // Windowed 32-sample real DFT to bins 4..12 (X[j] = bin j+4), folded by the
@@ -1906,10 +1877,10 @@ pal_demod(const Geom g, const FrameJob *__restrict__ jobs, const float *__restri
                 bpo += b * sine[i];
                 bqo += b * cosine[i];
             }
-            bp = wave_sum(bp) / blen;
-            bq = wave_sum(bq) / blen;
-            bpo = wave_sum(bpo) / blen;
-            bqo = wave_sum(bqo) / blen;
+            bp = wave_total(bp) / blen;
+            bq = wave_total(bq) / blen;
+            bpo = wave_total(bpo) / blen;
+            bqo = wave_total(bqo) / blen;
             if (lane == 0) {
                 const float vs = (((bp - bpo) * (bp - bpo) + (bq - bqo) * (bq - bqo)) < (bp * bp + bq * bq) * 2) ? 1.0f : -1.0f;
                 float lbp = (bp - bqo) * 0.5f, lbq = (bq + bpo) * 0.5f;
@@ -2078,10 +2049,10 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
                 bpo += b * sine[i];
                 bqo += b * cosine[i];
             }
-            bp = wave_sum(bp) / blen;
-            bq = wave_sum(bq) / blen;
-            bpo = wave_sum(bpo) / blen;
-            bqo = wave_sum(bqo) / blen;
+            bp = wave_total(bp) / blen;
+            bq = wave_total(bq) / blen;
+            bpo = wave_total(bpo) / blen;
+            bqo = wave_total(bqo) / blen;
             if (lane == 0) {
                 const float vs = (((bp - bpo) * (bp - bpo) + (bq - bqo) * (bq - bqo)) < (bp * bp + bq * bq) * 2) ? 1.0f : -1.0f;
                 float lbp = (bp - bqo) * 0.5f, lbq = (bq + bpo) * 0.5f;
@@ -2242,10 +2213,10 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 bpo += b * sine[i];
                 bqo += b * cosine[i];
             }
-            bp = wave_sum(bp) / blen;
-            bq = wave_sum(bq) / blen;
-            bpo = wave_sum(bpo) / blen;
-            bqo = wave_sum(bqo) / blen;
+            bp = wave_total(bp) / blen;
+            bq = wave_total(bq) / blen;
+            bpo = wave_total(bpo) / blen;
+            bqo = wave_total(bqo) / blen;
             if (lane == 0) {
                 const float vs = (((bp - bpo) * (bp - bpo) + (bq - bqo) * (bq - bqo)) < (bp * bp + bq * bq) * 2) ? 1.0f : -1.0f;
                 float lbp = (bp - bqo) * 0.5f, lbq = (bq + bpo) * 0.5f;
