@@ -148,7 +148,7 @@ __constant__ Tables c_tab;
 
 // What a tile-row load reads when its row is black (outside the tile's lines
 // or an absent field): every lane loads, so that the loads sit on every path
-// and the compiler's wait counts stay exact; the value is then dropped.
+// and the compiler's wait counts stay exact; the row's weight is 0.
 __device__ __align__(16) uint16_t g_no_row[16];
 
 // Error bound for FP32 bins: |bin error| <= kErrK * 2^-24 * L1, with L1 the
@@ -632,8 +632,11 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         rowp[h] = ok ? fp + (int64_t)((tileY + ry[h]) >> 1) * g.pitch + g.lmargin : nullptr;
         // the modes that run the inverse after their own forward transform
         // fold its scale (a power of two, so the products round the same)
-        // into the forward row weight
-        roww[h] = c_tab.wz[rz[h]] * c_tab.wy[ry[h]] * (kFoldScale ? kInvScale : 1.0f);
+        // into the forward row weight.  A black row (outside the tile's
+        // lines, or an absent field) loads a stand-in and gets weight 0: its
+        // bins and its share of the error bound are then exactly 0, as for
+        // black samples.
+        roww[h] = ok ? c_tab.wz[rz[h]] * c_tab.wy[ry[h]] * (kFoldScale ? kInvScale : 1.0f) : 0.0f;
     }
     // vertical-stage roles: forward lanes (z, k) = lane / 5, inverse lanes
     // (zs, k) = lane / 5 with z = 0,1,4,5
@@ -691,7 +694,6 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     float carry[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) carry[i] = 0.0f;
-    const uint32_t blk2 = (uint32_t)g.black * 0x10001u;   // two black samples
 
     // apply mode: the next tile's spectrum and mask are loaded into registers
     // while the current tile is inverted (its loads would otherwise wait on
@@ -726,8 +728,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
         const int tX = g.avs - kXT / 2 + (kXT / 2) * txl;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            // (a black row is substituted where the rows are used: written
-            // here, it would wait for the load it replaces)
+            // (a black row reads a stand-in, weighted 0 in the transform)
             const uint16_t *p = rowp[h] != nullptr ? rowp[h] + tX : g_no_row;
             rows[h][0] = gload_u4(p);
             rows[h][1] = gload_u4(p + 8);
@@ -755,8 +756,7 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 float s[16];
-                const uint4 blk4 = make_uint4(blk2, blk2, blk2, blk2);
-                unpack16(rowp[h] != nullptr ? rows[h][0] : blk4, rowp[h] != nullptr ? rows[h][1] : blk4, g.black, s);
+                unpack16(rows[h][0], rows[h][1], g.black, s);
                 c32 X[5];
                 l1 += xdft16(s, X) * roww[h];
                 const int m = (lane + 64 * h) & 15;
