@@ -456,8 +456,12 @@ __device__ __forceinline__ float xdft16(const float (&s)[16], c32 (&X)[5])
 // Even bins repeat every 8 samples, odd bins change sign.
 __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
 {
+    // Sample n = 0 and 4 (quarter-turn twiddles), then n = 1..3 with 8 - n:
+    // at t = 2 pi k n / 16, sample 8 - n has cos (-1)^k cos t and sin
+    // -(-1)^k sin t, so both come from the even-k and odd-k sums of
+    // C.x cos t and C.y sin t.  Sample n + 8 is E - O.
 #pragma unroll
-    for (int n = 0; n < 8; ++n) {
+    for (int n = 0; n <= 4; n += 4) {
         float E = 0.0f, O = 0.0f;
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
@@ -473,6 +477,28 @@ __device__ __forceinline__ void xidft16(const c32 (&C)[5], float (&out)[16])
         }
         out[n] = E + O;
         out[n + 8] = E - O;
+    }
+#pragma unroll
+    for (int n = 1; n < 4; ++n) {
+        float pc = 0.0f, ps = 0.0f, qc = 0.0f, qs = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const int k = j + 2;
+            const float c = cos32(2 * k * n), s = sin32(2 * k * n);
+            if (k & 1) {
+                qc = cmac(qc, C[j].x, c);
+                qs = cmac(qs, C[j].y, s);
+            } else {
+                pc = cmac(pc, C[j].x, c);
+                ps = cmac(ps, C[j].y, s);
+            }
+        }
+        const float En = pc - ps, Em = pc + ps;       // E at n and 8 - n
+        const float On = qc - qs, Om = -(qc + qs);    // O at n and 8 - n
+        out[n] = En + On;
+        out[n + 8] = En - On;
+        out[8 - n] = Em + Om;
+        out[16 - n] = Em - Om;
     }
 }
 
