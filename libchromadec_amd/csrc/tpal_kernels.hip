@@ -1507,8 +1507,10 @@ __device__ __forceinline__ float xdft32(const float (&s)[32], c32 (&X)[9])
 // Even bins repeat every 16 samples, odd bins change sign.
 __device__ __forceinline__ void xidft32(const c32 (&C)[9], float (&out)[32])
 {
+    // As xidft16: samples n = 0 and 8 directly, n = 1..7 with 16 - n from
+    // the even-k and odd-k sums of C.x cos t and C.y sin t (t = 2 pi k n / 32).
 #pragma unroll
-    for (int n = 0; n < 16; ++n) {
+    for (int n = 0; n <= 8; n += 8) {
         float E = 0.0f, O = 0.0f;
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
@@ -1524,6 +1526,28 @@ __device__ __forceinline__ void xidft32(const c32 (&C)[9], float (&out)[32])
         }
         out[n] = E + O;
         out[n + 16] = E - O;
+    }
+#pragma unroll
+    for (int n = 1; n < 8; ++n) {
+        float pc = 0.0f, ps = 0.0f, qc = 0.0f, qs = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int k = j + 4;
+            const float c = cos32(k * n), sn = sin32(k * n);
+            if (k & 1) {
+                qc = cmac(qc, C[j].x, c);
+                qs = cmac(qs, C[j].y, sn);
+            } else {
+                pc = cmac(pc, C[j].x, c);
+                ps = cmac(ps, C[j].y, sn);
+            }
+        }
+        const float En = pc - ps, Em = pc + ps;       // E at n and 16 - n
+        const float On = qc - qs, Om = -(qc + qs);    // O at n and 16 - n
+        out[n] = En + On;
+        out[n + 16] = En - On;
+        out[16 - n] = Em + Om;
+        out[32 - n] = Em - Om;
     }
 }
 
