@@ -1111,24 +1111,27 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
                         for (int h = 0; h < 2; ++h) {
                             const uint4 q0 = h ? rows[1][0] : rows[0][0], q1 = h ? rows[1][1] : rows[0][1];
                             const uint32_t raw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-                            double rw = (h ? rowp[1] : rowp[0]) != nullptr
-                                            ? L.wz[h ? rz[1] : rz[0]] * L.wy[h ? ry[1] : ry[0]]
-                                            : 0.0;
+                            // the row's z and y window factor, applied to the sums
+                            const double rw = (h ? rowp[1] : rowp[0]) != nullptr
+                                                  ? L.wz[h ? rz[1] : rz[0]] * L.wy[h ? ry[1] : ry[0]]
+                                                  : 0.0;
                             // opaque per bin pair: otherwise the 32 windowed samples
                             // are shared by the three pairs and held in registers
-                            asm volatile("" : "+v"(rw));
+                            double blk = black;
+                            asm volatile("" : "+v"(blk));
                             double ce = 0.0, co = 0.0, se = 0.0, so = 0.0;
 #pragma unroll
                             for (int n = 0; n < 16; n += 2) {
                                 const uint32_t w2 = raw[n >> 1];
-                                const double u0 = ((double)(w2 & 0xFFFFu) - black) * (rw * c_tab.wx_d[n]);
-                                const double u1 = ((double)(w2 >> 16) - black) * (rw * c_tab.wx_d[n + 1]);
+                                const double u0 = ((double)(w2 & 0xFFFFu) - blk) * c_tab.wx_d[n];
+                                const double u1 = ((double)(w2 >> 16) - blk) * c_tab.wx_d[n + 1];
                                 ce = fma(u0, kCos16[(X * n) & 15], ce);
                                 se = fma(u0, kSin16[(X * n) & 15], se);
                                 co = fma(u1, kCos16[(X * (n + 1)) & 15], co);
                                 so = fma(u1, kSin16[(X * (n + 1)) & 15], so);
                             }
-                            const double r0 = ce + co, i0 = -(se + so), r1 = ce - co, i1 = se - so;
+                            const double r0 = (ce + co) * rw, i0 = -(se + so) * rw, r1 = (ce - co) * rw,
+                                         i1 = (se - so) * rw;
                             if (h == 0) {
                                 ra[0] = r0; ia[0] = i0; rb[0] = r1; ib[0] = i1;
                             } else {
