@@ -2252,19 +2252,35 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
             const int number = n0 + li;
             float bp = 0, bq = 0, bpo = 0, bqo = 0;
             for (int i = g.cbs + lane; i < g.cbe; i += 64) {
-                auto px = [&](int ln) -> float {
-                    if (ln < 0 || ln >= g.fh) return 0.0f;  // blackLine (zeros)
-                    if (comp == nullptr) return g.black;
-                    return (float)((const gu16 *)comp)[(int64_t)ln * g.pitch + i + g.lmargin];
-                };
-                const float in0 = px(number), in1 = px(number - 1), in2 = px(number + 1);
-                const float in3 = px(number - 2), in4 = px(number + 2);
-                const float a = (in0 - (in3 + in4) * 0.5f) * 0.5f;
-                const float b = (in2 - in1) * 0.5f;
-                bp += a * sine[i];
-                bq += a * cosine[i];
-                bpo += b * sine[i];
-                bqo += b * cosine[i];
+                // lines number, -1, +1, -2, +2: all seven loads issued before
+                // any is used, and every loaded value used (times 0 where it
+                // does not count), so that none moves under a branch of its
+                // own.  A line outside the field is the black line of zeros,
+                // an absent field black (its loads read the filter table).
+                constexpr int kDl[5] = {0, -1, 1, -2, 2};
+                const uint16_t *const src = comp != nullptr ? comp : reinterpret_cast<const uint16_t *>(cfilt);
+                float raw[5];
+#pragma unroll
+                for (int d = 0; d < 5; ++d) {
+                    const int ln = min(max(number + kDl[d], 0), g.fh - 1);
+                    const int64_t o = comp != nullptr ? (int64_t)ln * g.pitch + i + g.lmargin : 0;
+                    raw[d] = (float)((const gu16 *)src)[o];
+                }
+                const float sn = sine[i], cs = cosine[i];
+                float in[5];
+#pragma unroll
+                for (int d = 0; d < 5; ++d) {
+                    const int ln = number + kDl[d];
+                    const bool inside = ln >= 0 && ln < g.fh;
+                    in[d] = raw[d] * (inside && comp != nullptr ? 1.0f : 0.0f) +
+                            (inside && comp == nullptr ? g.black : 0.0f);
+                }
+                const float a = (in[0] - (in[3] + in[4]) * 0.5f) * 0.5f;
+                const float b = (in[2] - in[1]) * 0.5f;
+                bp += a * sn;
+                bq += a * cs;
+                bpo += b * sn;
+                bqo += b * cs;
             }
             bp = wave_total(bp) / blen;
             bq = wave_total(bq) / blen;
@@ -2338,11 +2354,14 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     };
 
     // carrier (exactly one of sine/cosine is non-zero, +-1) of the 4 columns
+    // (every load issued before any is used and every value used: a column
+    // outside the field reads a clamped column and is weighted 0)
     float w[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int col = g.avs + c0 + e;
-        w[e] = (loads && col >= 0 && col < g.fw) ? sine[col] + cosine[col] : 0.0f;
+        const int cc = min(max(col, 0), g.fw - 1);
+        w[e] = (sine[cc] + cosine[cc]) * ((loads && col >= 0 && col < g.fw) ? 1.0f : 0.0f);
     }
     constexpr int NCF = pal2d ? 6 : 4;   // chroma taps, plus the luma taps for PAL_2D
     float cf[kQHalo + 1][NCF];
@@ -2355,10 +2374,19 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
 
     const OutSpec os = out_spec(g);
 
-    // rows n0-3 .. n0+3 of this thread's columns
+    // rows n0-3 .. n0+3 of this thread's columns, each row's loads issued
+    // three rows ahead of its sum (then line n0+4's, for the loop)
     float4 r[7];
+    Raw pre[3], qa;
+    pre[0] = issue(n0 - 3);
+    pre[1] = issue(n0 - 2);
+    pre[2] = issue(n0 - 1);
 #pragma unroll
-    for (int k = 0; k < 7; ++k) r[k] = sum(issue(n0 - 3 + k));
+    for (int k = 0; k < 7; ++k) {
+        r[k] = sum(pre[k % 3]);
+        if (k + 3 < 7) pre[k % 3] = issue(n0 + k);
+        else if (k + 3 == 7) qa = issue(n0 + 4);
+    }
     // Line li: rows r[0..6] are lines li-3 .. li+3; `use` holds the loads
     // of line li+4, `refill` receives those of line li+5.  The loop is
     // unrolled by two with the two buffers swapping roles, so that no load
@@ -2466,7 +2494,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
         for (int k = 0; k < 6; ++k) r[k] = r[k + 1];
         r[6] = sum(use);
     };
-    Raw qa = issue(n0 + 4), qb;
+    Raw qb;
     for (int li = 0; li < nl; li += 2) {
         line(li, qa, qb);
         if (li + 1 < nl) line(li + 1, qb, qa);
