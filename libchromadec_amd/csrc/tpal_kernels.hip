@@ -2245,50 +2245,62 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
     }
 
     // ---- burst detection per line (palcolour.cpp:344-409) -------------------
+    // A wave takes lines li = wave, wave + 4, ... (at most LINES / 4), in
+    // batches of kQ lines whose loads are all issued before any is summed:
+    // lines n, n-1, n+1, n-2 and n+2 of each, every loaded value used (times
+    // 0 where it does not count) so that none moves under a branch of its
+    // own.  A line outside the field is the black line of zeros, an absent
+    // field black (its loads read the filter table).
     {
         const int wave = tid >> 6, lane = tid & 63;
         const int blen = g.cbe - g.cbs;
-        for (int li = wave; li < nl; li += kDemodThreads / 64) {
-            const int number = n0 + li;
-            float bp = 0, bq = 0, bpo = 0, bqo = 0;
-            for (int i = g.cbs + lane; i < g.cbe; i += 64) {
-                // lines number, -1, +1, -2, +2: all seven loads issued before
-                // any is used, and every loaded value used (times 0 where it
-                // does not count), so that none moves under a branch of its
-                // own.  A line outside the field is the black line of zeros,
-                // an absent field black (its loads read the filter table).
-                constexpr int kDl[5] = {0, -1, 1, -2, 2};
-                const uint16_t *const src = comp != nullptr ? comp : reinterpret_cast<const uint16_t *>(cfilt);
-                float raw[5];
+        constexpr int kPer = LINES / (kDemodThreads / 64);
+        constexpr int kQ = kPer < 2 ? kPer : 2;
+        constexpr int kDl[5] = {0, -1, 1, -2, 2};
+        const uint16_t *const src = comp != nullptr ? comp : reinterpret_cast<const uint16_t *>(cfilt);
+        for (int l0 = wave; l0 < nl; l0 += 4 * kQ) {
+        float bp[kQ], bq[kQ], bpo[kQ], bqo[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) bp[q] = bq[q] = bpo[q] = bqo[q] = 0.0f;
+        for (int i = g.cbs + lane; i < g.cbe; i += 64) {
+            const float sn = sine[i], cs = cosine[i];
+            float raw[kQ][5];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q)
 #pragma unroll
                 for (int d = 0; d < 5; ++d) {
-                    const int ln = min(max(number + kDl[d], 0), g.fh - 1);
+                    const int ln = min(max(n0 + l0 + 4 * q + kDl[d], 0), g.fh - 1);
                     const int64_t o = comp != nullptr ? (int64_t)ln * g.pitch + i + g.lmargin : 0;
-                    raw[d] = (float)((const gu16 *)src)[o];
+                    raw[q][d] = (float)((const gu16 *)src)[o];
                 }
-                const float sn = sine[i], cs = cosine[i];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
                 float in[5];
 #pragma unroll
                 for (int d = 0; d < 5; ++d) {
-                    const int ln = number + kDl[d];
+                    const int ln = n0 + l0 + 4 * q + kDl[d];
                     const bool inside = ln >= 0 && ln < g.fh;
-                    in[d] = raw[d] * (inside && comp != nullptr ? 1.0f : 0.0f) +
+                    in[d] = raw[q][d] * (inside && comp != nullptr ? 1.0f : 0.0f) +
                             (inside && comp == nullptr ? g.black : 0.0f);
                 }
                 const float a = (in[0] - (in[3] + in[4]) * 0.5f) * 0.5f;
                 const float b = (in[2] - in[1]) * 0.5f;
-                bp += a * sn;
-                bq += a * cs;
-                bpo += b * sn;
-                bqo += b * cs;
+                bp[q] += a * sn;
+                bq[q] += a * cs;
+                bpo[q] += b * sn;
+                bqo[q] += b * cs;
             }
-            bp = wave_total(bp) / blen;
-            bq = wave_total(bq) / blen;
-            bpo = wave_total(bpo) / blen;
-            bqo = wave_total(bqo) / blen;
+        }
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int li = l0 + 4 * q;
+            if (li >= nl) break;
+            const float tp = wave_total(bp[q]) / blen, tq = wave_total(bq[q]) / blen;
+            const float tpo = wave_total(bpo[q]) / blen, tqo = wave_total(bqo[q]) / blen;
             if (lane == 0) {
-                const float vs = (((bp - bpo) * (bp - bpo) + (bq - bqo) * (bq - bqo)) < (bp * bp + bq * bq) * 2) ? 1.0f : -1.0f;
-                float lbp = (bp - bqo) * 0.5f, lbq = (bq + bpo) * 0.5f;
+                const float vs =
+                    (((tp - tpo) * (tp - tpo) + (tq - tqo) * (tq - tqo)) < (tp * tp + tq * tq) * 2) ? 1.0f : -1.0f;
+                float lbp = (tp - tqo) * 0.5f, lbq = (tq + tpo) * 0.5f;
                 const float norm = fmaxf(sqrtf(lbp * lbp + lbq * lbq), g.burst_floor);
                 lbp /= norm;
                 lbq /= norm;
@@ -2296,6 +2308,7 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
                 L.bq[li] = (lbp * g.chroma_gain_sin + lbq * g.chroma_gain_cos);
                 L.vsw[li] = vs;
             }
+        }
         }
     }
 
