@@ -2187,11 +2187,11 @@ pal_demod_quad(const Geom g, const FrameJob *__restrict__ jobs, const float *__r
 // line.  NC is the launch's FrameJob.nchroma: 4 (3D), 2 (2D) or 1 partial
 // planes summed per row, or 0 (CHD_DEC_PAL_2D), which filters the composite
 // itself and takes luma from the re-synthesised chroma.  A row's loads are
-// issued one line ahead and summed only after the current line's filter, so
-// HBM latency overlaps a line of work.
+// issued lines ahead and summed only after the current line's filter, so
+// HBM latency overlaps lines of work.
 // ---------------------------------------------------------------------------
 
-constexpr int kQ4Lines = 32;
+constexpr int kQ4Lines = 48;
 constexpr int kQ4ShortLines = 8;                       // lines per workgroup of small launches
 constexpr int kQ4Cols = 4 * kDemodThreads;             // columns held by the threads
 constexpr int kQ4MaxWidth = kQ4Cols - 2 * kQHalo;       // widest active line: halos on both sides
@@ -2787,10 +2787,10 @@ static void demod_kernels(const Geom &g, const FrameJob *jobs, int n_jobs, const
 #undef CHD_Q4
         // Every workgroup runs about as long as its lines plus the 6-line
         // halo, and the grid drains in rounds of the resident slots: pick
-        // the lines per workgroup (16 .. 32) that makes the fewest rounds of
-        // the shortest workgroups.  (For 64 PAL frames, 24 lines give 1536
-        // workgroups, two full rounds of 3 per CU, where 32 give 1152, one
-        // and a half.)
+        // the lines per workgroup (16 .. 48) that makes the fewest rounds of
+        // the shortest workgroups.  (For 64 PAL frames, 48 lines give 768
+        // workgroups, one full round of 3 per CU; 24 give two rounds with
+        // twice the halo rows, 32 one and a half.)
         int lpb = short_blocks ? kQ4ShortLines : kQ4Lines;
         if (!short_blocks) {
             const int64_t slots = (int64_t)std::max(1, resident_per_cu((const void *)kern)) * cu_count();
