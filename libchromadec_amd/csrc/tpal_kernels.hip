@@ -671,9 +671,9 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
     // capture): row ky+16 of plane z is (-1)^par[z] = sgn (-1)^z times row ky,
     // so after the temporal DFT T[ky+16][kz] = sgn T[ky][(kz+4) & 7] and only
     // rows 0..15 need the vertical stores and the temporal DFT.
-    // (the launcher checked the parity.  Spectra mode then writes the compact
-    // spectra of the paired learned path, rows ky < 16; apply mode reads them
-    // with masks from the paired inference, which repeat at (z + 4, y + 16).)
+    // (the launcher checked the parity.  The features mode then writes the
+    // magnitudes of rows ky < 16 for the paired inference, and the
+    // apply-forward mode reads its masks, which repeat at (z + 4, y + 16).)
     constexpr bool kHalf = HALF;
     const bool alt = kHalf || pbits == 0xAA || pbits == 0x55;
     const float sgn = par[0] ? -1.0f : 1.0f;
