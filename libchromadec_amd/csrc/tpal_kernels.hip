@@ -78,6 +78,12 @@ __device__ __forceinline__ void gstore_f4(float *p, float a, float b, float c, f
     gf32x4 v = {a, b, c, d};
     *(__attribute__((address_space(1))) gf32x4 *)p = v;
 }
+typedef float gf32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void gstore_f2(float *p, float a, float b)
+{
+    gf32x2 v = {a, b};
+    *(__attribute__((address_space(1))) gf32x2 *)p = v;
+}
 __device__ __forceinline__ c32 cadd(c32 a, c32 b) { return {a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ c32 csub(c32 a, c32 b) { return {a.x - b.x, a.y - b.y}; }
 
@@ -2231,8 +2237,10 @@ __device__ __forceinline__ float f4get(const float4 &v, int e) { return e == 0 ?
 
 // lpb: lines per workgroup (at most LINES), chosen by the launcher so that
 // the workgroups fill whole rounds of the resident slots.
+// The 4-plane (Transform-3D) form is held to 3 waves per SIMD (168 VGPRs,
+// two of them spilled): left free it takes 187 and runs at 2 waves, 3% slower.
 template <int NC, bool RAW, int LINES>
-__global__ void __launch_bounds__(kDemodThreads)
+__global__ void __launch_bounds__(kDemodThreads, NC == 4 ? 3 : 1)
 pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__restrict__ sine,
                 const float *__restrict__ cosine, const float *__restrict__ cfilt, int n_lblocks, int lpb, float *raw)
 {
@@ -2509,20 +2517,53 @@ pal_demod_quad4(const Geom g, const FrameJob *__restrict__ jobs, const float *__
             const int64_t obase = (int64_t)(frame_line - g.first + g.pad_top) * g.out_w + g.pad_left + c0;
             // taps with e + b of parity sel sit on sine columns (m terms: P01, Q23)
             const int ps = sel, pc = sel ^ 1;
+            // yuv444ps pixel pairs on 8-byte boundaries (output buffer, this
+            // line's offset and the plane size all even in floats) take one
+            // 8-byte store per plane.  (Not in the 1-plane and composite
+            // forms: there the second path costs them the fourth wave per
+            // SIMD.)
+            const bool pairs = !RAW && NC >= 2 && os.fmt == 1 /*YUV444PS*/ && c0 + 4 <= g.aw &&
+                               ((obase | os.plane | (int64_t)(reinterpret_cast<uintptr_t>(job.out) >> 2)) & 1) == 0 &&
+                               (reinterpret_cast<uintptr_t>(job.out) & 3) == 0;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float P01 = A01[ps][e], Q01 = A01[pc][e], Q23 = A23[ps][e], P23 = A23[pc][e];
-                const float PU = P01 + P23, PV = P01 - P23, QU = Q01 - Q23, QV = Q01 + Q23;
-                // PAL_2D luma: composite minus (PY sine + QY cosine) * 2, and
-                // column e is itself a sine column when (e & 1) == sel
-                float Y;
-                if constexpr (pal2d) Y = compv[e] - ((e & 1) == sel ? AY[ps][e] : AY[pc][e]) * w[e] * 2.0f;
-                else Y = compv[e] - f4get(r[3], e);
-                const float U = -(PU * bp + QU * bq) * 2.0f;
-                const float V = vsw * -(QV * bp - PV * bq) * 2.0f;
-                if (c0 + e < g.aw) {
-                    if constexpr (RAW) emit_pixel<true>(g, os, job, raw, job_id, frame_line, c0 + e, Y, U, V);
-                    else store_pixel(g, os, job.out, obase + e, Y, U, V);
+            for (int e2 = 0; e2 < 4; e2 += 2) {
+                float Yv[2], Uv[2], Vv[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int e = e2 + h;
+                    const float P01 = A01[ps][e], Q01 = A01[pc][e], Q23 = A23[ps][e], P23 = A23[pc][e];
+                    const float PU = P01 + P23, PV = P01 - P23, QU = Q01 - Q23, QV = Q01 + Q23;
+                    // PAL_2D luma: composite minus (PY sine + QY cosine) * 2, and
+                    // column e is itself a sine column when (e & 1) == sel
+                    float Y;
+                    if constexpr (pal2d) Y = compv[e] - ((e & 1) == sel ? AY[ps][e] : AY[pc][e]) * w[e] * 2.0f;
+                    else Y = compv[e] - f4get(r[3], e);
+                    Yv[h] = Y;
+                    Uv[h] = -(PU * bp + QU * bq) * 2.0f;
+                    Vv[h] = vsw * -(QV * bp - PV * bq) * 2.0f;
+                }
+                if (pairs) {
+                    // the values store_pixel forms
+                    float ey[2], cb[2], cr[2];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        ey[h] = fminf(fmaxf((Yv[h] - g.black) * g.ey_scale, os.eyLo), os.eyHi);
+                        cb[h] = fminf(fmaxf(Uv[h] * g.ecb_scale, os.ecLo), os.ecHi);
+                        cr[h] = fminf(fmaxf(Vv[h] * g.ecr_scale, os.ecLo), os.ecHi);
+                    }
+                    float *out = static_cast<float *>(job.out) + obase + e2;
+                    gstore_f2(out, ey[0], ey[1]);
+                    gstore_f2(out + os.plane, cb[0], cb[1]);
+                    gstore_f2(out + 2 * os.plane, cr[0], cr[1]);
+                } else {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int e = e2 + h;
+                        if (c0 + e < g.aw) {
+                            if constexpr (RAW) emit_pixel<true>(g, os, job, raw, job_id, frame_line, c0 + e, Yv[h], Uv[h], Vv[h]);
+                            else store_pixel(g, os, job.out, obase + e, Yv[h], Uv[h], Vv[h]);
+                        }
+                    }
                 }
             }
         }
