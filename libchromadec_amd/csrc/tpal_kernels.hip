@@ -1441,10 +1441,20 @@ tpal3d_tiles(const Geom g, const SlabJob *__restrict__ jobs, int n_jobs, int str
 constexpr int kA2pad = 17;
 struct Lds2 {
     c32 A[4][9][kA2pad];      // per group: (k, line) after x DFT / before x inverse
-    uint8_t keep[4][9][kA2pad];
+    uint32_t kw[4][9];        // per group: keep word of bin k, bit = line
     uint16_t flag[4][80];
     int nflag[4];
 };
+
+// Bit y of each 16-bit group of w moved to bit (24 - y) % 16: the 2D mask's
+// mirror row, for the four groups of a wave at once.
+__device__ __forceinline__ uint64_t mirror16x4(uint64_t w)
+{
+    w = __builtin_bitreverse64(w);   // group q to 3 - q, bit y to 15 - y
+    w = (w >> 32) | (w << 32);
+    w = ((w >> 16) & 0x0000FFFF0000FFFFull) | ((w & 0x0000FFFF0000FFFFull) << 16);   // groups back in place
+    return ((w << 9) & 0xFE00FE00FE00FE00ull) | ((w >> 7) & 0x01FF01FF01FF01FFull);   // 15 - y to (24 - y) % 16
+}
 
 __device__ __forceinline__ void load32(const uint16_t *p, float black, float (&s)[32])
 {
@@ -1645,7 +1655,6 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
         const float err = kErrK2 * kU * group_sum16(l1 * rw);
 #pragma unroll
         for (int k = 0; k < 9; ++k) L.A[q][k][ln] = c32{X[k].x * rw, X[k].y * rw};
-        if (ln == 0) L.nflag[q] = 0;
         __syncthreads();
 
         // ---- vertical 16-point DFT: lanes 0..8 of each group ---------------
@@ -1657,25 +1666,48 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
 #pragma unroll
             for (int m = 0; m < 16; ++m) L.A[q][ln][m] = v[m];
         }
-        for (int i = ln; i < 9 * kA2pad; i += 16) (&L.keep[q][0][0])[i] = 0;
         __syncthreads();
 
         // ---- mask: 80 visits per group (transform_pal_2d.cpp applyFilter) --
-        for (int v = ln; v < 80; v += 16) {
-            const int y = v / 5, x = 4 + (v - 5 * y);
-            const int yr = (24 - y) & 15, xr = 16 - x;
-            if (x == xr && y == yr) {
-                L.keep[q][x - 4][y] = 1;
-                continue;
+        // Step s decides visit (x, y) = (4 + s, ln) of every group against its
+        // partner (16 - x, (24 - y) % 16), so the wave's keep ballot holds the
+        // four groups' keep words of bin x (bit y), and moved to the mirror
+        // rows those of bin 16 - x: a kept visit keeps both bins.  The
+        // self-mirror bins (x = 8, y = 4, 12) are kept; a tile whose windowed
+        // samples are all zero keeps every pair.  Unsure comparisons go to
+        // the group's FP64 list (visit y * 5 + x - 4) in lane order.
+        {
+            const uint64_t zero = __builtin_amdgcn_ballot_w64(err == 0.0f);
+            constexpr uint64_t kSelf2 = 0x1010101010101010ull;
+            const DecideConst qu = decide_const(err, g.thr_uniform);
+            const int yr = (24 - ln) & 15;
+            uint64_t K[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) K[k] = 0;
+            int nfl = 0;
+#pragma unroll
+            for (int st = 0; st < 5; ++st) {
+                const int x = 4 + st, xr = 16 - x;
+                const c32 a = L.A[q][x - 4][ln], b = L.A[q][xr - 4][yr];
+                const DecideConst dq = g.thr_uniform >= 0.0f ? qu : decide_const(err, thr32[ln * 5 + st]);
+                uint64_t kb, fb;
+                decide32_ballots(a, b, dq, kb, fb);
+                kb |= zero;
+                fb &= ~zero;
+                if (x == 8) {
+                    kb |= kSelf2;
+                    fb &= ~kSelf2;
+                }
+                K[x - 4] |= kb;
+                K[xr - 4] |= mirror16x4(kb);
+                const uint32_t fq = (uint32_t)(fb >> (16 * q)) & 0xFFFFu;
+                if ((fq >> ln) & 1u) L.flag[q][nfl + __popc(fq & ((1u << ln) - 1u))] = (uint16_t)(ln * 5 + st);
+                nfl += __popc(fq);
             }
-            const int d = decide32(L.A[q][x - 4][y], L.A[q][xr - 4][yr], err,
-                                   g.thr_uniform >= 0.0f ? g.thr_uniform : thr32[v]);
-            if (d > 0) {
-                L.keep[q][x - 4][y] = 1;
-                L.keep[q][xr - 4][yr] = 1;
-            } else if (d < 0) {
-                const int slot = atomicAdd(&L.nflag[q], 1);
-                L.flag[q][slot] = (uint16_t)v;
+            if (ln == 0) {
+                L.nflag[q] = nfl;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) L.kw[q][k] = (uint32_t)(K[k] >> (16 * q)) & 0xFFFFu;
             }
         }
         __syncthreads();
@@ -1717,8 +1749,8 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
             }
             group_sum16_4d(a0, a1, b0, b1);
             if (mine && ln == 0 && decide64(a0, a1, b0, b1, thr64[v])) {
-                L.keep[q][x - 4][y] = 1;
-                L.keep[q][xr - 4][yr] = 1;
+                L.kw[q][x - 4] |= 1u << y;
+                L.kw[q][xr - 4] |= 1u << yr;
             }
         }
         __syncthreads();
@@ -1726,8 +1758,9 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
         // ---- masked inverse vertical ----------------------------------------
         if (ln < 9) {
             c32 v[16];
+            const uint32_t kwl = L.kw[q][ln];
 #pragma unroll
-            for (int m = 0; m < 16; ++m) v[m] = L.keep[q][ln][m] ? L.A[q][ln][m] : c32{0.0f, 0.0f};
+            for (int m = 0; m < 16; ++m) v[m] = ((kwl >> m) & 1u) ? L.A[q][ln][m] : c32{0.0f, 0.0f};
             fft<16, +1>(v);
 #pragma unroll
             for (int m = 0; m < 16; ++m) L.A[q][ln][m] = v[m];
