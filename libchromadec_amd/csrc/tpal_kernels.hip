@@ -168,34 +168,6 @@ constexpr float kErrK = 8.0f;    // 3D tile
 constexpr float kErrK2 = 12.0f;  // 2D tile (32-sample x DFT)
 constexpr float kU = 5.9604645e-08f;  // 2^-24
 
-// Keep/discard decision with an FP32 uncertainty test
-// (transform_pal_3d.cpp:313-327: discard when |a|^2 < |b|^2 t^2 or
-// |b|^2 < |a|^2 t^2).  Returns 1 keep, 0 discard, -1 when FP32 error `e` on
-// the bins could change the outcome.  |m_hat - m| <= 2|a|e + e^2, plus the
-// rounding of the products themselves.  The callers' bound e is zero only
-// for a tile whose windowed samples are all zero (a 2D padding step past its
-// strip, or a tile of black): every bin is then exactly zero and the pair is
-// kept (0 < 0 is false), as an FP64 evaluation decides.
-__device__ __forceinline__ void decide32_nz(c32 a, c32 b, float e, float th, bool &keep, bool &unsure)
-{
-    const float ma = a.x * a.x + a.y * a.y;
-    const float mb = b.x * b.x + b.y * b.y;
-    // |a| <= max(|a.x|,|a.y|) + (sqrt 2 - 1) min(|a.x|,|a.y|)
-    const float ax = fabsf(a.x), ay = fabsf(a.y), bx = fabsf(b.x), by = fabsf(b.y);
-    const float na = fmaf(0.41421357f, fminf(ax, ay), fmaxf(ax, ay));
-    const float nb = fmaf(0.41421357f, fminf(bx, by), fmaxf(bx, by));
-    const float da = e * (2.0f * na + e);
-    const float db = e * (2.0f * nb + e);
-    const float g1 = ma - mb * th, g2 = mb - ma * th;
-    const float r1 = 4.0f * kU * (ma + mb * th), r2 = 4.0f * kU * (mb + ma * th);
-    const float d1 = fmaf(db, th, da) + r1;
-    const float d2 = fmaf(da, th, db) + r2;
-    // bitwise ors: no short-circuit branches in the unrolled mask loop
-    unsure = (fabsf(g1) <= d1) | (fabsf(g2) <= d2);
-    const bool discard = (g1 < 0.0f) | (g2 < 0.0f);
-    keep = !unsure & !discard;
-}
-
 // max / min of |x| and |y| in one instruction each.  The compiler's fmaxf
 // first canonicalises each input (IEEE mode), which costs two more; the bins
 // here are never NaN, so the canonicalisation changes nothing.
@@ -212,7 +184,16 @@ __device__ __forceinline__ float absmin(float x, float y)
     return r;
 }
 
-// The terms of decide32_nz's bounds that depend only on the tile's e and the
+// Keep/discard decision with an FP32 uncertainty test
+// (transform_pal_3d.cpp:313-327: discard when |a|^2 < |b|^2 t^2 or
+// |b|^2 < |a|^2 t^2): keep, discard, or unsure when FP32 error `e` on the
+// bins could change the outcome.  |m_hat - m| <= 2|a|e + e^2, plus the
+// rounding of the products themselves.  The callers' bound e is zero only
+// for a tile whose windowed samples are all zero (a 2D padding step past its
+// strip, or a tile of black): every bin is then exactly zero and the pair is
+// kept (0 < 0 is false), as an FP64 evaluation decides; the callers keep such
+// tiles' pairs without the test.
+// The terms of the bounds that depend only on the tile's e and the
 // threshold th: d1 = 2e na + 2e th nb + e^2 (1 + th) + 4u ma + 4u th mb, and
 // d2 the same with a and b swapped.
 struct DecideConst {
@@ -223,8 +204,8 @@ __device__ __forceinline__ DecideConst decide_const(float e, float th)
     return DecideConst{th, 2.0f * e, 2.0f * e * th, e * e * (1.0f + th), 4.0f * kU * th};
 }
 
-// decide32_nz as wave ballots (taken on the comparisons themselves, so that no
-// per-lane bool is materialised): keep and unsure lane masks.
+// The decision as wave ballots (taken on the comparisons themselves, so that
+// no per-lane bool is materialised): keep and unsure lane masks.
 __device__ __forceinline__ void decide32_ballots(c32 a, c32 b, const DecideConst &q, uint64_t &keep,
                                                  uint64_t &unsure)
 {
@@ -240,16 +221,6 @@ __device__ __forceinline__ void decide32_ballots(c32 a, c32 b, const DecideConst
     unsure = __builtin_amdgcn_ballot_w64(fabsf(g1) <= d1) | __builtin_amdgcn_ballot_w64(fabsf(g2) <= d2);
     const uint64_t discard = __builtin_amdgcn_ballot_w64(fminf(g1, g2) < 0.0f);
     keep = __builtin_amdgcn_read_exec() & ~(unsure | discard);
-}
-
-// decide32_nz as one value: 1 keep, 0 discard, -1 unsure; a tile with e = 0
-// keeps every pair.
-__device__ __forceinline__ int decide32(c32 a, c32 b, float e, float th)
-{
-    if (e == 0.0f) return 1;
-    bool keep, unsure;
-    decide32_nz(a, b, e, th, keep, unsure);
-    return unsure ? -1 : (keep ? 1 : 0);
 }
 
 // cos / sin (2 pi j / 16) with the exact values at multiples of pi/2.
