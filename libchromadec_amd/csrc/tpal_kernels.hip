@@ -1752,10 +1752,16 @@ tpal2d_tiles(const Geom g, const FieldJob2D *__restrict__ jobs, int n_ty_max,
             const int c0 = tileX - g.avs + kColOrigin;
             float *dst = orow + c0;
             float v16[16];
+            // interior tiles (all 16 columns active) skip the per-column test
+            if (tileX >= g.avs && tileX + 16 <= g.ave) {
 #pragma unroll
-            for (int n = 0; n < 16; ++n) {
-                const int col = tileX + n;
-                v16[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
+                for (int n = 0; n < 16; ++n) v16[n] = carry[n] + out[n];
+            } else {
+#pragma unroll
+                for (int n = 0; n < 16; ++n) {
+                    const int col = tileX + n;
+                    v16[n] = (col >= g.avs && col < g.ave) ? carry[n] + out[n] : 0.0f;
+                }
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) gstore_f4(dst + 4 * i, v16[4 * i], v16[4 * i + 1], v16[4 * i + 2], v16[4 * i + 3]);
